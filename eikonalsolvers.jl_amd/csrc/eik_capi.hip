@@ -1,0 +1,302 @@
+// eik_capi.hip -- C ABI of libeik (declared in include/eik.h) and its host-side geometry.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "eik_internal.h"
+
+namespace eik {
+
+void make_inst(const eik_grid& g, InstDev& I) {
+    memset(&I, 0, sizeof(I));
+    for (int d = 0; d < 3; d++) {
+        I.n[d] = d < g.ndim ? (int32_t)g.n[d] : 1;
+        I.h[d] = g.d[d];
+        I.o[d] = d < g.ndim ? g.o[d] : 0.0;
+    }
+    if (g.kind == EIK_GRID2D_CART || g.kind == EIK_GRID3D_CART) I.h[1] = I.h[2] = g.d[0];
+    I.kind = g.kind;
+    I.dthrad = deg2rad(g.d[1]);
+    I.dphrad = deg2rad(g.d[2]);
+}
+
+static inline double gcoord(const eik_grid& g, int d, int64_t i) {
+    const double h = (g.kind == EIK_GRID2D_CART || g.kind == EIK_GRID3D_CART) ? g.d[0] : g.d[d];
+    return fma((double)i, h, g.o[d]);
+}
+
+// findenclosingbox (src/EikSolv/eikmainfwd.jl:1114-1186): 1 corner when the source sits on a node
+// (remainder of every coordinate within 1e-8), else the 2^ndim corners of its cell in the
+// reference's order, shifted back by one at the upper edge.
+int find_enclosing_box(const eik_grid& g, const double* src, int corn[8][3]) {
+    const int nd = g.ndim;
+    int ijk[3] = {0, 0, 0};
+    bool onnode = true;
+    for (int d = 0; d < nd; d++) {
+        const double h = (g.kind == EIK_GRID2D_CART || g.kind == EIK_GRID3D_CART) ? g.d[0] : g.d[d];
+        const double x = src[d] - g.o[d];
+        const double r = std::fmod(x, h);
+        const double q = std::nearbyint((x - r) / h);
+        ijk[d] = (int)std::floor(q);
+        if (!(std::fabs(r) <= 1e-8)) onnode = false;
+    }
+    if (onnode) {
+        for (int d = 0; d < 3; d++) corn[0][d] = d < nd ? ijk[d] : 0;
+        return 1;
+    }
+    for (int d = 0; d < nd; d++)
+        if (ijk[d] == g.n[d] - 1) ijk[d] -= 1;
+    static const int off2[4][2] = {{0, 0}, {1, 0}, {0, 1}, {1, 1}};
+    static const int off3[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1},
+                                   {1, 1, 0}, {0, 1, 1}, {1, 0, 1}, {1, 1, 1}};
+    const int nc = 1 << nd;
+    for (int c = 0; c < nc; c++)
+        for (int d = 0; d < 3; d++) corn[c][d] = d < nd ? ijk[d] + (nd == 2 ? off2[c][d] : off3[c][d]) : 0;
+    return nc;
+}
+
+// createfinegrid (src/EikSolv/eikmainfwd.jl:950-1104): window = enclosing box +- noderadius
+// coarse nodes clipped to the model, fine spacing h/df, fine size (delta ijk)*df+1.
+void build_fine(const eik_grid& g, const double* src, int df, int noderadius, FineDev& F, InstDev& I) {
+    const int nd = g.ndim;
+    memset(&F, 0, sizeof(F));
+    int corn[8][3];
+    const int nc = find_enclosing_box(g, src, corn);
+    int i1[3] = {0, 0, 0}, i2[3] = {0, 0, 0};
+    for (int d = 0; d < nd; d++) {
+        int mn = corn[0][d], mx = corn[0][d];
+        for (int c = 1; c < nc; c++) {
+            mn = std::min(mn, corn[c][d]);
+            mx = std::max(mx, corn[c][d]);
+        }
+        const int v1 = mn - noderadius, v2 = mx + noderadius;
+        F.outmin[d] = v1 < 0;
+        F.outmax[d] = v2 > g.n[d] - 1;
+        i1[d] = F.outmin[d] ? 0 : v1;
+        i2[d] = F.outmax[d] ? (int)g.n[d] - 1 : v2;
+    }
+    F.df = df;
+    eik_grid gf{};
+    gf.kind = g.kind;
+    gf.ndim = nd;
+    for (int d = 0; d < 3; d++) {
+        F.i1c[d] = d < nd ? i1[d] : 0;
+        F.nc[d] = d < nd ? (int32_t)g.n[d] : 1;
+        gf.n[d] = d < nd ? (int64_t)(i2[d] - i1[d]) * df + 1 : 1;
+    }
+    const bool cart = (g.kind == EIK_GRID2D_CART || g.kind == EIK_GRID3D_CART);
+    for (int d = 0; d < 3; d++) gf.d[d] = (cart ? g.d[0] : g.d[d]) / df;
+    for (int d = 0; d < nd; d++) gf.o[d] = gcoord(g, d, i1[d]);
+    make_inst(gf, I);
+    const int TX = nd == 2 ? 16 : 8, TZ = nd == 2 ? 1 : 8;
+    I.t[0] = (I.n[0] + TX - 1) / TX;
+    I.t[1] = (I.n[1] + TX - 1) / TX;
+    I.t[2] = nd == 3 ? (I.n[2] + TZ - 1) / TZ : 1;
+    F.ncorn = find_enclosing_box(gf, src, F.corn);
+}
+
+}  // namespace eik
+
+using namespace eik;
+
+extern "C" {
+
+const char* eik_version(void) { return "libeik 0.1 (gfx950)"; }
+
+const char* eik_last_error(const eik_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int eik_ctx_create(int32_t device, eik_ctx** out) {
+    if (!out) return EIK_EINVAL;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return EIK_EDEVICE;
+    if (device < 0 || device >= ndev) return EIK_EINVAL;
+    eik_ctx* ctx = new eik_ctx();
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc(&ctx->d_small, 64) != hipSuccess || hipHostMalloc(&ctx->h_small, 64, hipHostMallocDefault) != hipSuccess ||
+        hipEventCreate(&ctx->ev[0]) != hipSuccess || hipEventCreate(&ctx->ev[1]) != hipSuccess) {
+        delete ctx;
+        return EIK_EDEVICE;
+    }
+    *out = ctx;
+    return EIK_OK;
+}
+
+void eik_ctx_destroy(eik_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto& kv : ctx->bufs) kv.second.release();
+    if (ctx->d_small) (void)hipFree(ctx->d_small);
+    if (ctx->h_small) (void)hipHostFree(ctx->h_small);
+    for (int i = 0; i < 2; i++)
+        if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+static int check_grid(eik_ctx* ctx, const eik_grid* g) {
+    if (!g) return ctx->fail(EIK_EINVAL, "null grid");
+    if (g->ndim < 2 || g->ndim > 3) return ctx->fail(EIK_EASSERT, "2<=ndims(vel)<=3");
+    const bool k2 = g->kind == EIK_GRID2D_CART || g->kind == EIK_GRID2D_SPHERE;
+    const bool k3 = g->kind == EIK_GRID3D_CART || g->kind == EIK_GRID3D_SPHERE;
+    if (!(k2 || k3)) return ctx->fail(EIK_EINVAL, "unknown grid kind %d", g->kind);
+    if ((k2 && g->ndim != 2) || (k3 && g->ndim != 3)) return ctx->fail(EIK_EASSERT, "Ndim==length(grd.grsize)");
+    for (int d = 0; d < g->ndim; d++) {
+        if (g->n[d] < 1) return ctx->fail(EIK_EASSERT, "all(grsize.>0)");
+        if (g->n[d] > (int64_t)1 << 30) return ctx->fail(EIK_EINVAL, "grid too large");
+        if (!(g->d[d] > 0.0) && !(d > 0 && (g->kind == EIK_GRID2D_CART || g->kind == EIK_GRID3D_CART)))
+            return ctx->fail(EIK_EASSERT, "grid spacing must be > 0");
+    }
+    return EIK_OK;
+}
+
+static int update_vel_locked(eik_model* M, const double* vel) {
+    eik_ctx* ctx = M->ctx;
+    for (int64_t i = 0; i < M->N; i++)
+        if (!(vel[i] > 0.0)) return ctx->fail(EIK_EASSERT, "all(vel.>0.0)");
+    (void)hipSetDevice(ctx->device);
+    CK(hipMemcpyAsync(M->d_vel, vel, M->N * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    if (model_slowness(M)) return ctx->fail(EIK_EDEVICE, "slowness kernel failed");
+    CK(hipStreamSynchronize(ctx->stream));
+    return EIK_OK;
+}
+
+int eik_model_create(eik_ctx* ctx, const eik_grid* grid, const double* vel, eik_model** out) {
+    if (!ctx || !out || !vel) return EIK_EINVAL;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    *out = nullptr;
+    int rc = check_grid(ctx, grid);
+    if (rc) return rc;
+    eik_model* M = new eik_model();
+    M->ctx = ctx;
+    M->g = *grid;
+    if (grid->kind == EIK_GRID2D_CART || grid->kind == EIK_GRID3D_CART) M->g.d[1] = M->g.d[2] = grid->d[0];
+    M->N = 1;
+    for (int d = 0; d < grid->ndim; d++) M->N *= grid->n[d];
+    for (int d = grid->ndim; d < 3; d++) M->g.n[d] = 1;
+    (void)hipSetDevice(ctx->device);
+    if (hipMalloc(&M->d_vel, M->N * sizeof(double)) != hipSuccess ||
+        hipMalloc(&M->d_slow64, M->N * sizeof(double)) != hipSuccess ||
+        hipMalloc(&M->d_slow32, M->N * sizeof(float)) != hipSuccess) {
+        eik_model_destroy(M);
+        return ctx->fail(EIK_ENOMEM, "out of device memory (model)");
+    }
+    rc = update_vel_locked(M, vel);
+    if (rc) {
+        eik_model_destroy(M);
+        return rc;
+    }
+    *out = M;
+    return EIK_OK;
+}
+
+int eik_model_update_vel(eik_model* M, const double* vel) {
+    if (!M || !vel) return EIK_EINVAL;
+    std::lock_guard<std::mutex> lk(M->ctx->mu);
+    return update_vel_locked(M, vel);
+}
+
+void eik_model_destroy(eik_model* M) {
+    if (!M) return;
+    if (M->ctx) (void)hipSetDevice(M->ctx->device);
+    if (M->d_vel) (void)hipFree(M->d_vel);
+    if (M->d_slow64) (void)hipFree(M->d_slow64);
+    if (M->d_slow32) (void)hipFree(M->d_slow32);
+    delete M;
+}
+
+// checksrcrecposition (src/EikSolv/eikchecks.jl:4-50) and ExtraParams checks (eikstructs.jl:270-277)
+static int load_sources(eik_ctx* ctx, eik_model* M, int64_t nsrc, const double* coordsrc, const int64_t* rec_off,
+                        const double* coordrec, const eik_params* P, SourceSet& S) {
+    const int nd = M->g.ndim;
+    if (nsrc < 0 || (nsrc > 0 && (!coordsrc || !rec_off))) return ctx->fail(EIK_EINVAL, "bad source arguments");
+    if (!P) return ctx->fail(EIK_EINVAL, "null params");
+    if (P->radiussmoothgradsrc < 0) return ctx->fail(EIK_EASSERT, "radiussmoothgradsrc>= 0");
+    if (P->smoothgradkern < 0) return ctx->fail(EIK_EASSERT, "smoothgradkern>=0");
+    if (P->smoothgradkern > 0 && P->smoothgradkern % 2 == 0)
+        return ctx->fail(EIK_EINVAL, "ExtraParams: smoothgradkern must be an odd number");
+    if (P->precision != EIK_PREC_FP64 && P->precision != EIK_PREC_MIXED) return ctx->fail(EIK_EINVAL, "bad precision");
+    if (P->refinearoundsrc && (P->downscalefactor < 1 || P->noderadius < 0))
+        return ctx->fail(EIK_EINVAL, "bad grid refinement parameters");
+    if (P->refinearoundsrc && M->g.kind == EIK_GRID3D_SPHERE)
+        return ctx->fail(EIK_EINVAL, "createfinegrid(): spherical coordinates in 3D still work in progress...");
+    S.nsrc = nsrc;
+    S.nd = nd;
+    S.src.assign(coordsrc, coordsrc + nsrc * nd);
+    S.rec_off.assign(rec_off, rec_off + nsrc + 1);
+    if (nsrc > 0 && S.rec_off[0] != 0) return ctx->fail(EIK_EINVAL, "rec_off[0] must be 0");
+    for (int64_t s = 0; s < nsrc; s++)
+        if (S.rec_off[s + 1] < S.rec_off[s]) return ctx->fail(EIK_EINVAL, "rec_off must be non-decreasing");
+    const int64_t nrec = S.nrec();
+    if (nrec > 0 && !coordrec) return ctx->fail(EIK_EINVAL, "null receivers");
+    S.rec.assign(coordrec, coordrec + nrec * nd);
+    for (int d = 0; d < nd; d++) {
+        const double lo = gcoord(M->g, d, 0), hi = gcoord(M->g, d, M->g.n[d] - 1);
+        for (int64_t s = 0; s < nsrc; s++) {
+            const double x = S.src[s * nd + d];
+            if (!(lo <= x && x <= hi)) return ctx->fail(EIK_EASSERT, "source %lld outside the grid", (long long)s);
+        }
+        for (int64_t r = 0; r < nrec; r++) {
+            const double x = S.rec[r * nd + d];
+            if (!(lo <= x && x <= hi)) return ctx->fail(EIK_EASSERT, "receiver %lld outside the grid", (long long)r);
+        }
+    }
+    return EIK_OK;
+}
+
+int eik_traveltime(eik_ctx* ctx, eik_model* model, int64_t nsrc, const double* coordsrc, const int64_t* rec_off,
+                   const double* coordrec, const eik_params* params, double* ttpicks, double* ttgrid) {
+    if (!ctx || !model) return EIK_EINVAL;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    SourceSet S;
+    int rc = load_sources(ctx, model, nsrc, coordsrc, rec_off, coordrec, params, S);
+    if (rc) return rc;
+    if (S.nrec() > 0 && !ttpicks) return ctx->fail(EIK_EINVAL, "null picks buffer");
+    if (nsrc == 0) return EIK_OK;
+    for (double& x : ctx->stats) x = 0.0;
+    Timer total(ctx);
+    rc = forward_solve(ctx, model, S, params->precision, params);
+    if (rc) return rc;
+    rc = forward_picks(ctx, model, S, params->precision, ttpicks, ttgrid);
+    ctx->stats[0] = total.mark();
+    return rc;
+}
+
+int eik_misfit(eik_ctx* ctx, eik_model* model, int64_t nsrc, const double* coordsrc, const int64_t* rec_off,
+               const double* coordrec, const double* pickobs, const double* stdobs, const eik_params* params,
+               double* misf) {
+    if (!ctx || !model || !misf) return EIK_EINVAL;
+    const int64_t nrec = nsrc > 0 && rec_off ? rec_off[nsrc] : 0;
+    std::vector<double> picks((size_t)std::max<int64_t>(nrec, 1));
+    int rc = eik_traveltime(ctx, model, nsrc, coordsrc, rec_off, coordrec, params, picks.data(), nullptr);
+    if (rc) return rc;
+    double m = 0.0;
+    for (int64_t s = 0; s < nsrc; s++) {
+        double ms = 0.0;
+        for (int64_t r = rec_off[s]; r < rec_off[s + 1]; r++) {
+            const double dd = picks[r] - pickobs[r];
+            ms += (dd * dd) / (stdobs[r] * stdobs[r]);
+        }
+        m += ms;
+    }
+    *misf = 0.5 * m;
+    return EIK_OK;
+}
+
+int eik_gradient(eik_ctx* ctx, eik_model* model, int64_t nsrc, const double* coordsrc, const int64_t* rec_off,
+                 const double* coordrec, const double* pickobs, const double* stdobs, int32_t whichgrad,
+                 const eik_params* params, double* gradvel, double* gradsrc, double* misf) {
+    if (!ctx) return EIK_EINVAL;
+    return ctx->fail(EIK_EINVAL, "eik_gradient: not available in this build");
+}
+
+int eik_last_stats(const eik_ctx* ctx, double* out, int32_t nout) {
+    if (!ctx || !out) return EIK_EINVAL;
+    for (int i = 0; i < nout && i < 8; i++) out[i] = ctx->stats[i];
+    return EIK_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,652 @@
+// eik_fim.hip -- forward traveltime engine of libeik (gfx950).
+//
+// Tiled fast-iterative solver whose local update replays the reference's heap order (see
+// eik_stencil.h), batched over many sources and over the per-source refined grids in one launch
+// per round.  Layout in HBM:
+//   model : vel (FP64), slowness (FP64 and FP32), N = prod(grsize) each, index 1 fastest;
+//   pools : one traveltime grid per source instance (FP64, or FP32 in mixed precision), and one
+//           FP64 traveltime + FP64 slowness grid per refined near-source box.
+// Init (source) nodes are stored with the sign bit set (value -t); they count as accepted from
+// the start and never change.  Unreached nodes hold +inf.
+#include "eik_internal.h"
+
+namespace eik {
+
+// ================================================================= tile geometry
+template <int ND>
+struct TileShape;
+template <>
+struct TileShape<2> {
+    static constexpr int TX = 16, TY = 16, TZ = 1, H = 2;
+    static constexpr int LX = TX + 2 * H, LY = TY + 2 * H, LZ = 1;
+    static constexpr int NODES_PER_THREAD = 1;
+    static constexpr int MAXIT = 96;
+};
+template <>
+struct TileShape<3> {
+    static constexpr int TX = 8, TY = 8, TZ = 8, H = 2;
+    static constexpr int LX = TX + 2 * H, LY = TY + 2 * H, LZ = TZ + 2 * H;
+    static constexpr int NODES_PER_THREAD = 2;
+    static constexpr int MAXIT = 64;
+};
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ int find_inst(const InstDev* __restrict__ inst, int ninst, int64_t gid) {
+    int lo = 0, hi = ninst - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (inst[mid].tile_base <= gid)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+template <typename T>
+__device__ __forceinline__ bool is_fixed(T raw) {
+    return signbit(raw);
+}
+
+// one round: every listed tile is relaxed to local convergence in LDS
+template <int ND, typename T>
+__global__ void __launch_bounds__(kBlock) k_fim_round(const int64_t* __restrict__ list,
+                                                      const InstDev* __restrict__ inst, int ninst,
+                                                      T* __restrict__ ttpool, const T* __restrict__ slowpool,
+                                                      uint8_t* __restrict__ flags, unsigned* __restrict__ visits) {
+    using S = TileShape<ND>;
+    __shared__ T lds[S::LX * S::LY * S::LZ];
+    __shared__ int s_faces;
+    const int tid = threadIdx.x;
+    const int64_t gid = list[blockIdx.x];
+    const int ii = find_inst(inst, ninst, gid);
+    const InstDev I = inst[ii];
+    const int64_t tl = gid - I.tile_base;
+    const int tix = (int)(tl % I.t[0]);
+    const int tiy = (int)((tl / I.t[0]) % I.t[1]);
+    const int tiz = (ND == 3) ? (int)(tl / ((int64_t)I.t[0] * I.t[1])) : 0;
+    const int ox = tix * S::TX, oy = tiy * S::TY, oz = tiz * S::TZ;
+    T* __restrict__ tt = ttpool + I.tt_off;
+    const T* __restrict__ slow = slowpool + I.s_off;
+    const int64_t nx = I.n[0], nxy = (int64_t)I.n[0] * I.n[1];
+
+    if (tid == 0) s_faces = 0;
+    // stage tile + face halos (edges/corners of the halo are never read by the axis stencil)
+    for (int c = tid; c < S::LX * S::LY * S::LZ; c += kBlock) {
+        const int lx = c % S::LX, ly = (c / S::LX) % S::LY, lz = (ND == 3) ? c / (S::LX * S::LY) : 0;
+        int nout = (lx < S::H || lx >= S::H + S::TX) + (ly < S::H || ly >= S::H + S::TY);
+        if (ND == 3) nout += (lz < S::H || lz >= S::H + S::TZ);
+        T v = tinf<T>();
+        if (nout <= 1) {
+            const int gx = ox + lx - S::H, gy = oy + ly - S::H, gz = (ND == 3) ? oz + lz - S::H : 0;
+            if (gx >= 0 && gx < I.n[0] && gy >= 0 && gy < I.n[1] && (ND == 2 || (gz >= 0 && gz < I.n[2])))
+                v = tt[gx + nx * gy + nxy * gz];
+        }
+        lds[c] = v;
+    }
+
+    // per-node static data
+    constexpr int NP = S::NODES_PER_THREAD;
+    int lidx[NP];
+    bool act[NP];
+    unsigned ok1[NP], ok2[NP];
+    T sl[NP], dh[NP][3], v0[NP];
+    int gxyz[NP][3];
+#pragma unroll
+    for (int p = 0; p < NP; p++) {
+        const int lx = tid % S::TX;
+        const int ly = (tid / S::TX) % S::TY;
+        const int lz = (ND == 3) ? (tid / (S::TX * S::TY)) + p * (S::TZ / NP) : 0;
+        const int gx = ox + lx, gy = oy + ly, gz = oz + lz;
+        gxyz[p][0] = gx;
+        gxyz[p][1] = gy;
+        gxyz[p][2] = gz;
+        lidx[p] = (lx + S::H) + S::LX * ((ly + S::H) + S::LY * (ND == 3 ? lz + S::H : 0));
+        const bool inside = gx < I.n[0] && gy < I.n[1] && (ND == 2 || gz < I.n[2]);
+        act[p] = inside;
+        ok1[p] = ok2[p] = 0u;
+        sl[p] = T(0);
+        if (inside) {
+            const int g[3] = {gx, gy, gz};
+#pragma unroll
+            for (int ax = 0; ax < ND; ax++) {
+#pragma unroll
+                for (int l = 0; l < 2; l++) {
+                    const int k = 2 * ax + l;
+                    int nb[3] = {g[0], g[1], g[2]};
+                    nb[ax] += (l == 0) ? 1 : -1;
+                    const bool in1 = nb[ax] >= 0 && nb[ax] <= I.n[ax] - 1;
+                    bool in2 = in1;
+#pragma unroll
+                    for (int d = 0; d < ND; d++) in2 = in2 && nb[d] >= 1 && nb[d] <= I.n[d] - 2;
+                    ok1[p] |= (unsigned)in1 << k;
+                    ok2[p] |= (unsigned)in2 << k;
+                }
+            }
+            sl[p] = slow[gx + nx * gy + nxy * gz];
+            node_spacing<T>(I, g, dh[p]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < NP; p++) {
+        v0[p] = lds[lidx[p]];
+        if (act[p] && is_fixed(v0[p])) act[p] = false;
+    }
+
+    constexpr int stx = 1, sty = S::LX, stz = S::LX * S::LY;
+    const int strides[3] = {stx, sty, stz};
+    int it = 0;
+    unsigned nvis = 0;
+    for (; it < S::MAXIT; it++) {
+        int any = 0;
+#pragma unroll
+        for (int p = 0; p < NP; p++) {
+            if (!act[p]) continue;
+            T v1[2 * ND], v2[2 * ND];
+            unsigned f1 = 0u, f2 = 0u;
+            const int li = lidx[p];
+#pragma unroll
+            for (int ax = 0; ax < ND; ax++) {
+#pragma unroll
+                for (int l = 0; l < 2; l++) {
+                    const int k = 2 * ax + l;
+                    const int sgn = (l == 0) ? 1 : -1;
+                    T a = tinf<T>(), b = tinf<T>();
+                    if ((ok1[p] >> k) & 1u) a = lds[li + sgn * strides[ax]];
+                    if ((ok2[p] >> k) & 1u) b = lds[li + 2 * sgn * strides[ax]];
+                    f1 |= (unsigned)is_fixed(a) << k;
+                    f2 |= (unsigned)is_fixed(b) << k;
+                    v1[k] = fabs(a);
+                    v2[k] = fabs(b);
+                }
+            }
+            int code[3];
+            const T nv = replay<ND, T>(v1, v2, f1, f2, ok1[p], ok2[p], sl[p], dh[p], code);
+            const T cur = lds[li];
+            if (nv == nv && nv != cur && !(nv >= tinf<T>() && cur >= tinf<T>())) {
+                lds[li] = nv;
+                any = 1;
+            }
+        }
+        nvis++;
+        if (!__syncthreads_or(any)) break;
+    }
+    const bool converged = it < S::MAXIT;
+
+    // write back changed nodes and collect the faces whose 2-deep layer changed
+    int faces = 0;
+#pragma unroll
+    for (int p = 0; p < NP; p++) {
+        if (!act[p]) continue;
+        const T v = lds[lidx[p]];
+        if (v != v0[p]) {
+            const int gx = gxyz[p][0], gy = gxyz[p][1], gz = gxyz[p][2];
+            tt[gx + nx * gy + nxy * gz] = v;
+            const int lx = gx - ox, ly = gy - oy, lz = gz - oz;
+            if (lx < 2) faces |= 1;
+            if (lx >= S::TX - 2) faces |= 2;
+            if (ly < 2) faces |= 4;
+            if (ly >= S::TY - 2) faces |= 8;
+            if (ND == 3) {
+                if (lz < 2) faces |= 16;
+                if (lz >= S::TZ - 2) faces |= 32;
+            }
+        }
+    }
+    if (faces) atomicOr(&s_faces, faces);
+    __syncthreads();
+    if (tid == 0) {
+        const int f = s_faces;
+        uint8_t* fl = flags + I.tile_base;
+        const int64_t t0 = I.t[0], t01 = (int64_t)I.t[0] * I.t[1];
+        if ((f & 1) && tix > 0) fl[tl - 1] = 1;
+        if ((f & 2) && tix < I.t[0] - 1) fl[tl + 1] = 1;
+        if ((f & 4) && tiy > 0) fl[tl - t0] = 1;
+        if ((f & 8) && tiy < I.t[1] - 1) fl[tl + t0] = 1;
+        if (ND == 3) {
+            if ((f & 16) && tiz > 0) fl[tl - t01] = 1;
+            if ((f & 32) && tiz < I.t[2] - 1) fl[tl + t01] = 1;
+        }
+        if (!converged) fl[tl] = 1;
+        if (visits) atomicAdd(visits, nvis);
+    }
+}
+
+// gather the set flags into a list (order irrelevant) and clear them
+__global__ void k_compact(uint8_t* __restrict__ flags, int64_t ntiles, int64_t* __restrict__ list,
+                          unsigned* __restrict__ count) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntiles) return;
+    if (flags[t]) {
+        flags[t] = 0;
+        const unsigned pos = atomicAdd(count, 1u);
+        list[pos] = t;
+    }
+}
+
+template <typename T>
+__global__ void k_fill(T* __restrict__ p, int64_t n, T v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_slowness(const double* __restrict__ vel, double* __restrict__ s64, float* __restrict__ s32,
+                           int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const double s = 1.0 / vel[i];
+        s64[i] = s;
+        s32[i] = (float)s;
+    }
+}
+
+// mark the tile of node g and the face-neighbour tiles whose stencil reaches it
+__device__ __forceinline__ void seed_tiles(const InstDev& I, int nd, const int* g, uint8_t* flags) {
+    const int TX = nd == 2 ? 16 : 8, TY = TX, TZ = nd == 2 ? 1 : 8;
+    const int tx = g[0] / TX, ty = g[1] / TY, tz = nd == 3 ? g[2] / TZ : 0;
+    const int64_t t0 = I.t[0], t01 = (int64_t)I.t[0] * I.t[1];
+    const int64_t tl = tx + t0 * ty + t01 * tz;
+    uint8_t* fl = flags + I.tile_base;
+    fl[tl] = 1;
+    const int lx = g[0] - tx * TX, ly = g[1] - ty * TY, lz = nd == 3 ? g[2] - tz * TZ : 0;
+    if (lx < 2 && tx > 0) fl[tl - 1] = 1;
+    if (lx >= TX - 2 && tx < I.t[0] - 1) fl[tl + 1] = 1;
+    if (ly < 2 && ty > 0) fl[tl - t0] = 1;
+    if (ly >= TY - 2 && ty < I.t[1] - 1) fl[tl + t0] = 1;
+    if (nd == 3) {
+        if (lz < 2 && tz > 0) fl[tl - t01] = 1;
+        if (lz >= TZ - 2 && tz < I.t[2] - 1) fl[tl + t01] = 1;
+    }
+}
+
+// Source-box corners: tt = dist(corner, src)/vel[corner] (sourceboxloctt!, eikmainfwd.jl:1195-1322).
+// One thread per (instance, corner).  Works on the coarse grid (no refinement) and on the fine
+// boxes (vel = the nearest-neighbour fine velocity, read through the box's map).
+template <typename T>
+__global__ void k_corners(const CornerDev* __restrict__ cs, int ncs, const InstDev* __restrict__ inst,
+                          T* __restrict__ ttpool, const double* __restrict__ velc, const FineDev* __restrict__ fine,
+                          uint8_t* __restrict__ flags, int nd, int kind) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= ncs) return;
+    const CornerDev C = cs[q];
+    const InstDev I = inst[C.inst];
+    const int g[3] = {C.ijk[0], C.ijk[1], C.ijk[2]};
+    const int64_t l = g[0] + (int64_t)I.n[0] * (g[1] + (int64_t)I.n[1] * g[2]);
+    double v;
+    if (fine) {
+        const FineDev F = fine[C.inst];
+        int cg[3] = {0, 0, 0};
+        for (int d = 0; d < nd; d++) cg[d] = F.i1c[d] + nn_coarse(g[d], F.df);
+        v = velc[cg[0] + (int64_t)F.nc[0] * (cg[1] + (int64_t)F.nc[1] * cg[2])];
+    } else {
+        v = velc[l];
+    }
+    double xyz[3];
+    for (int d = 0; d < nd; d++) xyz[d] = fma((double)g[d], I.h[d], I.o[d]);
+    const double dist = distance2points(kind, nd, xyz, C.src);
+    ttpool[I.tt_off + l] = -(T)(dist / v);
+    seed_tiles(I, nd, g, flags);
+}
+
+// nearest-neighbour fine slowness of every refined box (createfinegrid, eikmainfwd.jl:1035-1056)
+__global__ void k_fine_setup(const FineDev* __restrict__ fine, int nfine, const InstDev* __restrict__ inst,
+                             const double* __restrict__ velc, double* __restrict__ slowf, double* __restrict__ ttf,
+                             int nd) {
+    const int f = blockIdx.y;
+    if (f >= nfine) return;
+    const FineDev F = fine[f];
+    const InstDev I = inst[f];
+    const int64_t n = (int64_t)I.n[0] * I.n[1] * I.n[2];
+    for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < n; l += (int64_t)gridDim.x * blockDim.x) {
+        int g[3];
+        g[0] = (int)(l % I.n[0]);
+        g[1] = (int)((l / I.n[0]) % I.n[1]);
+        g[2] = (int)(l / ((int64_t)I.n[0] * I.n[1]));
+        int cg[3] = {0, 0, 0};
+        for (int d = 0; d < nd; d++) cg[d] = F.i1c[d] + nn_coarse(g[d], F.df);
+        const double v = velc[cg[0] + (int64_t)F.nc[0] * (cg[1] + (int64_t)F.nc[1] * cg[2])];
+        slowf[I.s_off + l] = 1.0 / v;
+        ttf[I.tt_off + l] = EIK_INF;
+    }
+}
+
+// Transfer of the refined solution to the coarse grid (eikmainfwd.jl:689-754).  The heap FMM on
+// the fine box stops at the first accepted node on a box face that is not a model border, once
+// at least one coarse-coincident node has been transferred; the coarse init set is every
+// coarse-coincident fine node accepted up to and including that node.  With the converged fine
+// solution: t_c = min tt over coarse-coincident nodes, t_stop = min tt over face nodes with
+// tt >= t_c, transfer = coarse-coincident nodes with tt <= t_stop.  One block per source.
+template <typename T>
+__global__ void __launch_bounds__(256) k_transfer(const FineDev* __restrict__ fine, const InstDev* __restrict__ finst,
+                                                  const double* __restrict__ ttf, const InstDev* __restrict__ cinst,
+                                                  T* __restrict__ ttc, uint8_t* __restrict__ cflags,
+                                                  int* __restrict__ ntransfer, int nd) {
+    const int f = blockIdx.x;
+    const FineDev F = fine[f];
+    const InstDev I = finst[f];
+    const InstDev C = cinst[F.cinst];
+    const int64_t n = (int64_t)I.n[0] * I.n[1] * I.n[2];
+    __shared__ double red[256];
+    __shared__ int cnt;
+    auto coord = [&](int64_t l, int* g) {
+        g[0] = (int)(l % I.n[0]);
+        g[1] = (int)((l / I.n[0]) % I.n[1]);
+        g[2] = (int)(l / ((int64_t)I.n[0] * I.n[1]));
+    };
+    auto coincident = [&](const int* g) {
+        for (int d = 0; d < nd; d++)
+            if (g[d] % F.df != 0) return false;
+        return true;
+    };
+    auto blockmin = [&](double v) {
+        red[threadIdx.x] = v;
+        __syncthreads();
+        for (int s = 128; s > 0; s >>= 1) {
+            if (threadIdx.x < s) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + s]);
+            __syncthreads();
+        }
+        const double r = red[0];
+        __syncthreads();
+        return r;
+    };
+    double m = EIK_INF;
+    for (int64_t l = threadIdx.x; l < n; l += 256) {
+        int g[3];
+        coord(l, g);
+        if (coincident(g)) m = fmin(m, fabs(ttf[I.tt_off + l]));
+    }
+    const double tc = blockmin(m);
+    m = EIK_INF;
+    for (int64_t l = threadIdx.x; l < n; l += 256) {
+        int g[3];
+        coord(l, g);
+        const double raw = ttf[I.tt_off + l];
+        if (signbit(raw)) continue;  // init nodes are never popped
+        bool face = false;
+        for (int d = 0; d < nd; d++) {
+            if (g[d] == 0 && !F.outmin[d]) face = true;
+            if (g[d] == I.n[d] - 1 && !F.outmax[d]) face = true;
+        }
+        if (face && raw >= tc) m = fmin(m, raw);
+    }
+    const double tstop = blockmin(m);
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
+    for (int64_t l = threadIdx.x; l < n; l += 256) {
+        int g[3];
+        coord(l, g);
+        if (!coincident(g)) continue;
+        const double t = fabs(ttf[I.tt_off + l]);
+        if (!(t <= tstop)) continue;
+        int cg[3] = {0, 0, 0};
+        for (int d = 0; d < nd; d++) cg[d] = g[d] / F.df + F.i1c[d];
+        const int64_t lc = cg[0] + (int64_t)C.n[0] * (cg[1] + (int64_t)C.n[1] * cg[2]);
+        ttc[C.tt_off + lc] = -(T)t;
+        seed_tiles(C, nd, cg, cflags);
+        atomicAdd(&cnt, 1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) ntransfer[f] = cnt;
+}
+
+// receiver picks: bilinear_interp (eikforward2D.jl:17-87) / trilinear_interp (eikforward3D.jl:17-140)
+template <typename T>
+__global__ void k_receivers(const double* __restrict__ rec, const int* __restrict__ rsrc, int64_t nrec,
+                            const InstDev* __restrict__ inst, const T* __restrict__ ttpool, int nd, int kind,
+                            double* __restrict__ picks) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrec) return;
+    const InstDev I = inst[rsrc[r]];
+    const T* tt = ttpool + I.tt_off;
+    picks[r] = interp_receiver<T>(tt, I, nd, kind, rec + r * nd, nullptr, nullptr);
+}
+
+// final |tt| of a pool (removes the init-node sign), converted to FP64
+template <typename T>
+__global__ void k_export(const T* __restrict__ src, double* __restrict__ dst, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = fabs((double)src[i]);
+}
+
+// ================================================================= host-side solver driver
+
+static unsigned nblocks(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+template <int ND, typename T>
+static int fim_run_t(eik_ctx* ctx, const InstDev* d_inst, int ninst, int64_t ntiles, T* ttpool,
+                     const T* slowpool, int64_t* rounds_out, double* visits_out) {
+    DevBuf& list = ctx->buf("fim_list");
+    if (list.ensure((size_t)ntiles * sizeof(int64_t))) return ctx->fail(EIK_ENOMEM, "out of device memory (tile list)");
+    uint8_t* flags = (uint8_t*)ctx->buf("fim_flags").p;
+    unsigned* d_cnt = (unsigned*)ctx->d_small;
+    unsigned* h_cnt = (unsigned*)ctx->h_small;
+    int64_t rounds = 0;
+    CK(hipMemsetAsync(d_cnt + 1, 0, sizeof(unsigned), ctx->stream));
+    const int64_t maxrounds = 50000;
+    for (;;) {
+        CK(hipMemsetAsync(d_cnt, 0, sizeof(unsigned), ctx->stream));
+        k_compact<<<nblocks(ntiles, 256), 256, 0, ctx->stream>>>(flags, ntiles, (int64_t*)list.p, d_cnt);
+        CK(hipMemcpyAsync(h_cnt, d_cnt, sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
+        CK(hipStreamSynchronize(ctx->stream));
+        const unsigned nact = h_cnt[0];
+        if (nact == 0) break;
+        if (++rounds > maxrounds) return ctx->fail(EIK_ENOCONV, "traveltime solve did not converge");
+        k_fim_round<ND, T><<<nact, kBlock, 0, ctx->stream>>>((const int64_t*)list.p, d_inst, ninst, ttpool, slowpool,
+                                                               flags, d_cnt + 1);
+        CK(hipGetLastError());
+    }
+    CK(hipMemcpyAsync(h_cnt + 1, d_cnt + 1, sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    if (rounds_out) *rounds_out = rounds;
+    if (visits_out) *visits_out = (double)h_cnt[1];
+    return EIK_OK;
+}
+
+int fim_run(eik_ctx* ctx, int nd, int prec, const InstDev* d_inst, int ninst, int64_t ntiles, void* ttpool,
+            const void* slowpool, int64_t* rounds, double* visits) {
+    if (nd == 2)
+        return prec == EIK_PREC_FP64
+                   ? fim_run_t<2, double>(ctx, d_inst, ninst, ntiles, (double*)ttpool, (const double*)slowpool, rounds,
+                                          visits)
+                   : fim_run_t<2, float>(ctx, d_inst, ninst, ntiles, (float*)ttpool, (const float*)slowpool, rounds,
+                                         visits);
+    return prec == EIK_PREC_FP64
+               ? fim_run_t<3, double>(ctx, d_inst, ninst, ntiles, (double*)ttpool, (const double*)slowpool, rounds,
+                                      visits)
+               : fim_run_t<3, float>(ctx, d_inst, ninst, ntiles, (float*)ttpool, (const float*)slowpool, rounds, visits);
+}
+
+// ----------------------------------------------------------------- forward pass of one call
+int forward_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, int prec, const eik_params* P) {
+    const int nd = M->g.ndim;
+    const int kind = M->g.kind;
+    const int64_t N = M->N;
+    const int64_t nsrc = S.nsrc;
+    hipStream_t st = ctx->stream;
+    Timer tm(ctx);
+
+    // coarse instances: one grid per source, sharing the model's slowness
+    std::vector<InstDev> cinst(nsrc);
+    const int TX = nd == 2 ? 16 : 8, TZ = nd == 2 ? 1 : 8;
+    int tiles[3] = {(int)((M->g.n[0] + TX - 1) / TX), (int)((M->g.n[1] + TX - 1) / TX),
+                    nd == 3 ? (int)((M->g.n[2] + TZ - 1) / TZ) : 1};
+    const int64_t tiles_per = (int64_t)tiles[0] * tiles[1] * tiles[2];
+    for (int64_t s = 0; s < nsrc; s++) {
+        InstDev& I = cinst[s];
+        make_inst(M->g, I);
+        for (int d = 0; d < 3; d++) I.t[d] = tiles[d];
+        I.tt_off = s * N;
+        I.s_off = 0;
+        I.tile_base = s * tiles_per;
+    }
+    const int64_t ntiles_c = nsrc * tiles_per;
+    const size_t esz = prec == EIK_PREC_FP64 ? sizeof(double) : sizeof(float);
+    DevBuf& ttc = ctx->buf("tt_coarse");
+    if (ttc.ensure((size_t)nsrc * N * esz)) return ctx->fail(EIK_ENOMEM, "out of device memory (traveltime grids)");
+    DevBuf& dci = ctx->buf("inst_coarse");
+    if (dci.ensure(cinst.size() * sizeof(InstDev))) return ctx->fail(EIK_ENOMEM, "out of device memory");
+    CK(hipMemcpyAsync(dci.p, cinst.data(), cinst.size() * sizeof(InstDev), hipMemcpyHostToDevice, st));
+
+    int64_t maxt = ntiles_c;
+    std::vector<FineDev> fine;
+    std::vector<InstDev> finst;
+    int64_t ntiles_f = 0, nfine_pts = 0;
+    if (P->refinearoundsrc) {
+        fine.resize(nsrc);
+        finst.resize(nsrc);
+        for (int64_t s = 0; s < nsrc; s++) {
+            build_fine(M->g, S.src.data() + s * nd, P->downscalefactor, P->noderadius, fine[s], finst[s]);
+            fine[s].cinst = (int)s;
+            finst[s].tt_off = nfine_pts;
+            finst[s].s_off = nfine_pts;
+            finst[s].tile_base = ntiles_f;
+            nfine_pts += (int64_t)finst[s].n[0] * finst[s].n[1] * finst[s].n[2];
+            ntiles_f += (int64_t)finst[s].t[0] * finst[s].t[1] * finst[s].t[2];
+        }
+        maxt = std::max(maxt, ntiles_f);
+    }
+    DevBuf& flags = ctx->buf("fim_flags");
+    if (flags.ensure((size_t)maxt)) return ctx->fail(EIK_ENOMEM, "out of device memory (tile flags)");
+    CK(hipMemsetAsync(flags.p, 0, (size_t)maxt, st));
+
+    // coarse grids start at +inf
+    if (prec == EIK_PREC_FP64)
+        k_fill<double><<<nblocks(nsrc * N, 256), 256, 0, st>>>((double*)ttc.p, nsrc * N, EIK_INF);
+    else
+        k_fill<float><<<nblocks(nsrc * N, 256), 256, 0, st>>>((float*)ttc.p, nsrc * N, EIK_INFF);
+
+    std::vector<CornerDev> corners;
+    if (P->refinearoundsrc) {
+        DevBuf& dfi = ctx->buf("inst_fine");
+        DevBuf& dfd = ctx->buf("fine_desc");
+        DevBuf& ttf = ctx->buf("tt_fine");
+        DevBuf& slf = ctx->buf("slow_fine");
+        if (dfi.ensure(finst.size() * sizeof(InstDev)) || dfd.ensure(fine.size() * sizeof(FineDev)) ||
+            ttf.ensure((size_t)nfine_pts * sizeof(double)) || slf.ensure((size_t)nfine_pts * sizeof(double)))
+            return ctx->fail(EIK_ENOMEM, "out of device memory (refined grids)");
+        CK(hipMemcpyAsync(dfi.p, finst.data(), finst.size() * sizeof(InstDev), hipMemcpyHostToDevice, st));
+        CK(hipMemcpyAsync(dfd.p, fine.data(), fine.size() * sizeof(FineDev), hipMemcpyHostToDevice, st));
+        dim3 gs(64, (unsigned)nsrc);
+        k_fine_setup<<<gs, 256, 0, st>>>((const FineDev*)dfd.p, (int)nsrc, (const InstDev*)dfi.p, M->d_vel,
+                                         (double*)slf.p, (double*)ttf.p, nd);
+        for (int64_t s = 0; s < nsrc; s++) {
+            for (int c = 0; c < fine[s].ncorn; c++) {
+                CornerDev C{};
+                C.inst = (int)s;
+                for (int d = 0; d < 3; d++) C.ijk[d] = fine[s].corn[c][d];
+                for (int d = 0; d < nd; d++) C.src[d] = S.src[s * nd + d];
+                corners.push_back(C);
+            }
+        }
+        DevBuf& dcs = ctx->buf("corners");
+        if (dcs.ensure(corners.size() * sizeof(CornerDev))) return ctx->fail(EIK_ENOMEM, "out of device memory");
+        CK(hipMemcpyAsync(dcs.p, corners.data(), corners.size() * sizeof(CornerDev), hipMemcpyHostToDevice, st));
+        k_corners<double><<<nblocks((int64_t)corners.size(), 64), 64, 0, st>>>(
+            (const CornerDev*)dcs.p, (int)corners.size(), (const InstDev*)dfi.p, (double*)ttf.p, M->d_vel,
+            (const FineDev*)dfd.p, (uint8_t*)flags.p, nd, kind);
+        CK(hipGetLastError());
+        int64_t r = 0;
+        double vis = 0;
+        tm.mark();
+        int rc = fim_run(ctx, nd, EIK_PREC_FP64, (const InstDev*)dfi.p, (int)nsrc, ntiles_f, ttf.p, slf.p, &r, &vis);
+        if (rc) return rc;
+        ctx->stats[1] = tm.mark();
+        DevBuf& ntr = ctx->buf("ntransfer");
+        if (ntr.ensure((size_t)nsrc * sizeof(int))) return ctx->fail(EIK_ENOMEM, "out of device memory");
+        if (prec == EIK_PREC_FP64)
+            k_transfer<double><<<(unsigned)nsrc, 256, 0, st>>>((const FineDev*)dfd.p, (const InstDev*)dfi.p,
+                                                              (const double*)ttf.p, (const InstDev*)dci.p,
+                                                              (double*)ttc.p, (uint8_t*)flags.p, (int*)ntr.p, nd);
+        else
+            k_transfer<float><<<(unsigned)nsrc, 256, 0, st>>>((const FineDev*)dfd.p, (const InstDev*)dfi.p,
+                                                             (const double*)ttf.p, (const InstDev*)dci.p, (float*)ttc.p,
+                                                             (uint8_t*)flags.p, (int*)ntr.p, nd);
+        CK(hipGetLastError());
+    } else {
+        for (int64_t s = 0; s < nsrc; s++) {
+            int corn[8][3];
+            const int nc = find_enclosing_box(M->g, S.src.data() + s * nd, corn);
+            for (int c = 0; c < nc; c++) {
+                CornerDev C{};
+                C.inst = (int)s;
+                for (int d = 0; d < 3; d++) C.ijk[d] = corn[c][d];
+                for (int d = 0; d < nd; d++) C.src[d] = S.src[s * nd + d];
+                corners.push_back(C);
+            }
+        }
+        DevBuf& dcs = ctx->buf("corners");
+        if (dcs.ensure(corners.size() * sizeof(CornerDev))) return ctx->fail(EIK_ENOMEM, "out of device memory");
+        CK(hipMemcpyAsync(dcs.p, corners.data(), corners.size() * sizeof(CornerDev), hipMemcpyHostToDevice, st));
+        if (prec == EIK_PREC_FP64)
+            k_corners<double><<<nblocks((int64_t)corners.size(), 64), 64, 0, st>>>(
+                (const CornerDev*)dcs.p, (int)corners.size(), (const InstDev*)dci.p, (double*)ttc.p, M->d_vel, nullptr,
+                (uint8_t*)flags.p, nd, kind);
+        else
+            k_corners<float><<<nblocks((int64_t)corners.size(), 64), 64, 0, st>>>(
+                (const CornerDev*)dcs.p, (int)corners.size(), (const InstDev*)dci.p, (float*)ttc.p, M->d_vel, nullptr,
+                (uint8_t*)flags.p, nd, kind);
+        CK(hipGetLastError());
+        tm.mark();
+    }
+    int64_t rounds = 0;
+    double visits = 0;
+    const void* slow = prec == EIK_PREC_FP64 ? (const void*)M->d_slow64 : (const void*)M->d_slow32;
+    int rc = fim_run(ctx, nd, prec, (const InstDev*)dci.p, (int)nsrc, ntiles_c, ttc.p, slow, &rounds, &visits);
+    if (rc) return rc;
+    ctx->stats[2] = tm.mark();
+    ctx->stats[4] = visits;
+    ctx->stats[5] = (double)rounds;
+    return EIK_OK;
+}
+
+int forward_picks(eik_ctx* ctx, eik_model* M, const SourceSet& S, int prec, double* picks, double* ttgrid) {
+    hipStream_t st = ctx->stream;
+    const int nd = M->g.ndim;
+    const int64_t nrec = S.nrec();
+    const int64_t N = M->N;
+    if (nrec > 0) {
+        DevBuf& drec = ctx->buf("rec");
+        DevBuf& drs = ctx->buf("recsrc");
+        DevBuf& dpk = ctx->buf("picks");
+        if (drec.ensure((size_t)nrec * nd * sizeof(double)) || drs.ensure((size_t)nrec * sizeof(int)) ||
+            dpk.ensure((size_t)nrec * sizeof(double)))
+            return ctx->fail(EIK_ENOMEM, "out of device memory (receivers)");
+        std::vector<int> rs(nrec);
+        for (int64_t s = 0; s < S.nsrc; s++)
+            for (int64_t r = S.rec_off[s]; r < S.rec_off[s + 1]; r++) rs[r] = (int)s;
+        CK(hipMemcpyAsync(drec.p, S.rec.data(), (size_t)nrec * nd * sizeof(double), hipMemcpyHostToDevice, st));
+        CK(hipMemcpyAsync(drs.p, rs.data(), (size_t)nrec * sizeof(int), hipMemcpyHostToDevice, st));
+        const InstDev* dci = (const InstDev*)ctx->buf("inst_coarse").p;
+        void* ttc = ctx->buf("tt_coarse").p;
+        if (prec == EIK_PREC_FP64)
+            k_receivers<double><<<nblocks(nrec, 128), 128, 0, st>>>((const double*)drec.p, (const int*)drs.p, nrec, dci,
+                                                                    (const double*)ttc, nd, M->g.kind, (double*)dpk.p);
+        else
+            k_receivers<float><<<nblocks(nrec, 128), 128, 0, st>>>((const double*)drec.p, (const int*)drs.p, nrec, dci,
+                                                                   (const float*)ttc, nd, M->g.kind, (double*)dpk.p);
+        CK(hipGetLastError());
+        CK(hipMemcpyAsync(picks, dpk.p, (size_t)nrec * sizeof(double), hipMemcpyDeviceToHost, st));
+    }
+    if (ttgrid) {
+        DevBuf& ex = ctx->buf("export");
+        if (ex.ensure((size_t)N * sizeof(double))) return ctx->fail(EIK_ENOMEM, "out of device memory");
+        void* ttc = ctx->buf("tt_coarse").p;
+        for (int64_t s = 0; s < S.nsrc; s++) {
+            if (prec == EIK_PREC_FP64)
+                k_export<double><<<nblocks(N, 256), 256, 0, st>>>((const double*)ttc + s * N, (double*)ex.p, N);
+            else
+                k_export<float><<<nblocks(N, 256), 256, 0, st>>>((const float*)ttc + s * N, (double*)ex.p, N);
+            CK(hipMemcpyAsync(ttgrid + s * N, ex.p, (size_t)N * sizeof(double), hipMemcpyDeviceToHost, st));
+        }
+    }
+    CK(hipStreamSynchronize(st));
+    return EIK_OK;
+}
+
+int model_slowness(eik_model* M) {
+    k_slowness<<<nblocks(M->N, 256), 256, 0, M->ctx->stream>>>(M->d_vel, M->d_slow64, M->d_slow32, M->N);
+    return hipGetLastError() == hipSuccess ? EIK_OK : EIK_EDEVICE;
+}
+
+}  // namespace eik

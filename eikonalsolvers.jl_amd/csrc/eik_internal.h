@@ -1,0 +1,287 @@
+// eik_internal.h -- types shared by the kernels and the C-ABI host code of libeik.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/eik.h"
+#include "eik_stencil.h"
+
+// ------------------------------------------------------------------ device-side descriptors
+namespace eik {
+
+// One grid solved by the tiled solver: a source's coarse grid or its refined near-source box.
+struct InstDev {
+    int32_t n[3];        // grid nodes per axis (n[2] = 1 in 2D)
+    int32_t t[3];        // tiles per axis
+    int64_t tt_off;      // element offset of this grid in its traveltime pool
+    int64_t s_off;       // element offset of its slowness in the slowness pool
+    int64_t tile_base;   // first global tile id (tile flags / lists)
+    double h[3];         // node spacing per axis (Cartesian hgrid; spherical dr, dtheta[deg], dphi[deg])
+    double o[3];         // origin (cooinit)
+    double dthrad;       // spherical: dtheta in radians
+    double dphrad;       // spherical 3D: dphi in radians
+    int32_t kind;        // EIK_GRID*
+    int32_t pad;
+};
+
+// Refined box around one source (createfinegrid, src/EikSolv/eikmainfwd.jl:950-1104)
+struct FineDev {
+    int32_t i1c[3];      // coarse index of the box origin
+    int32_t nc[3];       // coarse grid size (to index the coarse velocity)
+    int32_t df;          // downscalefactor
+    int32_t outmin[3];   // box clipped at the model's lower border along each axis
+    int32_t outmax[3];   // ... upper border
+    int32_t ncorn;       // source-box corners on the fine grid
+    int32_t corn[8][3];
+    int32_t cinst;       // coarse instance of the same source
+};
+
+// One source-box corner to initialise (sourceboxloctt!, eikmainfwd.jl:1195-1322)
+struct CornerDev {
+    int32_t inst;
+    int32_t ijk[3];
+    double src[3];
+};
+
+// fine index -> index inside the coarse window, nearest-neighbour rule of createfinegrid
+// (eikmainfwd.jl:1039-1041: round up when the remainder is >= df/2+1), 0-based
+__host__ __device__ __forceinline__ int nn_coarse(int i0, int df) {
+    const int i_f = i0 + 1;
+    const int di = (i_f - 1) / df;
+    const int ri = i_f - di * df;
+    return ((double)ri >= (double)df / 2.0 + 1.0) ? di + 1 : di;
+}
+
+__host__ __device__ __forceinline__ double deg2rad(double x) { return x * (M_PI / 180.0); }
+
+// distance2points (src/EikSolv/eikutils.jl:47-73); 3D sphere: chord length (this build's semantics)
+__host__ __device__ __forceinline__ double distance2points(int kind, int nd, const double* a, const double* b) {
+#pragma clang fp contract(off)
+    if (kind == EIK_GRID2D_SPHERE) {
+        const double r1 = a[0], t1 = a[1], r2 = b[0], t2 = b[1];
+        return sqrt(r1 * r1 + r2 * r2 - 2.0 * r1 * r2 * cos(deg2rad(t1 - t2)));
+    }
+    if (kind == EIK_GRID3D_SPHERE) {
+        const double xa = a[0] * sin(deg2rad(a[1])) * cos(deg2rad(a[2]));
+        const double ya = a[0] * sin(deg2rad(a[1])) * sin(deg2rad(a[2]));
+        const double za = a[0] * cos(deg2rad(a[1]));
+        const double xb = b[0] * sin(deg2rad(b[1])) * cos(deg2rad(b[2]));
+        const double yb = b[0] * sin(deg2rad(b[1])) * sin(deg2rad(b[2]));
+        const double zb = b[0] * cos(deg2rad(b[1]));
+        return sqrt((xa - xb) * (xa - xb) + (ya - yb) * (ya - yb) + (za - zb) * (za - zb));
+    }
+    double s = 0.0;
+    for (int d = 0; d < nd; d++) s += (a[d] - b[d]) * (a[d] - b[d]);
+    return sqrt(s);
+}
+
+// per-node stencil spacing (eikforward2D.jl:115-119, eikforward3D.jl:168-172)
+template <typename T>
+__device__ __forceinline__ void node_spacing(const InstDev& I, const int* g, T* dh) {
+#pragma clang fp contract(off)
+    if (I.kind == EIK_GRID2D_CART || I.kind == EIK_GRID3D_CART) {
+        dh[0] = dh[1] = dh[2] = (T)I.h[0];
+        return;
+    }
+    const double r = fma((double)g[0], I.h[0], I.o[0]);
+    dh[0] = (T)I.h[0];
+    dh[1] = (T)(r * I.dthrad);
+    if (I.kind == EIK_GRID3D_SPHERE) {
+        const double th = fma((double)g[1], I.h[1], I.o[1]);
+        dh[2] = (T)(r * sin(deg2rad(th)) * I.dphrad);
+    } else {
+        dh[2] = (T)0;
+    }
+}
+
+// Receiver interpolation on a traveltime grid (|value| strips the init-node sign).
+// bilinear_interp (eikforward2D.jl:17-87); trilinear_interp (eikforward3D.jl:17-140) keeps the
+// reference's weight xd*(1-yd)*(1-zd) for corner f101 so picks and gradients stay consistent
+// with it.  coef/idx (optional) receive the weights and node indices in the reference's order.
+template <typename T>
+__host__ __device__ __forceinline__ double interp_receiver(const T* tt, const InstDev& I, int nd, int kind,
+                                                          const double* pt, double* coef, int64_t* idx) {
+#pragma clang fp contract(off)
+    const bool sph = (kind == EIK_GRID2D_SPHERE || kind == EIK_GRID3D_SPHERE);
+    if (nd == 2) {
+        const double dx = I.h[0], dy = sph ? I.h[1] : I.h[0];
+        const double xh = (pt[0] - I.o[0]) / dx, yh = (pt[1] - I.o[1]) / dy;
+        int64_t i = (int64_t)floor(xh + 1.0), j = (int64_t)floor(yh + 1.0);
+        const int64_t nx = I.n[0], ny = I.n[1];
+        if (i == nx) i -= 1;
+        else if (i > nx) i -= 2;
+        if (j == ny) j -= 1;
+        else if (j > ny) j -= 2;
+        const double xd = xh - (double)(i - 1), yd = yh - (double)(j - 1);
+        const int64_t l00 = (i - 1) + nx * (j - 1), l10 = l00 + 1, l01 = l00 + nx, l11 = l01 + 1;
+        if (coef) {
+            coef[0] = (1.0 - xd) * (1.0 - yd);
+            coef[1] = (1.0 - yd) * xd;
+            coef[2] = (1.0 - xd) * yd;
+            coef[3] = xd * yd;
+            idx[0] = l00;
+            idx[1] = l10;
+            idx[2] = l01;
+            idx[3] = l11;
+        }
+        if (!tt) return 0.0;
+        const double f00 = fabs((double)tt[l00]), f10 = fabs((double)tt[l10]), f01 = fabs((double)tt[l01]),
+                     f11 = fabs((double)tt[l11]);
+        return f00 * (1.0 - xd) * (1.0 - yd) + f10 * (1.0 - yd) * xd + f01 * (1.0 - xd) * yd + f11 * xd * yd;
+    }
+    const double dx = I.h[0], dy = sph ? I.h[1] : I.h[0], dz = sph ? I.h[2] : I.h[0];
+    const double xh = (pt[0] - I.o[0]) / dx, yh = (pt[1] - I.o[1]) / dy, zh = (pt[2] - I.o[2]) / dz;
+    const double x = pt[0] - I.o[0], y = pt[1] - I.o[1], z = pt[2] - I.o[2];
+    int64_t i = (int64_t)floor(xh + 1.0), j = (int64_t)floor(yh + 1.0), k = (int64_t)floor(zh + 1.0);
+    const int64_t nx = I.n[0], ny = I.n[1], nz = I.n[2];
+    if (i == nx) i -= 1;
+    else if (i > nx) i -= 2;
+    if (j == ny) j -= 1;
+    else if (j > ny) j -= 2;
+    if (k == nz) k -= 1;
+    else if (k > nz) k -= 2;
+    const double x0 = (double)(i - 1) * dx, y0 = (double)(j - 1) * dy, z0 = (double)(k - 1) * dz;
+    const double x1 = (double)i * dx, y1 = (double)j * dy, z1 = (double)k * dz;
+    const double xd = (x - x0) / (x1 - x0), yd = (y - y0) / (y1 - y0), zd = (z - z0) / (z1 - z0);
+    const int64_t nxy = nx * ny;
+    const int64_t l000 = (i - 1) + nx * (j - 1) + nxy * (k - 1);
+    const int64_t l100 = l000 + 1, l010 = l000 + nx, l001 = l000 + nxy, l110 = l010 + 1, l011 = l010 + nxy,
+                  l101 = l001 + 1, l111 = l011 + 1;
+    if (coef) {
+        coef[0] = (1 - xd) * (1 - yd) * (1 - zd);
+        coef[1] = xd * (1 - yd) * (1 - zd);
+        coef[2] = (1 - xd) * yd * (1 - zd);
+        coef[3] = (1 - xd) * (1 - yd) * zd;
+        coef[4] = xd * yd * (1 - zd);
+        coef[5] = (1 - xd) * yd * zd;
+        coef[6] = xd * (1 - yd) * (1 - zd);
+        coef[7] = xd * yd * zd;
+        idx[0] = l000;
+        idx[1] = l100;
+        idx[2] = l010;
+        idx[3] = l001;
+        idx[4] = l110;
+        idx[5] = l011;
+        idx[6] = l101;
+        idx[7] = l111;
+    }
+    if (!tt) return 0.0;
+    const double f000 = fabs((double)tt[l000]), f100 = fabs((double)tt[l100]), f010 = fabs((double)tt[l010]),
+                 f001 = fabs((double)tt[l001]), f110 = fabs((double)tt[l110]), f011 = fabs((double)tt[l011]),
+                 f101 = fabs((double)tt[l101]), f111 = fabs((double)tt[l111]);
+    return f000 * (1 - xd) * (1 - yd) * (1 - zd) + f100 * xd * (1 - yd) * (1 - zd) +
+           f010 * (1 - xd) * yd * (1 - zd) + f001 * (1 - xd) * (1 - yd) * zd + f110 * xd * yd * (1 - zd) +
+           f011 * (1 - xd) * yd * zd + f101 * xd * (1 - yd) * (1 - zd) + f111 * xd * yd * zd;
+}
+
+}  // namespace eik
+
+// ------------------------------------------------------------------ host-side objects
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    int ensure(size_t n) {
+        if (n <= cap && p) return 0;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        size_t want = n ? n : 16;
+        if (hipMalloc(&p, want) != hipSuccess) {
+            p = nullptr;
+            return -1;
+        }
+        cap = want;
+        return 0;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+struct eik_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::map<std::string, DevBuf> bufs;
+    void* d_small = nullptr;  // device counters
+    void* h_small = nullptr;  // pinned host mirror of the counters
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    double stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    std::mutex mu;
+    DevBuf& buf(const char* name) { return bufs[name]; }
+    int fail(int code, const char* fmt, ...) {
+        char b[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(b, sizeof(b), fmt, ap);
+        va_end(ap);
+        err = b;
+        return code;
+    }
+};
+
+struct eik_model {
+    eik_ctx* ctx = nullptr;
+    eik_grid g{};
+    int64_t N = 0;
+    double* d_vel = nullptr;
+    double* d_slow64 = nullptr;
+    float* d_slow32 = nullptr;
+};
+
+#define CK(x)                                                                                    \
+    do {                                                                                         \
+        hipError_t e_ = (x);                                                                     \
+        if (e_ != hipSuccess) return ctx->fail(EIK_EDEVICE, "%s: %s", #x, hipGetErrorString(e_)); \
+    } while (0)
+
+namespace eik {
+
+struct SourceSet {
+    int64_t nsrc = 0;
+    int nd = 2;
+    std::vector<double> src;      // nsrc*nd
+    std::vector<int64_t> rec_off; // nsrc+1
+    std::vector<double> rec;      // nrec*nd
+    int64_t nrec() const { return rec_off.empty() ? 0 : rec_off.back(); }
+};
+
+// elapsed device time between marks on the context stream
+struct Timer {
+    eik_ctx* c;
+    explicit Timer(eik_ctx* ctx) : c(ctx) {
+        (void)hipEventRecord(c->ev[0], c->stream);
+    }
+    double mark() {
+        (void)hipEventRecord(c->ev[1], c->stream);
+        (void)hipEventSynchronize(c->ev[1]);
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
+        std::swap(c->ev[0], c->ev[1]);
+        return (double)ms;
+    }
+};
+
+// host geometry (eik_geom.cpp)
+void make_inst(const eik_grid& g, InstDev& I);
+int find_enclosing_box(const eik_grid& g, const double* src, int corn[8][3]);
+void build_fine(const eik_grid& g, const double* src, int df, int noderadius, FineDev& F, InstDev& I);
+
+// forward (eik_fim.hip)
+int forward_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, int prec, const eik_params* P);
+int forward_picks(eik_ctx* ctx, eik_model* M, const SourceSet& S, int prec, double* picks, double* ttgrid);
+int model_slowness(eik_model* M);
+int fim_run(eik_ctx* ctx, int nd, int prec, const InstDev* d_inst, int ninst, int64_t ntiles, void* ttpool,
+            const void* slowpool, int64_t* rounds, double* visits);
+
+}  // namespace eik

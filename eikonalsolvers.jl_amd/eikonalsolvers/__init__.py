@@ -1,0 +1,12 @@
+"""EikonalSolvers.jl's public API on a native MI355X (gfx950) engine.
+
+Host-side mirror of the reference's exports (src/EikonalSolvers.jl:23-37) for the traveltime and
+gradient path; the computation runs in libeik.so (hand-written HIP, include/eik.h).
+"""
+from .grids import (ExtraParams, Grid2DCart, Grid2DSphere, Grid3DCart, Grid3DSphere, GridRefinementPars)
+from .api import (Context, EikModel, default_context, eikgradient, eikttimemisfit, eiktraveltime)
+
+__all__ = [
+    "ExtraParams", "GridRefinementPars", "Grid2DCart", "Grid3DCart", "Grid2DSphere", "Grid3DSphere",
+    "eiktraveltime", "eikgradient", "eikttimemisfit", "EikModel", "Context", "default_context",
+]

@@ -1,0 +1,114 @@
+/*
+ * eik.h -- C ABI of libeik, the MI355X (gfx950) eikonal engine.
+ *
+ * This is the drop-in boundary under the host-language API of EikonalSolvers.jl.  Each entry
+ * point replaces one public function of the reference (cited per declaration); the host mirror
+ * (Python ctypes in eikonalsolvers.jl_amd/eikonalsolvers, or a Julia ccall wrapper) converts
+ * the reference's arguments into the plain arrays below.
+ *
+ * Conventions
+ *   - Arrays are column-major with index 1 fastest, exactly like the reference's Julia arrays
+ *     (cart2lin, src/EikSolv/eikutils.jl:6-19).  Indices crossing this ABI are 0-based.
+ *   - coordsrc: nsrc rows of ndim doubles (packed rows).  Receivers of source s are rows
+ *     rec_off[s] .. rec_off[s+1]-1 of coordrec (packed rows); picks use the same packing.
+ *   - All output buffers are caller-allocated host memory.  Device memory is owned by the
+ *     context and model handles.
+ *   - Return value: EIK_OK (0) or a negative EIK_E* code; eik_last_error() gives the message.
+ *     No exception ever crosses the ABI.  Validation failures that the reference raises with
+ *     @assert return EIK_EASSERT so the host wrapper can raise AssertionError.
+ *   - A context serialises its calls; use one context per host thread for concurrency.
+ */
+#ifndef EIK_H
+#define EIK_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EIK_OK 0
+#define EIK_EASSERT (-1)      /* reference @assert (AssertionError) */
+#define EIK_EINVAL (-2)       /* unsupported case (reference error(...) / ErrorException) */
+#define EIK_ENOMEM (-3)       /* device or host allocation failed */
+#define EIK_EDEVICE (-4)      /* HIP runtime error */
+#define EIK_ENOCONV (-5)      /* iterative solve did not converge within its round limit */
+
+/* grid kinds (src/EikSolv/eikstructs.jl:33-219) */
+#define EIK_GRID2D_CART 0
+#define EIK_GRID3D_CART 1
+#define EIK_GRID2D_SPHERE 2
+#define EIK_GRID3D_SPHERE 3
+
+/* arithmetic of the traveltime stencil */
+#define EIK_PREC_FP64 0   /* FP64 everywhere: the reference's arithmetic */
+#define EIK_PREC_MIXED 1  /* FP32 coarse stencil; FP64 near-source solve, adjoint and reductions */
+
+/* whichgrad (src/EikSolv/eikmaingrad.jl:40) */
+#define EIK_GRADVEL 1
+#define EIK_GRADSRCLOC 2
+#define EIK_GRADVELANDSRCLOC 3
+
+typedef struct {
+    int32_t kind;   /* EIK_GRID* */
+    int32_t ndim;   /* 2 or 3 */
+    int64_t n[3];   /* grsize */
+    double d[3];    /* Cartesian: hgrid (all three); spherical: dr, dtheta[deg], dphi[deg] */
+    double o[3];    /* cooinit */
+} eik_grid;
+
+/* ExtraParams (src/EikSolv/eikstructs.jl:246-282) plus precision */
+typedef struct {
+    int32_t refinearoundsrc;     /* default 1 */
+    int32_t downscalefactor;     /* grdrefpars.downscalefactor, default 5 */
+    int32_t noderadius;          /* grdrefpars.noderadius, default 3 */
+    int32_t radiussmoothgradsrc; /* default 3 */
+    int32_t smoothgradkern;      /* default 0 (odd when > 0) */
+    int32_t precision;           /* EIK_PREC_* */
+    int32_t reserved[2];         /* must be 0 */
+} eik_params;
+
+typedef struct eik_ctx eik_ctx;
+typedef struct eik_model eik_model;
+
+/* Library/context lifetime.  No reference counterpart (the reference is stateless Julia code):
+ * a context owns one HIP device, one stream and the device scratch pools. */
+int eik_ctx_create(int32_t device, eik_ctx** out);
+void eik_ctx_destroy(eik_ctx* ctx);
+const char* eik_last_error(const eik_ctx* ctx);
+const char* eik_version(void);
+
+/* Resident velocity model.  Replaces the `vel` argument that the reference passes to every call
+ * (eikmainfwd.jl:33, eikmaingrad.jl:31): the model is validated once (@assert all(vel.>0.0),
+ * eikmainfwd.jl:45) and kept in HBM as FP64 velocity plus FP64/FP32 slowness. */
+int eik_model_create(eik_ctx* ctx, const eik_grid* grid, const double* vel, eik_model** out);
+int eik_model_update_vel(eik_model* model, const double* vel);
+void eik_model_destroy(eik_model* model);
+
+/* eiktraveltime (src/EikSolv/eikmainfwd.jl:33-143): picks for every source's receivers and,
+ * when ttgrid != NULL, the nsrc traveltime grids (nsrc*N doubles, source-major). */
+int eik_traveltime(eik_ctx* ctx, eik_model* model, int64_t nsrc, const double* coordsrc,
+                   const int64_t* rec_off, const double* coordrec, const eik_params* params,
+                   double* ttpicks, double* ttgrid);
+
+/* eikgradient (src/EikSolv/eikmaingrad.jl:31-164).  gradvel (N doubles) and gradsrc (nsrc*ndim,
+ * packed rows) may be NULL when not requested by whichgrad; misf = 0.5*sum((t-tobs)^2/std^2). */
+int eik_gradient(eik_ctx* ctx, eik_model* model, int64_t nsrc, const double* coordsrc,
+                 const int64_t* rec_off, const double* coordrec, const double* pickobs,
+                 const double* stdobs, int32_t whichgrad, const eik_params* params,
+                 double* gradvel, double* gradsrc, double* misf);
+
+/* eikttimemisfit (src/EikSolv/eikmaingrad.jl:1302-1328) */
+int eik_misfit(eik_ctx* ctx, eik_model* model, int64_t nsrc, const double* coordsrc,
+               const int64_t* rec_off, const double* coordrec, const double* pickobs,
+               const double* stdobs, const eik_params* params, double* misf);
+
+/* Timing of the last call's device work (ms), split by phase; no reference counterpart.
+ * out[0] = total, [1] = near-source (fine grid) solve, [2] = coarse solve, [3] = adjoint,
+ * [4] = number of tile activations (node-visits / tile size) in the coarse solve,
+ * [5] = coarse solve rounds. */
+int eik_last_stats(const eik_ctx* ctx, double* out, int32_t nout);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
