@@ -115,7 +115,8 @@ int eik_ctx_create(int32_t device, eik_ctx** out) {
     ctx->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc(&ctx->d_small, 64) != hipSuccess || hipHostMalloc(&ctx->h_small, 64, hipHostMallocDefault) != hipSuccess ||
-        hipEventCreate(&ctx->ev[0]) != hipSuccess || hipEventCreate(&ctx->ev[1]) != hipSuccess) {
+        hipEventCreate(&ctx->ev[0]) != hipSuccess || hipEventCreate(&ctx->ev[1]) != hipSuccess ||
+        hipEventCreate(&ctx->kev[0]) != hipSuccess || hipEventCreate(&ctx->kev[1]) != hipSuccess) {
         delete ctx;
         return EIK_EDEVICE;
     }
@@ -130,8 +131,10 @@ void eik_ctx_destroy(eik_ctx* ctx) {
     for (auto& kv : ctx->bufs) kv.second.release();
     if (ctx->d_small) (void)hipFree(ctx->d_small);
     if (ctx->h_small) (void)hipHostFree(ctx->h_small);
-    for (int i = 0; i < 2; i++)
+    for (int i = 0; i < 2; i++) {
         if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
+        if (ctx->kev[i]) (void)hipEventDestroy(ctx->kev[i]);
+    }
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }

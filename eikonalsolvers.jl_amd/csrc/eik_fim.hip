@@ -415,46 +415,54 @@ static unsigned nblocks(int64_t n, int b) { return (unsigned)((n + b - 1) / b); 
 
 template <int ND, typename T>
 static int fim_run_t(eik_ctx* ctx, const InstDev* d_inst, int ninst, int64_t ntiles, T* ttpool,
-                     const T* slowpool, int64_t* rounds_out, double* visits_out) {
+                     const T* slowpool, FimStats* fs) {
     DevBuf& list = ctx->buf("fim_list");
     if (list.ensure((size_t)ntiles * sizeof(int64_t))) return ctx->fail(EIK_ENOMEM, "out of device memory (tile list)");
     uint8_t* flags = (uint8_t*)ctx->buf("fim_flags").p;
     unsigned* d_cnt = (unsigned*)ctx->d_small;
     unsigned* h_cnt = (unsigned*)ctx->h_small;
-    int64_t rounds = 0;
+    FimStats st{};
     CK(hipMemsetAsync(d_cnt + 1, 0, sizeof(unsigned), ctx->stream));
     const int64_t maxrounds = 50000;
+    bool pending = false;  // a timed round launch whose events are not read yet
     for (;;) {
         CK(hipMemsetAsync(d_cnt, 0, sizeof(unsigned), ctx->stream));
         k_compact<<<nblocks(ntiles, 256), 256, 0, ctx->stream>>>(flags, ntiles, (int64_t*)list.p, d_cnt);
         CK(hipMemcpyAsync(h_cnt, d_cnt, sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
         CK(hipStreamSynchronize(ctx->stream));
+        if (pending) {
+            float ms = 0.f;
+            CK(hipEventElapsedTime(&ms, ctx->kev[0], ctx->kev[1]));
+            st.kernel_ms += ms;
+            pending = false;
+        }
         const unsigned nact = h_cnt[0];
         if (nact == 0) break;
-        if (++rounds > maxrounds) return ctx->fail(EIK_ENOCONV, "traveltime solve did not converge");
+        if (++st.rounds > maxrounds) return ctx->fail(EIK_ENOCONV, "traveltime solve did not converge");
+        st.activations += nact;
+        CK(hipEventRecord(ctx->kev[0], ctx->stream));
         k_fim_round<ND, T><<<nact, kBlock, 0, ctx->stream>>>((const int64_t*)list.p, d_inst, ninst, ttpool, slowpool,
                                                                flags, d_cnt + 1);
         CK(hipGetLastError());
+        CK(hipEventRecord(ctx->kev[1], ctx->stream));
+        pending = true;
     }
     CK(hipMemcpyAsync(h_cnt + 1, d_cnt + 1, sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
     CK(hipStreamSynchronize(ctx->stream));
-    if (rounds_out) *rounds_out = rounds;
-    if (visits_out) *visits_out = (double)h_cnt[1];
+    st.sweeps = (double)h_cnt[1];
+    if (fs) *fs = st;
     return EIK_OK;
 }
 
 int fim_run(eik_ctx* ctx, int nd, int prec, const InstDev* d_inst, int ninst, int64_t ntiles, void* ttpool,
-            const void* slowpool, int64_t* rounds, double* visits) {
+            const void* slowpool, FimStats* fs) {
     if (nd == 2)
         return prec == EIK_PREC_FP64
-                   ? fim_run_t<2, double>(ctx, d_inst, ninst, ntiles, (double*)ttpool, (const double*)slowpool, rounds,
-                                          visits)
-                   : fim_run_t<2, float>(ctx, d_inst, ninst, ntiles, (float*)ttpool, (const float*)slowpool, rounds,
-                                         visits);
+                   ? fim_run_t<2, double>(ctx, d_inst, ninst, ntiles, (double*)ttpool, (const double*)slowpool, fs)
+                   : fim_run_t<2, float>(ctx, d_inst, ninst, ntiles, (float*)ttpool, (const float*)slowpool, fs);
     return prec == EIK_PREC_FP64
-               ? fim_run_t<3, double>(ctx, d_inst, ninst, ntiles, (double*)ttpool, (const double*)slowpool, rounds,
-                                      visits)
-               : fim_run_t<3, float>(ctx, d_inst, ninst, ntiles, (float*)ttpool, (const float*)slowpool, rounds, visits);
+               ? fim_run_t<3, double>(ctx, d_inst, ninst, ntiles, (double*)ttpool, (const double*)slowpool, fs)
+               : fim_run_t<3, float>(ctx, d_inst, ninst, ntiles, (float*)ttpool, (const float*)slowpool, fs);
 }
 
 // ----------------------------------------------------------------- forward pass of one call
@@ -546,10 +554,9 @@ int forward_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, int prec, cons
             (const CornerDev*)dcs.p, (int)corners.size(), (const InstDev*)dfi.p, (double*)ttf.p, M->d_vel,
             (const FineDev*)dfd.p, (uint8_t*)flags.p, nd, kind);
         CK(hipGetLastError());
-        int64_t r = 0;
-        double vis = 0;
         tm.mark();
-        int rc = fim_run(ctx, nd, EIK_PREC_FP64, (const InstDev*)dfi.p, (int)nsrc, ntiles_f, ttf.p, slf.p, &r, &vis);
+        FimStats fst{};
+        int rc = fim_run(ctx, nd, EIK_PREC_FP64, (const InstDev*)dfi.p, (int)nsrc, ntiles_f, ttf.p, slf.p, &fst);
         if (rc) return rc;
         ctx->stats[1] = tm.mark();
         DevBuf& ntr = ctx->buf("ntransfer");
@@ -589,14 +596,15 @@ int forward_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, int prec, cons
         CK(hipGetLastError());
         tm.mark();
     }
-    int64_t rounds = 0;
-    double visits = 0;
     const void* slow = prec == EIK_PREC_FP64 ? (const void*)M->d_slow64 : (const void*)M->d_slow32;
-    int rc = fim_run(ctx, nd, prec, (const InstDev*)dci.p, (int)nsrc, ntiles_c, ttc.p, slow, &rounds, &visits);
+    FimStats cst{};
+    int rc = fim_run(ctx, nd, prec, (const InstDev*)dci.p, (int)nsrc, ntiles_c, ttc.p, slow, &cst);
     if (rc) return rc;
     ctx->stats[2] = tm.mark();
-    ctx->stats[4] = visits;
-    ctx->stats[5] = (double)rounds;
+    ctx->stats[4] = cst.activations;
+    ctx->stats[5] = (double)cst.rounds;
+    ctx->stats[6] = cst.kernel_ms;
+    ctx->stats[7] = cst.sweeps;
     return EIK_OK;
 }
 

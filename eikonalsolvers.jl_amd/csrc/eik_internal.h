@@ -215,7 +215,8 @@ struct eik_ctx {
     std::map<std::string, DevBuf> bufs;
     void* d_small = nullptr;  // device counters
     void* h_small = nullptr;  // pinned host mirror of the counters
-    hipEvent_t ev[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};   // phase timer
+    hipEvent_t kev[2] = {nullptr, nullptr};  // per-launch kernel timer
     double stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     std::mutex mu;
     DevBuf& buf(const char* name) { return bufs[name]; }
@@ -281,7 +282,13 @@ void build_fine(const eik_grid& g, const double* src, int df, int noderadius, Fi
 int forward_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, int prec, const eik_params* P);
 int forward_picks(eik_ctx* ctx, eik_model* M, const SourceSet& S, int prec, double* picks, double* ttgrid);
 int model_slowness(eik_model* M);
+struct FimStats {
+    int64_t rounds = 0;        // solver rounds (one launch each)
+    double activations = 0;    // tile activations (blocks launched)
+    double sweeps = 0;         // local relaxation sweeps summed over activations
+    double kernel_ms = 0;      // summed duration of the round launches
+};
 int fim_run(eik_ctx* ctx, int nd, int prec, const InstDev* d_inst, int ninst, int64_t ntiles, void* ttpool,
-            const void* slowpool, int64_t* rounds, double* visits);
+            const void* slowpool, FimStats* fs);
 
 }  // namespace eik

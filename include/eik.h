@@ -104,8 +104,9 @@ int eik_misfit(eik_ctx* ctx, eik_model* model, int64_t nsrc, const double* coord
 
 /* Timing of the last call's device work (ms), split by phase; no reference counterpart.
  * out[0] = total, [1] = near-source (fine grid) solve, [2] = coarse solve, [3] = adjoint,
- * [4] = number of tile activations (node-visits / tile size) in the coarse solve,
- * [5] = coarse solve rounds. */
+ * [4] = tile activations of the coarse solve (node-visits = activations x tile nodes),
+ * [5] = coarse solve rounds (launches), [6] = summed duration of those launches (ms, HIP events
+ * on the library's stream), [7] = local relaxation sweeps summed over the activations. */
 int eik_last_stats(const eik_ctx* ctx, double* out, int32_t nout);
 
 #ifdef __cplusplus
