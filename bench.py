@@ -1,0 +1,141 @@
+#!/usr/bin/env python3
+"""Benchmark of the eikonal hot path on one MI355X.
+
+Workload: 3D Cartesian grid 201x201x101 (h = 1), v = 3 + 0.02 z + Gaussian random field
+(std 0.2 km/s, correlation 15 cells, clipped to [1.5, 6]), 16 sources at depth 60-90, 64 receivers
+on the top face, default ExtraParams (refinement around the source, df = 5, noderadius = 3).
+A step = one eiktraveltime call over the 16 sources with the model resident in HBM.
+value = source-solves/s in the reference's FP64 arithmetic.  The mixed-precision (FP32 stencil)
+rate, the gradient-evaluation rate, the roofline of the dominant kernel and a CPU baseline (the
+oracle restating the reference's heap FMM, on a bounded sample of sources) ride on the same line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "eikonalsolvers.jl_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+import numpy as np  # noqa: E402
+
+METRIC = "source-solves/s + gradient evals/s, 3D 201²×101 grid; serving p50/p99 latency"
+HBM_PEAK_GBS = 8000.0
+TILE_NODES_3D = 512
+BYTES_PER_VISIT = {"fp64": 24, "mixed": 12}  # slowness + tt read + tt write per node-visit
+
+
+def run_steps(model, src, recs, ep, steps, warmup):
+    for _ in range(warmup):
+        model.traveltime(src, recs, extraparams=ep)
+    kern_ms = acts = rounds = 0.0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        model.traveltime(src, recs, extraparams=ep)  # synchronous: returns after the device is idle
+        st = model.ctx.last_stats()
+        kern_ms += st["_kernel_ms"]
+        acts += st["tile_activations"]
+        rounds += st["rounds"]
+    dt = time.perf_counter() - t0
+    return dt, kern_ms, acts, rounds
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=0, help="sources timed on the CPU (0 = threads used)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it "
+                                   "(sources would shard by chunk, see DESIGN.md)", "n_gpus": args.gpus}))
+        return 2
+
+    import eikonalsolvers as eik
+    from eikonalsolvers.grids import ExtraParams
+    from eikonalsolvers.synthetic import c2_3d
+
+    grd, vel, src, recs, _velobs = c2_3d()
+    nsrc = src.shape[0]
+    model = eik.EikModel(vel, grd)
+
+    res = {}
+    for prec in ("fp64", "mixed"):
+        ep = ExtraParams(precision=prec)
+        dt, kern_ms, acts, rounds = run_steps(model, src, recs, ep, args.steps, args.warmup)
+        visits = acts * TILE_NODES_3D
+        res[prec] = {
+            "solves_per_s": nsrc * args.steps / dt,
+            "ms_per_step": 1e3 * dt / args.steps,
+            "kernel_ms_per_step": kern_ms / args.steps,
+            "launches_per_step": rounds / args.steps,
+            "node_visits_per_step": visits / args.steps,
+            "visits_per_node": visits / args.steps / (nsrc * grd.npts),
+            "achieved_gbs": (visits * BYTES_PER_VISIT[prec] / 1e9) / (kern_ms / 1e3) if kern_ms > 0 else None,
+        }
+
+    # parity spot check of the timed configuration against the oracle (same inputs)
+    out = {
+        "metric": METRIC,
+        "value": round(res["fp64"]["solves_per_s"], 3),
+        "unit": "source-solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(res["fp64"]["ms_per_step"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (counter-based RNG, regenerated identically on every machine)",
+        "config": {"workload": "3D Cartesian 201x201x101 h=1, GRF velocity, 16 sources, 64 top-face receivers, "
+                               "refinearoundsrc=true (df=5, noderadius=3), eiktraveltime",
+                   "nsrc": nsrc, "grid": list(grd.grsize), "nrec_per_src": 64},
+        "mixed_precision": {"solves_per_s": round(res["mixed"]["solves_per_s"], 3),
+                            "ms_per_step": round(res["mixed"]["ms_per_step"], 3), "dtype": "f32 stencil / f64 near-source"},
+    }
+    f = res["fp64"]
+    out["roofline"] = {
+        "bound": "hbm",
+        "kernel": "k_fim_round<3,double>",
+        "achieved": round(f["achieved_gbs"], 2) if f["achieved_gbs"] else None,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(f["achieved_gbs"] / HBM_PEAK_GBS, 5) if f["achieved_gbs"] else None,
+        "traffic": None,
+        "bytes_per_node_visit": BYTES_PER_VISIT["fp64"],
+        "node_visits_per_step": f["node_visits_per_step"],
+        "visits_per_node": round(f["visits_per_node"], 3),
+        "launches_per_step": f["launches_per_step"],
+        "kernel_ms_per_step": round(f["kernel_ms_per_step"], 3),
+    }
+
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+
+        nthreads = max(1, min(os.cpu_count() or 1, 16))
+        ns = args.cpu_sample or nthreads
+        ns = min(ns, nsrc)
+        t0 = time.perf_counter()
+        picks_cpu = O.traveltime(vel, grd, src[:ns], recs[:ns], extraparams=ExtraParams(), nthreads=nthreads)
+        tcpu = time.perf_counter() - t0
+        picks_gpu = model.traveltime(src[:ns], recs[:ns], extraparams=ExtraParams())
+        rel = max(float(np.max(np.abs(a - b) / np.abs(b))) for a, b in zip(picks_gpu, picks_cpu))
+        out["cpu_baseline"] = {"value": round(ns / tcpu, 4), "unit": "source-solves/s", "cores": min(nthreads, ns),
+                               "kind": "port",
+                               "sample": f"{ns} of the 16 sources of the same workload, heap FMM with refinement "
+                                         f"(C restatement of the reference), OpenMP by source chunk, {tcpu:.2f} s"}
+        out["parity_fp64_picks_max_rel_vs_oracle"] = rel
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,1058 @@
+// eik_adj.hip -- discrete-adjoint gradient of libeik (gfx950).
+//
+// The reference assembles per source the derivative matrices D_d in FMM order
+// (createsparsederivativematrices!, src/EikSolv/eikmainfwd.jl:827-946), forms
+// A = sum_d 2 diag(D_d t) D_d (calc2diagDttD, eikmaingrad.jl:751-868), and back-substitutes
+// A^T lambda = -P_B^T r (solveadjointeq, :488-535).  Here:
+//   * k_adj_prep replays every node's last stencil evaluation at the converged traveltimes
+//     (the codes the heap recorded) and stores G_d = 2*(D_d t)_i, so A_ij = G_d(i) * coef;
+//   * the triangular solve runs as tiled rounds like the forward, upwind:
+//       lambda_j = (b_j - sum_{i: j in stencil(i)} A_ij lambda_i) / A_jj,
+//     each node gathering from the <= 4*ND rows whose stencil reaches it (no atomics).  The
+//     dependency graph is acyclic, so the iteration terminates with the exact solution;
+//   * with refinement the Nh right-hand sides of the fine adjoint (computestuff_finegrid!,
+//     :540-632) only ever appear contracted with q = dpsi/du_s of the coarse solve, so one fine
+//     solve with rhs -H_B^T q gives the same gradient (calc_duh_dva_finegrid!, :636-741,
+//     calc_dus_dsr_finegrid, :1184-1231).
+#include <cstring>
+
+#include "eik_internal.h"
+
+namespace eik {
+
+constexpr int kAB = 256;
+
+// ------------------------------------------------------------------ per-node helpers
+__device__ __forceinline__ int unpack_code(uint16_t c, int d) { return (int)((c >> (3 * d)) & 7u) - 2; }
+
+// spacing along axis d at node g (arc length for the angular axes of spherical grids)
+__device__ __forceinline__ double axis_h(const InstDev& I, const int* g, int d) {
+#pragma clang fp contract(off)
+    if (I.kind == EIK_GRID2D_CART || I.kind == EIK_GRID3D_CART || d == 0) return I.h[0];
+    const double r = fma((double)g[0], I.h[0], I.o[0]);
+    if (d == 1) return r * I.dthrad;
+    const double th = fma((double)g[1], I.h[1], I.o[1]);
+    return r * sin(deg2rad(th)) * I.dphrad;
+}
+
+// coefficient of stencil position p (0 = the row's own node) for code c on axis d
+// (setcoeffderiv!, eikmaingrad.jl:1037-1090): signcod * firstord/secondord[p]
+__device__ __forceinline__ double stencil_coef(int c, int p, double h) {
+#pragma clang fp contract(off)
+    const double sg = (c > 0) ? -1.0 : 1.0;
+    const int ac = c < 0 ? -c : c;
+    double cf;
+    if (ac == 1)
+        cf = (p == 0) ? -1.0 / h : 1.0 / h;
+    else
+        cf = (p == 0) ? -3.0 / (2.0 * h) : (p == 1 ? 4.0 / (2.0 * h) : -1.0 / (2.0 * h));
+    return sg * cf;
+}
+
+template <typename T>
+__device__ __forceinline__ double tabs(T v) {
+    return fabs((double)v);
+}
+
+// Codes and G_d of every node: replay of the final stencil evaluation (eikmainfwd.jl:794-809)
+// and 2*(D_d t)_i accumulated over the stencil in acceptance order (far node first).
+template <int ND, typename T>
+__global__ void __launch_bounds__(kAB) k_adj_prep(const InstDev* __restrict__ inst, const T* __restrict__ ttpool,
+                                                  const T* __restrict__ slowpool, uint16_t* __restrict__ code_out,
+                                                  double* __restrict__ G, int64_t gstride) {
+#pragma clang fp contract(off)
+    const InstDev I = inst[blockIdx.y];
+    const int64_t n = (int64_t)I.n[0] * I.n[1] * I.n[2];
+    const T* tt = ttpool + I.tt_off;
+    const T* slow = slowpool + I.s_off;
+    const int64_t nx = I.n[0], nxy = (int64_t)I.n[0] * I.n[1];
+    for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < n; l += (int64_t)gridDim.x * blockDim.x) {
+        const int g[3] = {(int)(l % I.n[0]), (int)((l / I.n[0]) % I.n[1]), (int)(l / nxy)};
+        const T raw = tt[l];
+        const double t0 = tabs(raw);
+        uint16_t pc = 0;
+        double Gd[3] = {0.0, 0.0, 0.0};
+        if (!signbit(raw) && t0 <= I.tstop) {
+            T v1[2 * ND], v2[2 * ND];
+            unsigned f1 = 0, f2 = 0, ok1 = 0, ok2 = 0;
+#pragma unroll
+            for (int ax = 0; ax < ND; ax++) {
+#pragma unroll
+                for (int s = 0; s < 2; s++) {
+                    const int k = 2 * ax + s;
+                    int nb[3] = {g[0], g[1], g[2]};
+                    nb[ax] += (s == 0) ? 1 : -1;
+                    const bool in1 = nb[ax] >= 0 && nb[ax] <= I.n[ax] - 1;
+                    bool in2 = in1;
+#pragma unroll
+                    for (int d = 0; d < ND; d++) in2 = in2 && nb[d] >= 1 && nb[d] <= I.n[d] - 2;
+                    T a = tinf<T>(), b = tinf<T>();
+                    if (in1) a = tt[nb[0] + nx * nb[1] + nxy * nb[2]];
+                    if (in2) {
+                        int nb2[3] = {g[0], g[1], g[2]};
+                        nb2[ax] += (s == 0) ? 2 : -2;
+                        b = tt[nb2[0] + nx * nb2[1] + nxy * nb2[2]];
+                    }
+                    ok1 |= (unsigned)in1 << k;
+                    ok2 |= (unsigned)in2 << k;
+                    f1 |= (unsigned)signbit(a) << k;
+                    f2 |= (unsigned)signbit(b) << k;
+                    v1[k] = fabs(a);
+                    v2[k] = fabs(b);
+                }
+            }
+            T dh[3];
+            node_spacing<T>(I, g, dh);
+            int code[3] = {0, 0, 0};
+            (void)replay<ND, T>(v1, v2, f1, f2, ok1, ok2, slow[l], dh, code);
+#pragma unroll
+            for (int d = 0; d < ND; d++) {
+                const int c = code[d];
+                pc |= (uint16_t)((c + 2) << (3 * d));
+                if (c == 0) continue;
+                const int ac = c < 0 ? -c : c;
+                const int sg = (c > 0) ? -1 : 1;
+                double tmp1 = 0.0;
+                for (int p = ac; p >= 0; p--) {
+                    int q[3] = {g[0], g[1], g[2]};
+                    q[d] += sg * p;
+                    const double cf = stencil_coef(c, p, axis_h(I, q, d));
+                    tmp1 += cf * tabs(tt[q[0] + nx * q[1] + nxy * q[2]]);
+                }
+                Gd[d] = 2.0 * tmp1;
+            }
+        } else {
+#pragma unroll
+            for (int d = 0; d < ND; d++) pc |= (uint16_t)(2 << (3 * d));
+        }
+        code_out[I.tt_off + l] = pc;
+#pragma unroll
+        for (int d = 0; d < ND; d++) G[d * gstride + I.tt_off + l] = Gd[d];
+    }
+}
+
+// Coefficient A_ij with which row i (at offset o = i - j along axis d) references node j,
+// or 0 when j is not in i's stencil.
+__device__ __forceinline__ double contrib(const InstDev& I, const int* gi, int d, int o, uint16_t ci, double Gi) {
+    const int c = unpack_code(ci, d);
+    if (c == 0) return 0.0;
+    if ((c > 0) != (o > 0)) return 0.0;
+    const int ac = c < 0 ? -c : c, ao = o < 0 ? -o : o;
+    if (ao > ac) return 0.0;
+    return Gi * stencil_coef(c, ao, axis_h(I, gi, d));
+}
+
+// diagonal A_jj = sum_d G_d(j) * coef(self)
+template <int ND>
+__device__ __forceinline__ double adj_diag(const InstDev& I, const int* g, uint16_t cj, const double* Gj) {
+#pragma clang fp contract(off)
+    double a = 0.0;
+#pragma unroll
+    for (int d = 0; d < ND; d++) {
+        const int c = unpack_code(cj, d);
+        if (c == 0) continue;
+        a += Gj[d] * stencil_coef(c, 0, axis_h(I, g, d));
+    }
+    return a;
+}
+
+template <int ND>
+struct AdjTile;
+template <>
+struct AdjTile<2> {
+    static constexpr int TX = 16, TY = 16, TZ = 1, H = 2, LX = 20, LY = 20, LZ = 1, NP = 1, MAXIT = 128;
+};
+template <>
+struct AdjTile<3> {
+    static constexpr int TX = 8, TY = 8, TZ = 8, H = 2, LX = 12, LY = 12, LZ = 12, NP = 2, MAXIT = 96;
+};
+
+// one adjoint round over the listed tiles
+template <int ND>
+__global__ void __launch_bounds__(kAB) k_adj_round(const int64_t* __restrict__ list, const InstDev* __restrict__ inst,
+                                                   int ninst, const uint16_t* __restrict__ codep,
+                                                   const double* __restrict__ Gp, int64_t gstride,
+                                                   const double* __restrict__ bp, double* __restrict__ lamp,
+                                                   uint8_t* __restrict__ flags) {
+#pragma clang fp contract(off)
+    using S = AdjTile<ND>;
+    constexpr int NL = S::LX * S::LY * S::LZ;
+    __shared__ double s_lam[NL];
+    __shared__ double s_G[ND][NL];
+    __shared__ uint16_t s_code[NL];
+    __shared__ int s_faces;
+    const int tid = threadIdx.x;
+    const int64_t gid = list[blockIdx.x];
+    int lo = 0, hi = ninst - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (inst[mid].tile_base <= gid)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const InstDev I = inst[lo];
+    const int64_t tl = gid - I.tile_base;
+    const int tix = (int)(tl % I.t[0]), tiy = (int)((tl / I.t[0]) % I.t[1]);
+    const int tiz = ND == 3 ? (int)(tl / ((int64_t)I.t[0] * I.t[1])) : 0;
+    const int ox = tix * S::TX, oy = tiy * S::TY, oz = tiz * S::TZ;
+    const int64_t nx = I.n[0], nxy = (int64_t)I.n[0] * I.n[1];
+    const uint16_t* code = codep + I.tt_off;
+    const double* b = bp + I.tt_off;
+    double* lam = lamp + I.tt_off;
+    if (tid == 0) s_faces = 0;
+    for (int c = tid; c < NL; c += kAB) {
+        const int lx = c % S::LX, ly = (c / S::LX) % S::LY, lz = ND == 3 ? c / (S::LX * S::LY) : 0;
+        int nout = (lx < S::H || lx >= S::H + S::TX) + (ly < S::H || ly >= S::H + S::TY);
+        if (ND == 3) nout += (lz < S::H || lz >= S::H + S::TZ);
+        double lv = 0.0;
+        uint16_t cv = 0x1B6;  // all codes zero ((0+2) in every 3-bit field)
+        double gv[3] = {0.0, 0.0, 0.0};
+        if (nout <= 1) {
+            const int gx = ox + lx - S::H, gy = oy + ly - S::H, gz = ND == 3 ? oz + lz - S::H : 0;
+            if (gx >= 0 && gx < I.n[0] && gy >= 0 && gy < I.n[1] && (ND == 2 || (gz >= 0 && gz < I.n[2]))) {
+                const int64_t l = gx + nx * gy + nxy * gz;
+                lv = lam[l];
+                cv = code[l];
+#pragma unroll
+                for (int d = 0; d < ND; d++) gv[d] = Gp[d * gstride + I.tt_off + l];
+            }
+        }
+        s_lam[c] = lv;
+        s_code[c] = cv;
+#pragma unroll
+        for (int d = 0; d < ND; d++) s_G[d][c] = gv[d];
+    }
+    __syncthreads();
+
+    constexpr int NS = 4 * ND;  // candidate rows: axis d, offset +-1, +-2
+    double w[S::NP][NS];
+    int off[NS];
+    double diag[S::NP], bj[S::NP], l0[S::NP];
+    bool act[S::NP];
+    int lidx[S::NP], gxyz[S::NP][3];
+    const int strides[3] = {1, S::LX, S::LX * S::LY};
+#pragma unroll
+    for (int k = 0; k < NS; k++) {
+        const int d = k / 4, o = (k % 4 < 2) ? ((k % 2) ? -1 : 1) : ((k % 2) ? -2 : 2);
+        off[k] = o * strides[d];
+    }
+#pragma unroll
+    for (int p = 0; p < S::NP; p++) {
+        const int lx = tid % S::TX, ly = (tid / S::TX) % S::TY;
+        const int lz = ND == 3 ? (tid / (S::TX * S::TY)) + p * (S::TZ / S::NP) : 0;
+        const int g[3] = {ox + lx, oy + ly, oz + lz};
+        gxyz[p][0] = g[0];
+        gxyz[p][1] = g[1];
+        gxyz[p][2] = g[2];
+        lidx[p] = (lx + S::H) + S::LX * ((ly + S::H) + S::LY * (ND == 3 ? lz + S::H : 0));
+        const bool inside = g[0] < I.n[0] && g[1] < I.n[1] && (ND == 2 || g[2] < I.n[2]);
+        const uint16_t cj = s_code[lidx[p]];
+        bool hasrow = false;
+#pragma unroll
+        for (int d = 0; d < ND; d++) hasrow = hasrow || unpack_code(cj, d) != 0;
+        act[p] = inside && hasrow;
+        diag[p] = 1.0;
+        bj[p] = 0.0;
+#pragma unroll
+        for (int k = 0; k < NS; k++) w[p][k] = 0.0;
+        if (act[p]) {
+            double Gj[3] = {0, 0, 0};
+#pragma unroll
+            for (int d = 0; d < ND; d++) Gj[d] = s_G[d][lidx[p]];
+            diag[p] = adj_diag<ND>(I, g, cj, Gj);
+            bj[p] = b[g[0] + nx * g[1] + nxy * g[2]];
+#pragma unroll
+            for (int k = 0; k < NS; k++) {
+                const int d = k / 4, o = (k % 4 < 2) ? ((k % 2) ? -1 : 1) : ((k % 2) ? -2 : 2);
+                int gi[3] = {g[0], g[1], g[2]};
+                gi[d] += o;
+                if (gi[d] < 0 || gi[d] > I.n[d] - 1) continue;
+                const int li = lidx[p] + off[k];
+                w[p][k] = contrib(I, gi, d, o, s_code[li], s_G[d][li]);
+            }
+        }
+        l0[p] = s_lam[lidx[p]];
+    }
+    __syncthreads();
+
+    int it = 0;
+    for (; it < S::MAXIT; it++) {
+        int any = 0;
+#pragma unroll
+        for (int p = 0; p < S::NP; p++) {
+            if (!act[p]) continue;
+            double x = bj[p];
+#pragma unroll
+            for (int k = 0; k < NS; k++)
+                if (w[p][k] != 0.0) x -= s_lam[lidx[p] + off[k]] * w[p][k];
+            x = x / diag[p];
+            if (x != s_lam[lidx[p]]) {
+                s_lam[lidx[p]] = x;
+                any = 1;
+            }
+        }
+        if (!__syncthreads_or(any)) break;
+    }
+    int faces = 0;
+#pragma unroll
+    for (int p = 0; p < S::NP; p++) {
+        if (!act[p]) continue;
+        const double v = s_lam[lidx[p]];
+        if (v != l0[p]) {
+            const int gx = gxyz[p][0], gy = gxyz[p][1], gz = gxyz[p][2];
+            lam[gx + nx * gy + nxy * gz] = v;
+            const int lx = gx - ox, ly = gy - oy, lz = gz - oz;
+            if (lx < 2) faces |= 1;
+            if (lx >= S::TX - 2) faces |= 2;
+            if (ly < 2) faces |= 4;
+            if (ly >= S::TY - 2) faces |= 8;
+            if (ND == 3) {
+                if (lz < 2) faces |= 16;
+                if (lz >= S::TZ - 2) faces |= 32;
+            }
+        }
+    }
+    if (faces) atomicOr(&s_faces, faces);
+    __syncthreads();
+    if (tid == 0) {
+        const int f = s_faces;
+        uint8_t* fl = flags + I.tile_base;
+        const int64_t t0 = I.t[0], t01 = (int64_t)I.t[0] * I.t[1];
+        if ((f & 1) && tix > 0) fl[tl - 1] = 1;
+        if ((f & 2) && tix < I.t[0] - 1) fl[tl + 1] = 1;
+        if ((f & 4) && tiy > 0) fl[tl - t0] = 1;
+        if ((f & 8) && tiy < I.t[1] - 1) fl[tl + t0] = 1;
+        if (ND == 3) {
+            if ((f & 16) && tiz > 0) fl[tl - t01] = 1;
+            if ((f & 32) && tiz < I.t[2] - 1) fl[tl + t01] = 1;
+        }
+        if (it >= S::MAXIT) fl[tl] = 1;
+    }
+}
+
+// sum over the rows that reference node j of A_ij * lambda_i (dpsi/du_s at source points:
+// transpose(sum(twoDttDS)) * lambda, eikmaingrad.jl:341-344)
+template <int ND>
+__device__ double gather_rows(const InstDev& I, const int* g, const uint16_t* code, const double* Gp, int64_t gstride,
+                              const double* lam) {
+#pragma clang fp contract(off)
+    const int64_t nx = I.n[0], nxy = (int64_t)I.n[0] * I.n[1];
+    double acc = 0.0;
+    for (int k = 0; k < 4 * ND; k++) {
+        const int d = k / 4, o = (k % 4 < 2) ? ((k % 2) ? -1 : 1) : ((k % 2) ? -2 : 2);
+        int gi[3] = {g[0], g[1], g[2]};
+        gi[d] += o;
+        if (gi[d] < 0 || gi[d] > I.n[d] - 1) continue;
+        const int64_t li = gi[0] + nx * gi[1] + nxy * gi[2];
+        const double a = contrib(I, gi, d, o, code[I.tt_off + li], Gp[d * gstride + I.tt_off + li]);
+        if (a != 0.0) acc += a * lam[I.tt_off + li];
+    }
+    return acc;
+}
+
+// receivers: picks, residual factors, rhs b = -P^T (pd / std^2), per-source misfit terms
+template <typename T>
+__global__ void k_adj_rhs(const double* __restrict__ rec, const int* __restrict__ rsrc, int64_t nrec,
+                          const InstDev* __restrict__ inst, const T* __restrict__ ttpool, int nd, int kind,
+                          const double* __restrict__ pobs, const double* __restrict__ sobs, double* __restrict__ bpool,
+                          double* __restrict__ mterm, uint8_t* __restrict__ flags) {
+#pragma clang fp contract(off)
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrec) return;
+    const InstDev I = inst[rsrc[r]];
+    double cf[8];
+    int64_t idx[8];
+    const double pick = interp_receiver<T>(ttpool + I.tt_off, I, nd, kind, rec + r * nd, cf, idx);
+    const double pd = pick - pobs[r];
+    const double s2 = sobs[r] * sobs[r];
+    mterm[r] = (pd * pd) / s2;
+    const double fact = pd / s2;
+    const int nc = nd == 2 ? 4 : 8;
+    for (int c = 0; c < nc; c++) {
+        atomicAdd(&bpool[I.tt_off + idx[c]], -cf[c] * fact);
+        const int64_t l = idx[c];
+        const int g[3] = {(int)(l % I.n[0]), (int)((l / I.n[0]) % I.n[1]), (int)(l / ((int64_t)I.n[0] * I.n[1]))};
+        seed_tiles(I, nd, g, flags);
+    }
+}
+
+// per-source misfit sum in receiver order, then the total in source order (calcgradsomesrc,
+// eikmaingrad.jl:237: totmisf += 0.5*misfit1src)
+__global__ void k_misfit(const double* __restrict__ mterm, const int64_t* __restrict__ rec_off, int64_t nsrc,
+                         double* __restrict__ out) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double tot = 0.0;
+    for (int64_t s = 0; s < nsrc; s++) {
+        double m = 0.0;
+        for (int64_t r = rec_off[s]; r < rec_off[s + 1]; r++) m += mterm[r];
+        tot += 0.5 * m;
+    }
+    out[0] = tot;
+}
+
+// source points of every instance: dpsiup = sum_i A^S_iq lambda_i, dpsidus = (P_A^T fact)_q
+template <int ND>
+__global__ void k_srcterms(const SrcPt* __restrict__ pts, int npts, const InstDev* __restrict__ inst,
+                           const uint16_t* __restrict__ code, const double* __restrict__ Gp, int64_t gstride,
+                           const double* __restrict__ lam, const double* __restrict__ b, double* __restrict__ dpsiup,
+                           double* __restrict__ dpsidus) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= npts) return;
+    const SrcPt P = pts[q];
+    const InstDev I = inst[P.inst];
+    const int64_t l = P.node;
+    const int g[3] = {(int)(l % I.n[0]), (int)((l / I.n[0]) % I.n[1]), (int)(l / ((int64_t)I.n[0] * I.n[1]))};
+    dpsiup[q] = gather_rows<ND>(I, g, code, Gp, gstride, lam);
+    dpsidus[q] = -b[I.tt_off + l];
+}
+
+template __global__ void k_adj_prep<2, double>(const InstDev*, const double*, const double*, uint16_t*, double*, int64_t);
+template __global__ void k_adj_prep<3, double>(const InstDev*, const double*, const double*, uint16_t*, double*, int64_t);
+template __global__ void k_adj_prep<2, float>(const InstDev*, const float*, const float*, uint16_t*, double*, int64_t);
+template __global__ void k_adj_prep<3, float>(const InstDev*, const float*, const float*, uint16_t*, double*, int64_t);
+
+}  // namespace eik
+
+namespace eik {
+
+// scatter of per-point results into a pool-shaped array (value at the point's node)
+__global__ void k_scatter_pts(const SrcPt* __restrict__ pts, int npts, const InstDev* __restrict__ inst,
+                              const double* __restrict__ val, double* __restrict__ pool) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < npts) pool[inst[pts[q].inst].tt_off + pts[q].node] = val[q];
+}
+
+// T2Vb (eikmaingrad.jl:417-423): 2*lambda/v^3 at every non-source node of every source image
+template <typename T>
+__global__ void k_grad_img(const InstDev* __restrict__ inst, const T* __restrict__ ttpool,
+                           const double* __restrict__ lam, const double* __restrict__ vel, double* __restrict__ img,
+                           int64_t N) {
+#pragma clang fp contract(off)
+    const InstDev I = inst[blockIdx.y];
+    for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < N; l += (int64_t)gridDim.x * blockDim.x) {
+        const T raw = ttpool[I.tt_off + l];
+        double g = 0.0;
+        if (!signbit(raw)) {
+            const double v = vel[l];
+            g = 2.0 * lam[I.tt_off + l] / (v * v * v);
+        }
+        img[I.tt_off + l] = g;
+    }
+}
+
+// dus/dva = -diag(dist/vel^2) at the coarse source-box corners (no refinement, :428-479)
+__global__ void k_src_own(const SrcPt* __restrict__ pts, int npts, const InstDev* __restrict__ inst,
+                          const double* __restrict__ vel, const double* __restrict__ srcxyz, int nd, int kind,
+                          const double* __restrict__ dpsiup, const double* __restrict__ dpsidus,
+                          double* __restrict__ t2, double* __restrict__ t1) {
+#pragma clang fp contract(off)
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= npts) return;
+    const SrcPt P = pts[q];
+    const InstDev I = inst[P.inst];
+    const int64_t l = P.node;
+    const int g[3] = {(int)(l % I.n[0]), (int)((l / I.n[0]) % I.n[1]), (int)(l / ((int64_t)I.n[0] * I.n[1]))};
+    double xyz[3];
+    for (int d = 0; d < nd; d++) xyz[d] = fma((double)g[d], I.h[d], I.o[d]);
+    const double dist = distance2points(kind, nd, xyz, srcxyz + P.inst * nd);
+    const double v = vel[l];
+    const double dd = -dist * (1.0 / (v * v));
+    t2[q] = dd * dpsiup[q];
+    t1[q] = dd * dpsidus[q];
+}
+
+// fine rhs -H_B^T q and H_A^T q for the transferred points (q = dpsiup + dpsidus, coarse)
+__global__ void k_fine_rhs(const SrcPt* __restrict__ xp, int nxp, const InstDev* __restrict__ finst,
+                           const double* __restrict__ ttf, const double* __restrict__ dpsiup,
+                           const double* __restrict__ dpsidus, double* __restrict__ bf, double* __restrict__ haq,
+                           uint8_t* __restrict__ flags, int nd) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nxp) return;
+    const SrcPt P = xp[q];
+    const InstDev I = finst[P.inst];
+    const int64_t lf = P.aux;
+    const double qv = dpsiup[q] + dpsidus[q];
+    if (signbit(ttf[I.tt_off + lf])) {
+        haq[I.tt_off + lf] = qv;
+    } else {
+        bf[I.tt_off + lf] = -qv;
+        const int g[3] = {(int)(lf % I.n[0]), (int)((lf / I.n[0]) % I.n[1]), (int)(lf / ((int64_t)I.n[0] * I.n[1]))};
+        seed_tiles(I, nd, g, flags);
+    }
+}
+
+// z_p of the fine grid: 2*mu/w^3 (T2Wb) at accepted non-source nodes, dtau/dw * (dgi + H_A^T q)
+// (T2Wa + T1Wa) at the fine source-box corners
+__global__ void k_fine_z(const InstDev* __restrict__ finst, const FineDev* __restrict__ fine,
+                         const double* __restrict__ ttf, const double* __restrict__ mu, const double* __restrict__ dgi,
+                         const double* __restrict__ haq, const double* __restrict__ velc,
+                         const double* __restrict__ srcxyz, int nd, int kind, double* __restrict__ z) {
+#pragma clang fp contract(off)
+    const int f = blockIdx.y;
+    const InstDev I = finst[f];
+    const FineDev F = fine[f];
+    const int64_t n = (int64_t)I.n[0] * I.n[1] * I.n[2];
+    for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < n; l += (int64_t)gridDim.x * blockDim.x) {
+        const double raw = ttf[I.tt_off + l];
+        const int g[3] = {(int)(l % I.n[0]), (int)((l / I.n[0]) % I.n[1]), (int)(l / ((int64_t)I.n[0] * I.n[1]))};
+        int cg[3] = {0, 0, 0};
+        for (int d = 0; d < nd; d++) cg[d] = F.i1c[d] + nn_coarse(g[d], F.df);
+        const double w = velc[cg[0] + (int64_t)F.nc[0] * (cg[1] + (int64_t)F.nc[1] * cg[2])];
+        double zv = 0.0;
+        if (signbit(raw)) {
+            double xyz[3];
+            for (int d = 0; d < nd; d++) xyz[d] = fma((double)g[d], I.h[d], I.o[d]);
+            const double dist = distance2points(kind, nd, xyz, srcxyz + f * nd);
+            const double dd = -dist * (1.0 / (w * w));
+            zv = dd * (dgi[I.tt_off + l] + haq[I.tt_off + l]);
+        } else if (fabs(raw) <= I.tstop) {
+            zv = 2.0 * mu[I.tt_off + l] / (w * w * w);
+        }
+        z[I.tt_off + l] = zv;
+    }
+}
+
+// coarse source point a gathers z over the accepted fine nodes whose nearest coarse node is a
+// (dwq_dva keeps only coarse nodes that are transferred points, eikmaingrad.jl:717-733)
+__global__ void k_fine_gather(const SrcPt* __restrict__ xp, int nxp, const InstDev* __restrict__ finst,
+                              const FineDev* __restrict__ fine, const double* __restrict__ ttf,
+                              const double* __restrict__ z, double* __restrict__ t2, double* __restrict__ t1, int nd) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nxp) return;
+    const SrcPt P = xp[q];
+    const InstDev I = finst[P.inst];
+    const FineDev F = fine[P.inst];
+    const int64_t lf = P.aux;
+    const int g[3] = {(int)(lf % I.n[0]), (int)((lf / I.n[0]) % I.n[1]), (int)(lf / ((int64_t)I.n[0] * I.n[1]))};
+    int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0}, w[3] = {0, 0, 0};
+    for (int d = 0; d < nd; d++) {
+        w[d] = g[d] / F.df;
+        lo[d] = max(0, (w[d] - 1) * F.df);
+        hi[d] = min(I.n[d] - 1, (w[d] + 1) * F.df);
+    }
+    double acc = 0.0;
+    for (int k = lo[2]; k <= hi[2]; k++)
+        for (int j = lo[1]; j <= hi[1]; j++)
+            for (int i = lo[0]; i <= hi[0]; i++) {
+                if (nn_coarse(i, F.df) != w[0] || nn_coarse(j, F.df) != w[1]) continue;
+                if (nd == 3 && nn_coarse(k, F.df) != w[2]) continue;
+                const int64_t l = i + (int64_t)I.n[0] * (j + (int64_t)I.n[1] * k);
+                if (!(fabs(ttf[I.tt_off + l]) <= I.tstop)) continue;
+                acc += z[I.tt_off + l];
+            }
+    t2[q] = acc;
+    t1[q] = 0.0;
+}
+
+// taper factor of a source at node g (smoothgradaroundsrc!, utils.jl:121-338)
+__device__ __forceinline__ double taper_factor(const TaperDev& T, const InstDev& I, int nd, int kind, const int* g) {
+#pragma clang fp contract(off)
+    if (!T.on) return 1.0;
+    for (int d = 0; d < nd; d++)
+        if (g[d] < T.lo[d] || g[d] > T.hi[d]) return 1.0;
+    double r;
+    if (kind == EIK_GRID2D_SPHERE) {
+        const double xs = T.src[0] * cos(deg2rad(T.src[1])), ys = T.src[0] * sin(deg2rad(T.src[1]));
+        const double rr = fma((double)g[0], I.h[0], I.o[0]), th = fma((double)g[1], I.h[1], I.o[1]);
+        const double xc = rr * cos(deg2rad(th)), yc = rr * sin(deg2rad(th));
+        r = sqrt((xc - xs) * (xc - xs) + (ys - yc) * (ys - yc));
+    } else if (nd == 2) {
+        const double xc = fma((double)g[0], I.h[0], I.o[0]), yc = fma((double)g[1], I.h[1], I.o[1]);
+        r = sqrt((xc - T.src[0]) * (xc - T.src[0]) + (T.src[1] - yc) * (T.src[1] - yc));
+    } else {
+        const double xc = fma((double)g[0], I.h[0], I.o[0]), yc = fma((double)g[1], I.h[1], I.o[1]),
+                     zc = fma((double)g[2], I.h[2], I.o[2]);
+        r = sqrt((xc - T.src[0]) * (xc - T.src[0]) + (T.src[1] - yc) * (T.src[1] - yc) +
+                 (T.src[2] - zc) * (T.src[2] - zc));
+    }
+    return (r <= T.rmax) ? r / T.rmax : 1.0;
+}
+
+// The reference reuses one per-chunk gradient buffer across sources and only overwrites the
+// non-source nodes (T2Vb), so a source node starts from the previous source's tapered value
+// before T2Va and T1Va are added (calcgradsomesrc, eikmaingrad.jl:182-247).  Sequential over
+// sources, parallel over each source's points.
+__global__ void k_grad_quirk(const SrcPt* __restrict__ pts, const int* __restrict__ poff, int nsrc,
+                             const InstDev* __restrict__ inst, const TaperDev* __restrict__ tap, int nd, int kind,
+                             const double* __restrict__ t2, const double* __restrict__ t1, double* __restrict__ img) {
+#pragma clang fp contract(off)
+    for (int s = 0; s < nsrc; s++) {
+        for (int q = poff[s] + threadIdx.x; q < poff[s + 1]; q += blockDim.x) {
+            const InstDev I = inst[s];
+            const int64_t l = pts[q].node;
+            double prev = 0.0;
+            if (s > 0) {
+                const int g[3] = {(int)(l % I.n[0]), (int)((l / I.n[0]) % I.n[1]),
+                                  (int)(l / ((int64_t)I.n[0] * I.n[1]))};
+                prev = taper_factor(tap[s - 1], I, nd, kind, g) * img[inst[s - 1].tt_off + l];
+            }
+            img[I.tt_off + l] = (prev + t2[q]) + t1[q];
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_taper(const InstDev* __restrict__ inst, const TaperDev* __restrict__ tap, int nd, int kind,
+                        double* __restrict__ img) {
+    const int s = blockIdx.x;
+    const TaperDev T = tap[s];
+    if (!T.on) return;
+    const InstDev I = inst[s];
+    int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    for (int d = 0; d < nd; d++) {
+        lo[d] = max(T.lo[d], 0);
+        hi[d] = min(T.hi[d], I.n[d] - 1);
+        if (hi[d] < lo[d]) return;
+    }
+    const int ex = hi[0] - lo[0] + 1, ey = hi[1] - lo[1] + 1, ez = hi[2] - lo[2] + 1;
+    for (int c = threadIdx.x; c < ex * ey * ez; c += blockDim.x) {
+        const int g[3] = {lo[0] + c % ex, lo[1] + (c / ex) % ey, lo[2] + c / (ex * ey)};
+        const double a = taper_factor(T, I, nd, kind, g);
+        if (a != 1.0) img[I.tt_off + g[0] + (int64_t)I.n[0] * (g[1] + (int64_t)I.n[1] * g[2])] *= a;
+    }
+}
+
+// gradvelall .+= gradvel1, source by source (eikmaingrad.jl:246)
+__global__ void k_sum_sources(const double* __restrict__ img, int nsrc, int64_t N, double* __restrict__ out) {
+#pragma clang fp contract(off)
+    const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= N) return;
+    double a = 0.0;
+    for (int s = 0; s < nsrc; s++) a += img[(int64_t)s * N + l];
+    out[l] = a;
+}
+
+// zero-padded Gaussian smoothing of the summed gradient (smoothgradient, utils.jl:73-96)
+__global__ void k_smooth(const double* __restrict__ in, double* __restrict__ out, const double* __restrict__ ker,
+                         int l, int nd, int n0, int n1, int n2) {
+#pragma clang fp contract(off)
+    const int64_t N = (int64_t)n0 * n1 * n2;
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= N) return;
+    const int g[3] = {(int)(idx % n0), (int)((idx / n0) % n1), (int)(idx / ((int64_t)n0 * n1))};
+    const int m = l / 2;
+    const int n[3] = {n0, n1, n2};
+    int keep[3], nk = 0;
+    for (int d = 0; d < nd; d++)
+        if (n[d] != 1) keep[nk++] = d;
+    int64_t kn = 1;
+    for (int d = 0; d < nk; d++) kn *= l;
+    double acc = 0.0;
+    for (int64_t q = 0; q < kn; q++) {
+        int64_t r = q;
+        int c[3] = {g[0], g[1], g[2]};
+        bool ok = true;
+        for (int d = 0; d < nk; d++) {
+            const int o = (int)(r % l) - m;
+            r /= l;
+            c[keep[d]] += o;
+            if (c[keep[d]] < 0 || c[keep[d]] > n[keep[d]] - 1) ok = false;
+        }
+        if (ok) acc += in[c[0] + (int64_t)n0 * (c[1] + (int64_t)n1 * c[2])] * ker[q];
+    }
+    out[idx] = acc;
+}
+
+// partderivttsrcpos (eikmaingrad.jl:1235-1273)
+__device__ __forceinline__ void partder(int nd, const double* xyzpt, const double* src, double vel, double hgrid,
+                                        double* der) {
+#pragma clang fp contract(off)
+    double xs[3];
+    for (int d = 0; d < nd; d++) {
+        double al = fmod(src[d], hgrid);
+        if (al < 0) al += hgrid;
+        xs[d] = (al <= 2.220446049250313e-16) ? src[d] + 1e-6 * hgrid : src[d];
+    }
+    double s2 = 0.0;
+    for (int d = 0; d < nd; d++) s2 += (xyzpt[d] - xs[d]) * (xyzpt[d] - xs[d]);
+    const double denom = vel * sqrt(s2);
+    for (int d = 0; d < nd; d++) der[d] = (denom == 0.0) ? 0.0 : -(xyzpt[d] - xs[d]) / denom;
+}
+
+// gradient w.r.t. the source position (calcgrads_singlesrc!, :362-399); one thread per source.
+// Without refinement over the coarse corners; with refinement over the fine corners, where
+// dgi + H_A^T q equals (du_h/dtau_s)^T q + H_A^T q of calc_dus_dsr_finegrid.
+__global__ void k_gradsrc(int nsrc, int nd, const SrcPt* __restrict__ pts, const int* __restrict__ poff,
+                          const InstDev* __restrict__ inst, const double* __restrict__ vel,
+                          const double* __restrict__ srcxyz, const double* __restrict__ a1,
+                          const double* __restrict__ a2, const FineDev* __restrict__ fine,
+                          const double* __restrict__ velc, const double* __restrict__ haq, double* __restrict__ out) {
+#pragma clang fp contract(off)
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsrc) return;
+    double A[3] = {0, 0, 0}, B[3] = {0, 0, 0};
+    for (int q = poff[s]; q < poff[s + 1]; q++) {
+        const SrcPt P = pts[q];
+        const InstDev I = inst[P.inst];
+        const int64_t l = P.node;
+        const int g[3] = {(int)(l % I.n[0]), (int)((l / I.n[0]) % I.n[1]), (int)(l / ((int64_t)I.n[0] * I.n[1]))};
+        double xyz[3], v;
+        for (int d = 0; d < nd; d++) xyz[d] = fma((double)g[d], I.h[d], I.o[d]);
+        if (fine) {
+            const FineDev F = fine[P.inst];
+            int cg[3] = {0, 0, 0};
+            for (int d = 0; d < nd; d++) cg[d] = F.i1c[d] + nn_coarse(g[d], F.df);
+            v = velc[cg[0] + (int64_t)F.nc[0] * (cg[1] + (int64_t)F.nc[1] * cg[2])];
+        } else {
+            v = vel[l];
+        }
+        double der[3];
+        partder(nd, xyz, srcxyz + s * nd, v, I.h[0], der);
+        if (fine) {
+            const double w = a1[q] + haq[I.tt_off + l];
+            for (int d = 0; d < nd; d++) A[d] += der[d] * w;
+        } else {
+            for (int d = 0; d < nd; d++) {
+                A[d] += der[d] * a1[q];
+                B[d] += der[d] * a2[q];
+            }
+        }
+    }
+    for (int d = 0; d < nd; d++) out[s * nd + d] = A[d] + B[d];
+}
+
+// ================================================================= host driver
+
+static unsigned nb(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+template <int ND>
+static int adj_run_t(eik_ctx* ctx, const InstDev* d_inst, int ninst, int64_t ntiles, const uint16_t* code,
+                     const double* G, int64_t gstride, const double* b, double* lam, FimStats* fs) {
+    DevBuf& list = ctx->buf("fim_list");
+    if (list.ensure((size_t)ntiles * sizeof(int64_t))) return ctx->fail(EIK_ENOMEM, "out of device memory (tile list)");
+    uint8_t* flags = (uint8_t*)ctx->buf("fim_flags").p;
+    unsigned* d_cnt = (unsigned*)ctx->d_small;
+    unsigned* h_cnt = (unsigned*)ctx->h_small;
+    FimStats st{};
+    for (;;) {
+        CK(hipMemsetAsync(d_cnt, 0, sizeof(unsigned), ctx->stream));
+        k_compact<<<nb(ntiles, 256), 256, 0, ctx->stream>>>(flags, ntiles, (int64_t*)list.p, d_cnt);
+        CK(hipMemcpyAsync(h_cnt, d_cnt, sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
+        CK(hipStreamSynchronize(ctx->stream));
+        const unsigned nact = h_cnt[0];
+        if (nact == 0) break;
+        if (++st.rounds > 50000) return ctx->fail(EIK_ENOCONV, "adjoint solve did not converge");
+        st.activations += nact;
+        k_adj_round<ND><<<nact, kAB, 0, ctx->stream>>>((const int64_t*)list.p, d_inst, ninst, code, G, gstride, b, lam,
+                                                        flags);
+        CK(hipGetLastError());
+    }
+    if (fs) *fs = st;
+    return EIK_OK;
+}
+
+static int adj_run(eik_ctx* ctx, int nd, const InstDev* d_inst, int ninst, int64_t ntiles, const uint16_t* code,
+                   const double* G, int64_t gstride, const double* b, double* lam, FimStats* fs) {
+    return nd == 2 ? adj_run_t<2>(ctx, d_inst, ninst, ntiles, code, G, gstride, b, lam, fs)
+                   : adj_run_t<3>(ctx, d_inst, ninst, ntiles, code, G, gstride, b, lam, fs);
+}
+
+template <typename T>
+static void launch_prep(int nd, hipStream_t st, const InstDev* inst, int ninst, const void* tt, const void* slow,
+                        uint16_t* code, double* G, int64_t gstride) {
+    dim3 grid(256, (unsigned)ninst);
+    if (nd == 2)
+        k_adj_prep<2, T><<<grid, kAB, 0, st>>>(inst, (const T*)tt, (const T*)slow, code, G, gstride);
+    else
+        k_adj_prep<3, T><<<grid, kAB, 0, st>>>(inst, (const T*)tt, (const T*)slow, code, G, gstride);
+}
+
+static void taper_box(const eik_grid& g, const double* src, int R, TaperDev& T) {
+    std::memset(&T, 0, sizeof(T));
+    T.on = R > 0;
+    for (int d = 0; d < g.ndim; d++) T.src[d] = src[d];
+    T.rmax = R * g.d[0];
+    if (!T.on) return;
+    int c[8][3];
+    const int nc = find_enclosing_box(g, src, c);
+    if (nc == 1) {
+        for (int d = 0; d < g.ndim; d++) {
+            T.lo[d] = c[0][d] - R;
+            T.hi[d] = c[0][d] + R;
+        }
+    } else {
+        T.lo[0] = c[0][0] - R;
+        T.hi[0] = c[1][0] + R;
+        T.lo[1] = c[0][1] - R;
+        T.hi[1] = c[2][1] + R;
+        if (g.ndim == 3) {  // the reference builds the k range from the i index (utils.jl:257-258)
+            T.lo[2] = c[0][0] - R;
+            T.hi[2] = c[3][0] + R;
+        }
+    }
+}
+
+int gradient_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, const eik_params* P, int whichgrad,
+                   const double* pickobs, const double* stdobs, double* gradvel, double* gradsrc, double* misf) {
+    const int nd = M->g.ndim, kind = M->g.kind, prec = P->precision;
+    const int64_t N = M->N, nsrc = S.nsrc, nrec = S.nrec();
+    const bool refine = P->refinearoundsrc != 0;
+    hipStream_t st = ctx->stream;
+    int rc = forward_solve(ctx, M, S, prec, P);
+    if (rc) return rc;
+    Timer tm(ctx);
+    const InstDev* dci = (const InstDev*)ctx->buf("inst_coarse").p;
+    void* ttc = ctx->buf("tt_coarse").p;
+    const void* slow = prec == EIK_PREC_FP64 ? (const void*)M->d_slow64 : (const void*)M->d_slow32;
+    const int TX = nd == 2 ? 16 : 8, TZ = nd == 2 ? 1 : 8;
+    const int64_t tiles_per = (int64_t)((M->g.n[0] + TX - 1) / TX) * ((M->g.n[1] + TX - 1) / TX) *
+                              (nd == 3 ? (M->g.n[2] + TZ - 1) / TZ : 1);
+    const int64_t P_c = nsrc * N;
+
+    DevBuf &code_c = ctx->buf("adj_code"), &G_c = ctx->buf("adj_G"), &lam_c = ctx->buf("adj_lam"),
+           &b_c = ctx->buf("adj_b");
+    if (code_c.ensure((size_t)P_c * sizeof(uint16_t)) || G_c.ensure((size_t)P_c * nd * sizeof(double)) ||
+        lam_c.ensure((size_t)P_c * sizeof(double)) || b_c.ensure((size_t)P_c * sizeof(double)))
+        return ctx->fail(EIK_ENOMEM, "out of device memory (adjoint pools)");
+    CK(hipMemsetAsync(lam_c.p, 0, (size_t)P_c * sizeof(double), st));
+    CK(hipMemsetAsync(b_c.p, 0, (size_t)P_c * sizeof(double), st));
+    if (prec == EIK_PREC_FP64)
+        launch_prep<double>(nd, st, dci, (int)nsrc, ttc, slow, (uint16_t*)code_c.p, (double*)G_c.p, P_c);
+    else
+        launch_prep<float>(nd, st, dci, (int)nsrc, ttc, slow, (uint16_t*)code_c.p, (double*)G_c.p, P_c);
+    CK(hipGetLastError());
+
+    // receivers: rhs, misfit
+    DevBuf &drec = ctx->buf("rec"), &drs = ctx->buf("recsrc"), &dpo = ctx->buf("pobs"), &dso = ctx->buf("sobs"),
+           &dmt = ctx->buf("mterm"), &droff = ctx->buf("rec_off"), &dmis = ctx->buf("misf");
+    if (drec.ensure((size_t)std::max<int64_t>(nrec, 1) * nd * sizeof(double)) ||
+        drs.ensure((size_t)std::max<int64_t>(nrec, 1) * sizeof(int)) ||
+        dpo.ensure((size_t)std::max<int64_t>(nrec, 1) * sizeof(double)) ||
+        dso.ensure((size_t)std::max<int64_t>(nrec, 1) * sizeof(double)) ||
+        dmt.ensure((size_t)std::max<int64_t>(nrec, 1) * sizeof(double)) ||
+        droff.ensure((size_t)(nsrc + 1) * sizeof(int64_t)) || dmis.ensure(sizeof(double)))
+        return ctx->fail(EIK_ENOMEM, "out of device memory (receivers)");
+    std::vector<int> rs((size_t)std::max<int64_t>(nrec, 1));
+    for (int64_t s = 0; s < nsrc; s++)
+        for (int64_t r = S.rec_off[s]; r < S.rec_off[s + 1]; r++) rs[r] = (int)s;
+    if (nrec > 0) {
+        CK(hipMemcpyAsync(drec.p, S.rec.data(), (size_t)nrec * nd * sizeof(double), hipMemcpyHostToDevice, st));
+        CK(hipMemcpyAsync(drs.p, rs.data(), (size_t)nrec * sizeof(int), hipMemcpyHostToDevice, st));
+        CK(hipMemcpyAsync(dpo.p, pickobs, (size_t)nrec * sizeof(double), hipMemcpyHostToDevice, st));
+        CK(hipMemcpyAsync(dso.p, stdobs, (size_t)nrec * sizeof(double), hipMemcpyHostToDevice, st));
+    }
+    CK(hipMemcpyAsync(droff.p, S.rec_off.data(), (size_t)(nsrc + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    uint8_t* flags = (uint8_t*)ctx->buf("fim_flags").p;
+    if (nrec > 0) {
+        if (prec == EIK_PREC_FP64)
+            k_adj_rhs<double><<<nb(nrec, 128), 128, 0, st>>>((const double*)drec.p, (const int*)drs.p, nrec, dci,
+                                                             (const double*)ttc, nd, kind, (const double*)dpo.p,
+                                                             (const double*)dso.p, (double*)b_c.p, (double*)dmt.p, flags);
+        else
+            k_adj_rhs<float><<<nb(nrec, 128), 128, 0, st>>>((const double*)drec.p, (const int*)drs.p, nrec, dci,
+                                                            (const float*)ttc, nd, kind, (const double*)dpo.p,
+                                                            (const double*)dso.p, (double*)b_c.p, (double*)dmt.p, flags);
+        CK(hipGetLastError());
+    }
+    k_misfit<<<1, 1, 0, st>>>((const double*)dmt.p, (const int64_t*)droff.p, nsrc, (double*)dmis.p);
+    CK(hipMemcpyAsync(misf, dmis.p, sizeof(double), hipMemcpyDeviceToHost, st));
+
+    FimStats ast{};
+    rc = adj_run(ctx, nd, dci, (int)nsrc, nsrc * tiles_per, (const uint16_t*)code_c.p, (const double*)G_c.p, P_c,
+                 (const double*)b_c.p, (double*)lam_c.p, &ast);
+    if (rc) return rc;
+
+    // coarse source points, grouped by source
+    std::vector<SrcPt> pts;
+    std::vector<int> poff(nsrc + 1, 0);
+    if (refine) {
+        const int cap = ctx->xcap;
+        std::vector<int> ntr(nsrc);
+        std::vector<SrcPt> xl((size_t)nsrc * cap);
+        CK(hipMemcpyAsync(ntr.data(), ctx->buf("ntransfer").p, nsrc * sizeof(int), hipMemcpyDeviceToHost, st));
+        CK(hipMemcpyAsync(xl.data(), ctx->buf("xlist").p, xl.size() * sizeof(SrcPt), hipMemcpyDeviceToHost, st));
+        CK(hipStreamSynchronize(st));
+        for (int64_t s = 0; s < nsrc; s++) {
+            if (ntr[s] > cap) return ctx->fail(EIK_EINVAL, "transferred-point list overflow");
+            for (int k = 0; k < ntr[s]; k++) pts.push_back(xl[(size_t)s * cap + k]);
+            poff[s + 1] = (int)pts.size();
+        }
+    } else {
+        for (int64_t s = 0; s < nsrc; s++) {
+            int c[8][3];
+            const int ncn = find_enclosing_box(M->g, S.src.data() + s * nd, c);
+            for (int k = 0; k < ncn; k++)
+                pts.push_back(SrcPt{(int32_t)s, -1, c[k][0] + M->g.n[0] * (c[k][1] + M->g.n[1] * c[k][2])});
+            poff[s + 1] = (int)pts.size();
+        }
+    }
+    const int npts = (int)pts.size();
+    DevBuf &dpts = ctx->buf("pts"), &dpoff = ctx->buf("poff"), &dup = ctx->buf("dpsiup"), &dus = ctx->buf("dpsidus"),
+           &dt2 = ctx->buf("t2"), &dt1 = ctx->buf("t1"), &dsrc = ctx->buf("srcxyz");
+    if (dpts.ensure(std::max(npts, 1) * sizeof(SrcPt)) || dpoff.ensure((nsrc + 1) * sizeof(int)) ||
+        dup.ensure(std::max(npts, 1) * sizeof(double)) || dus.ensure(std::max(npts, 1) * sizeof(double)) ||
+        dt2.ensure(std::max(npts, 1) * sizeof(double)) || dt1.ensure(std::max(npts, 1) * sizeof(double)) ||
+        dsrc.ensure((size_t)nsrc * nd * sizeof(double)))
+        return ctx->fail(EIK_ENOMEM, "out of device memory (source points)");
+    CK(hipMemcpyAsync(dpts.p, pts.data(), npts * sizeof(SrcPt), hipMemcpyHostToDevice, st));
+    CK(hipMemcpyAsync(dpoff.p, poff.data(), (nsrc + 1) * sizeof(int), hipMemcpyHostToDevice, st));
+    CK(hipMemcpyAsync(dsrc.p, S.src.data(), (size_t)nsrc * nd * sizeof(double), hipMemcpyHostToDevice, st));
+    if (npts > 0) {
+        if (nd == 2)
+            k_srcterms<2><<<nb(npts, 64), 64, 0, st>>>((const SrcPt*)dpts.p, npts, dci, (const uint16_t*)code_c.p,
+                                                        (const double*)G_c.p, P_c, (const double*)lam_c.p,
+                                                        (const double*)b_c.p, (double*)dup.p, (double*)dus.p);
+        else
+            k_srcterms<3><<<nb(npts, 64), 64, 0, st>>>((const SrcPt*)dpts.p, npts, dci, (const uint16_t*)code_c.p,
+                                                        (const double*)G_c.p, P_c, (const double*)lam_c.p,
+                                                        (const double*)b_c.p, (double*)dup.p, (double*)dus.p);
+        CK(hipGetLastError());
+    }
+
+    // fine-grid adjoint (refinement)
+    std::vector<SrcPt> fpts;
+    std::vector<int> fpoff(nsrc + 1, 0);
+    FimStats fst{};
+    if (refine) {
+        const InstDev* dfi = (const InstDev*)ctx->buf("inst_fine").p;
+        const FineDev* dfd = (const FineDev*)ctx->buf("fine_desc").p;
+        const double* ttf = (const double*)ctx->buf("tt_fine").p;
+        const double* slf = (const double*)ctx->buf("slow_fine").p;
+        std::vector<InstDev> finst(nsrc);
+        std::vector<FineDev> fdesc(nsrc);
+        CK(hipMemcpyAsync(finst.data(), dfi, nsrc * sizeof(InstDev), hipMemcpyDeviceToHost, st));
+        CK(hipMemcpyAsync(fdesc.data(), dfd, nsrc * sizeof(FineDev), hipMemcpyDeviceToHost, st));
+        CK(hipStreamSynchronize(st));
+        int64_t P_f = 0, ntiles_f = 0;
+        for (int64_t s = 0; s < nsrc; s++) {
+            P_f = std::max(P_f, finst[s].tt_off + (int64_t)finst[s].n[0] * finst[s].n[1] * finst[s].n[2]);
+            ntiles_f = std::max(ntiles_f, finst[s].tile_base + (int64_t)finst[s].t[0] * finst[s].t[1] * finst[s].t[2]);
+            for (int k = 0; k < fdesc[s].ncorn; k++) {
+                const int* c = fdesc[s].corn[k];
+                fpts.push_back(SrcPt{(int32_t)s, -1, c[0] + (int64_t)finst[s].n[0] * (c[1] + (int64_t)finst[s].n[1] * c[2])});
+            }
+            fpoff[s + 1] = (int)fpts.size();
+        }
+        const int nfp = (int)fpts.size();
+        DevBuf &code_f = ctx->buf("adjf_code"), &G_f = ctx->buf("adjf_G"), &lam_f = ctx->buf("adjf_lam"),
+               &b_f = ctx->buf("adjf_b"), &haq = ctx->buf("adjf_haq"), &dgi = ctx->buf("adjf_dgi"),
+               &zf = ctx->buf("adjf_z"), &dfp = ctx->buf("fpts"), &dfpv = ctx->buf("fpts_val"),
+               &dfpu = ctx->buf("fpts_u");
+        if (code_f.ensure((size_t)P_f * sizeof(uint16_t)) || G_f.ensure((size_t)P_f * nd * sizeof(double)) ||
+            lam_f.ensure((size_t)P_f * sizeof(double)) || b_f.ensure((size_t)P_f * sizeof(double)) ||
+            haq.ensure((size_t)P_f * sizeof(double)) || dgi.ensure((size_t)P_f * sizeof(double)) ||
+            zf.ensure((size_t)P_f * sizeof(double)) || dfp.ensure(std::max(nfp, 1) * sizeof(SrcPt)) ||
+            dfpv.ensure(std::max(nfp, 1) * sizeof(double)) || dfpu.ensure(std::max(nfp, 1) * sizeof(double)))
+            return ctx->fail(EIK_ENOMEM, "out of device memory (fine adjoint)");
+        CK(hipMemsetAsync(lam_f.p, 0, (size_t)P_f * sizeof(double), st));
+        CK(hipMemsetAsync(b_f.p, 0, (size_t)P_f * sizeof(double), st));
+        CK(hipMemsetAsync(haq.p, 0, (size_t)P_f * sizeof(double), st));
+        CK(hipMemsetAsync(dgi.p, 0, (size_t)P_f * sizeof(double), st));
+        CK(hipMemcpyAsync(dfp.p, fpts.data(), nfp * sizeof(SrcPt), hipMemcpyHostToDevice, st));
+        launch_prep<double>(nd, st, dfi, (int)nsrc, ttf, slf, (uint16_t*)code_f.p, (double*)G_f.p, P_f);
+        if (npts > 0)
+            k_fine_rhs<<<nb(npts, 64), 64, 0, st>>>((const SrcPt*)dpts.p, npts, dfi, ttf, (const double*)dup.p,
+                                                    (const double*)dus.p, (double*)b_f.p, (double*)haq.p, flags, nd);
+        CK(hipGetLastError());
+        rc = adj_run(ctx, nd, dfi, (int)nsrc, ntiles_f, (const uint16_t*)code_f.p, (const double*)G_f.p, P_f,
+                     (const double*)b_f.p, (double*)lam_f.p, &fst);
+        if (rc) return rc;
+        if (nfp > 0) {
+            if (nd == 2)
+                k_srcterms<2><<<nb(nfp, 64), 64, 0, st>>>((const SrcPt*)dfp.p, nfp, dfi, (const uint16_t*)code_f.p,
+                                                           (const double*)G_f.p, P_f, (const double*)lam_f.p,
+                                                           (const double*)b_f.p, (double*)dfpv.p, (double*)dfpu.p);
+            else
+                k_srcterms<3><<<nb(nfp, 64), 64, 0, st>>>((const SrcPt*)dfp.p, nfp, dfi, (const uint16_t*)code_f.p,
+                                                           (const double*)G_f.p, P_f, (const double*)lam_f.p,
+                                                           (const double*)b_f.p, (double*)dfpv.p, (double*)dfpu.p);
+            k_scatter_pts<<<nb(nfp, 64), 64, 0, st>>>((const SrcPt*)dfp.p, nfp, dfi, (const double*)dfpv.p,
+                                                      (double*)dgi.p);
+        }
+        dim3 gz(256, (unsigned)nsrc);
+        k_fine_z<<<gz, 256, 0, st>>>(dfi, dfd, ttf, (const double*)lam_f.p, (const double*)dgi.p,
+                                     (const double*)haq.p, M->d_vel, (const double*)dsrc.p, nd, kind, (double*)zf.p);
+        if (npts > 0)
+            k_fine_gather<<<nb(npts, 64), 64, 0, st>>>((const SrcPt*)dpts.p, npts, dfi, dfd, ttf, (const double*)zf.p,
+                                                       (double*)dt2.p, (double*)dt1.p, nd);
+        CK(hipGetLastError());
+        if (gradsrc && whichgrad != EIK_GRADVEL) {
+            DevBuf& dgs = ctx->buf("gradsrc");
+            DevBuf& dfpo = ctx->buf("fpoff");
+            if (dgs.ensure((size_t)nsrc * nd * sizeof(double)) || dfpo.ensure((nsrc + 1) * sizeof(int)))
+                return ctx->fail(EIK_ENOMEM, "out of device memory");
+            CK(hipMemcpyAsync(dfpo.p, fpoff.data(), (nsrc + 1) * sizeof(int), hipMemcpyHostToDevice, st));
+            k_gradsrc<<<nb(nsrc, 64), 64, 0, st>>>((int)nsrc, nd, (const SrcPt*)dfp.p, (const int*)dfpo.p, dfi,
+                                                   M->d_vel, (const double*)dsrc.p, (const double*)dfpv.p, nullptr,
+                                                   dfd, M->d_vel, (const double*)haq.p, (double*)dgs.p);
+            CK(hipGetLastError());
+            CK(hipMemcpyAsync(gradsrc, dgs.p, (size_t)nsrc * nd * sizeof(double), hipMemcpyDeviceToHost, st));
+        }
+    } else {
+        if (npts > 0)
+            k_src_own<<<nb(npts, 64), 64, 0, st>>>((const SrcPt*)dpts.p, npts, dci, M->d_vel, (const double*)dsrc.p, nd,
+                                                   kind, (const double*)dup.p, (const double*)dus.p, (double*)dt2.p,
+                                                   (double*)dt1.p);
+        if (gradsrc && whichgrad != EIK_GRADVEL) {
+            DevBuf& dgs = ctx->buf("gradsrc");
+            if (dgs.ensure((size_t)nsrc * nd * sizeof(double))) return ctx->fail(EIK_ENOMEM, "out of device memory");
+            k_gradsrc<<<nb(nsrc, 64), 64, 0, st>>>((int)nsrc, nd, (const SrcPt*)dpts.p, (const int*)dpoff.p, dci,
+                                                   M->d_vel, (const double*)dsrc.p, (const double*)dup.p,
+                                                   (const double*)dus.p, nullptr, nullptr, nullptr, (double*)dgs.p);
+            CK(hipGetLastError());
+            CK(hipMemcpyAsync(gradsrc, dgs.p, (size_t)nsrc * nd * sizeof(double), hipMemcpyDeviceToHost, st));
+        }
+    }
+
+    if (gradvel && whichgrad != EIK_GRADSRCLOC) {
+        DevBuf &img = ctx->buf("grad_img"), &dtap = ctx->buf("taper"), &gout = ctx->buf("grad_out");
+        if (img.ensure((size_t)P_c * sizeof(double)) || dtap.ensure((size_t)nsrc * sizeof(TaperDev)) ||
+            gout.ensure((size_t)N * 2 * sizeof(double)))
+            return ctx->fail(EIK_ENOMEM, "out of device memory (gradient images)");
+        std::vector<TaperDev> tap(nsrc);
+        for (int64_t s = 0; s < nsrc; s++) taper_box(M->g, S.src.data() + s * nd, P->radiussmoothgradsrc, tap[s]);
+        CK(hipMemcpyAsync(dtap.p, tap.data(), nsrc * sizeof(TaperDev), hipMemcpyHostToDevice, st));
+        dim3 gi(512, (unsigned)nsrc);
+        if (prec == EIK_PREC_FP64)
+            k_grad_img<double><<<gi, 256, 0, st>>>(dci, (const double*)ttc, (const double*)lam_c.p, M->d_vel,
+                                                   (double*)img.p, N);
+        else
+            k_grad_img<float><<<gi, 256, 0, st>>>(dci, (const float*)ttc, (const double*)lam_c.p, M->d_vel,
+                                                  (double*)img.p, N);
+        k_grad_quirk<<<1, 256, 0, st>>>((const SrcPt*)dpts.p, (const int*)dpoff.p, (int)nsrc, dci,
+                                        (const TaperDev*)dtap.p, nd, kind, (const double*)dt2.p, (const double*)dt1.p,
+                                        (double*)img.p);
+        k_taper<<<(unsigned)nsrc, 256, 0, st>>>(dci, (const TaperDev*)dtap.p, nd, kind, (double*)img.p);
+        k_sum_sources<<<nb(N, 256), 256, 0, st>>>((const double*)img.p, (int)nsrc, N, (double*)gout.p);
+        CK(hipGetLastError());
+        double* res = (double*)gout.p;
+        if (P->smoothgradkern > 0) {
+            const int l = P->smoothgradkern;
+            int nk = 0;
+            for (int d = 0; d < nd; d++) nk += M->g.n[d] != 1;
+            int64_t kn = 1;
+            for (int d = 0; d < nk; d++) kn *= l;
+            std::vector<double> ker(kn);
+            const int m = l / 2;
+            const double s2 = ((double)l / 2.0) - sqrt((double)l / 2.0);
+            double ks = 0.0;
+            for (int64_t q = 0; q < kn; q++) {
+                int64_t r = q;
+                double rr2 = 0.0;
+                for (int d = 0; d < nk; d++) {
+                    const int64_t pos = r % l - m;
+                    r /= l;
+                    rr2 += (double)(pos * pos);
+                }
+                const double rr = sqrt(rr2);
+                ker[q] = exp(-(rr * rr) / (2.0 * s2));
+                ks += ker[q];
+            }
+            for (auto& x : ker) x /= ks;
+            DevBuf& dk = ctx->buf("smooth_ker");
+            if (dk.ensure(kn * sizeof(double))) return ctx->fail(EIK_ENOMEM, "out of device memory");
+            CK(hipMemcpyAsync(dk.p, ker.data(), kn * sizeof(double), hipMemcpyHostToDevice, st));
+            k_smooth<<<nb(N, 256), 256, 0, st>>>((const double*)gout.p, (double*)gout.p + N, (const double*)dk.p, l, nd,
+                                                 (int)M->g.n[0], (int)M->g.n[1], (int)M->g.n[2]);
+            CK(hipGetLastError());
+            res = (double*)gout.p + N;
+        }
+        CK(hipMemcpyAsync(gradvel, res, (size_t)N * sizeof(double), hipMemcpyDeviceToHost, st));
+    }
+    CK(hipStreamSynchronize(st));
+    ctx->stats[3] = tm.mark();
+    return EIK_OK;
+}
+
+}  // namespace eik

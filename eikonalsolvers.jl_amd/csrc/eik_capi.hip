@@ -18,6 +18,7 @@ void make_inst(const eik_grid& g, InstDev& I) {
     I.kind = g.kind;
     I.dthrad = deg2rad(g.d[1]);
     I.dphrad = deg2rad(g.d[2]);
+    I.tstop = INFINITY;
 }
 
 static inline double gcoord(const eik_grid& g, int d, int64_t i) {
@@ -292,8 +293,32 @@ int eik_misfit(eik_ctx* ctx, eik_model* model, int64_t nsrc, const double* coord
 int eik_gradient(eik_ctx* ctx, eik_model* model, int64_t nsrc, const double* coordsrc, const int64_t* rec_off,
                  const double* coordrec, const double* pickobs, const double* stdobs, int32_t whichgrad,
                  const eik_params* params, double* gradvel, double* gradsrc, double* misf) {
-    if (!ctx) return EIK_EINVAL;
-    return ctx->fail(EIK_EINVAL, "eik_gradient: not available in this build");
+    if (!ctx || !model || !misf) return EIK_EINVAL;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    if (whichgrad < EIK_GRADVEL || whichgrad > EIK_GRADVELANDSRCLOC)
+        return ctx->fail(EIK_EASSERT, "Symbol 'whichgrad' not in (:gradvel, :gradsrcloc, :gradvelandsrcloc)");
+    SourceSet S;
+    int rc = load_sources(ctx, model, nsrc, coordsrc, rec_off, coordrec, params, S);
+    if (rc) return rc;
+    const int kind = model->g.kind;
+    if (kind == EIK_GRID3D_SPHERE)
+        return ctx->fail(EIK_EINVAL, "createsparsederivativematrices!(): Not yet implemented for Grid3DSphere.");
+    if (kind == EIK_GRID2D_SPHERE && whichgrad != EIK_GRADVEL)
+        return ctx->fail(EIK_EINVAL, "gradsrcloc is not available on spherical grids");
+    if (S.nrec() > 0 && (!pickobs || !stdobs)) return ctx->fail(EIK_EINVAL, "null observations");
+    if (whichgrad != EIK_GRADSRCLOC && !gradvel) return ctx->fail(EIK_EINVAL, "null gradvel buffer");
+    if (whichgrad != EIK_GRADVEL && !gradsrc) return ctx->fail(EIK_EINVAL, "null gradsrc buffer");
+    if (nsrc == 0) {
+        *misf = 0.0;
+        if (gradvel && whichgrad != EIK_GRADSRCLOC) memset(gradvel, 0, model->N * sizeof(double));
+        return EIK_OK;
+    }
+    for (double& x : ctx->stats) x = 0.0;
+    Timer total(ctx);
+    rc = gradient_solve(ctx, model, S, params, whichgrad, pickobs, stdobs, gradvel, gradsrc, misf);
+    ctx->stats[0] = total.mark();
+    return rc;
 }
 
 int eik_last_stats(const eik_ctx* ctx, double* out, int32_t nout) {

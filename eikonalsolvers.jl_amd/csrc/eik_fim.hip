@@ -241,25 +241,6 @@ __global__ void k_slowness(const double* __restrict__ vel, double* __restrict__ 
     }
 }
 
-// mark the tile of node g and the face-neighbour tiles whose stencil reaches it
-__device__ __forceinline__ void seed_tiles(const InstDev& I, int nd, const int* g, uint8_t* flags) {
-    const int TX = nd == 2 ? 16 : 8, TY = TX, TZ = nd == 2 ? 1 : 8;
-    const int tx = g[0] / TX, ty = g[1] / TY, tz = nd == 3 ? g[2] / TZ : 0;
-    const int64_t t0 = I.t[0], t01 = (int64_t)I.t[0] * I.t[1];
-    const int64_t tl = tx + t0 * ty + t01 * tz;
-    uint8_t* fl = flags + I.tile_base;
-    fl[tl] = 1;
-    const int lx = g[0] - tx * TX, ly = g[1] - ty * TY, lz = nd == 3 ? g[2] - tz * TZ : 0;
-    if (lx < 2 && tx > 0) fl[tl - 1] = 1;
-    if (lx >= TX - 2 && tx < I.t[0] - 1) fl[tl + 1] = 1;
-    if (ly < 2 && ty > 0) fl[tl - t0] = 1;
-    if (ly >= TY - 2 && ty < I.t[1] - 1) fl[tl + t0] = 1;
-    if (nd == 3) {
-        if (lz < 2 && tz > 0) fl[tl - t01] = 1;
-        if (lz >= TZ - 2 && tz < I.t[2] - 1) fl[tl + t01] = 1;
-    }
-}
-
 // Source-box corners: tt = dist(corner, src)/vel[corner] (sourceboxloctt!, eikmainfwd.jl:1195-1322).
 // One thread per (instance, corner).  Works on the coarse grid (no refinement) and on the fine
 // boxes (vel = the nearest-neighbour fine velocity, read through the box's map).
@@ -316,19 +297,21 @@ __global__ void k_fine_setup(const FineDev* __restrict__ fine, int nfine, const 
 // at least one coarse-coincident node has been transferred; the coarse init set is every
 // coarse-coincident fine node accepted up to and including that node.  With the converged fine
 // solution: t_c = min tt over coarse-coincident nodes, t_stop = min tt over face nodes with
-// tt >= t_c, transfer = coarse-coincident nodes with tt <= t_stop.  One block per source.
+// tt >= t_c, transfer = coarse-coincident nodes with tt <= t_stop.  One block per source; the
+// transferred points are listed in coarse-window order (xlist, cap entries per source) and
+// t_stop is stored in the fine instance for the adjoint.
 template <typename T>
-__global__ void __launch_bounds__(256) k_transfer(const FineDev* __restrict__ fine, const InstDev* __restrict__ finst,
+__global__ void __launch_bounds__(256) k_transfer(const FineDev* __restrict__ fine, InstDev* __restrict__ finst,
                                                   const double* __restrict__ ttf, const InstDev* __restrict__ cinst,
                                                   T* __restrict__ ttc, uint8_t* __restrict__ cflags,
-                                                  int* __restrict__ ntransfer, int nd) {
+                                                  int* __restrict__ ntransfer, SrcPt* __restrict__ xlist, int cap,
+                                                  int nd) {
     const int f = blockIdx.x;
     const FineDev F = fine[f];
     const InstDev I = finst[f];
     const InstDev C = cinst[F.cinst];
     const int64_t n = (int64_t)I.n[0] * I.n[1] * I.n[2];
     __shared__ double red[256];
-    __shared__ int cnt;
     auto coord = [&](int64_t l, int* g) {
         g[0] = (int)(l % I.n[0]);
         g[1] = (int)((l / I.n[0]) % I.n[1]);
@@ -371,23 +354,26 @@ __global__ void __launch_bounds__(256) k_transfer(const FineDev* __restrict__ fi
         if (face && raw >= tc) m = fmin(m, raw);
     }
     const double tstop = blockmin(m);
-    if (threadIdx.x == 0) cnt = 0;
-    __syncthreads();
-    for (int64_t l = threadIdx.x; l < n; l += 256) {
-        int g[3];
-        coord(l, g);
-        if (!coincident(g)) continue;
-        const double t = fabs(ttf[I.tt_off + l]);
-        if (!(t <= tstop)) continue;
-        int cg[3] = {0, 0, 0};
-        for (int d = 0; d < nd; d++) cg[d] = g[d] / F.df + F.i1c[d];
-        const int64_t lc = cg[0] + (int64_t)C.n[0] * (cg[1] + (int64_t)C.n[1] * cg[2]);
-        ttc[C.tt_off + lc] = -(T)t;
-        seed_tiles(C, nd, cg, cflags);
-        atomicAdd(&cnt, 1);
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) ntransfer[f] = cnt;
+    if (threadIdx.x != 0) return;
+    finst[f].tstop = tstop;
+    int nw[3] = {1, 1, 1};
+    for (int d = 0; d < nd; d++) nw[d] = (I.n[d] - 1) / F.df + 1;
+    int cnt = 0;
+    for (int wz = 0; wz < nw[2]; wz++)
+        for (int wy = 0; wy < nw[1]; wy++)
+            for (int wx = 0; wx < nw[0]; wx++) {
+                const int64_t lf = (int64_t)wx * F.df +
+                                   (int64_t)I.n[0] * ((int64_t)wy * F.df + (int64_t)I.n[1] * ((int64_t)wz * F.df));
+                const double t = fabs(ttf[I.tt_off + lf]);
+                if (!(t <= tstop)) continue;
+                const int cg[3] = {F.i1c[0] + wx, F.i1c[1] + wy, F.i1c[2] + wz};
+                const int64_t lc = cg[0] + (int64_t)C.n[0] * (cg[1] + (int64_t)C.n[1] * cg[2]);
+                ttc[C.tt_off + lc] = -(T)t;
+                seed_tiles(C, nd, cg, cflags);
+                if (cnt < cap) xlist[(int64_t)f * cap + cnt] = SrcPt{F.cinst, (int32_t)lf, lc};
+                cnt++;
+            }
+    ntransfer[f] = cnt;
 }
 
 // receiver picks: bilinear_interp (eikforward2D.jl:17-87) / trilinear_interp (eikforward3D.jl:17-140)
@@ -560,15 +546,25 @@ int forward_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, int prec, cons
         if (rc) return rc;
         ctx->stats[1] = tm.mark();
         DevBuf& ntr = ctx->buf("ntransfer");
-        if (ntr.ensure((size_t)nsrc * sizeof(int))) return ctx->fail(EIK_ENOMEM, "out of device memory");
+        DevBuf& xl = ctx->buf("xlist");
+        int cap = 1;
+        for (int64_t s = 0; s < nsrc; s++) {
+            int w = 1;
+            for (int d = 0; d < nd; d++) w *= (finst[s].n[d] - 1) / fine[s].df + 1;
+            cap = std::max(cap, w);
+        }
+        ctx->xcap = cap;
+        if (ntr.ensure((size_t)nsrc * sizeof(int)) || xl.ensure((size_t)nsrc * cap * sizeof(SrcPt)))
+            return ctx->fail(EIK_ENOMEM, "out of device memory");
         if (prec == EIK_PREC_FP64)
-            k_transfer<double><<<(unsigned)nsrc, 256, 0, st>>>((const FineDev*)dfd.p, (const InstDev*)dfi.p,
+            k_transfer<double><<<(unsigned)nsrc, 256, 0, st>>>((const FineDev*)dfd.p, (InstDev*)dfi.p,
                                                               (const double*)ttf.p, (const InstDev*)dci.p,
-                                                              (double*)ttc.p, (uint8_t*)flags.p, (int*)ntr.p, nd);
+                                                              (double*)ttc.p, (uint8_t*)flags.p, (int*)ntr.p,
+                                                              (SrcPt*)xl.p, cap, nd);
         else
-            k_transfer<float><<<(unsigned)nsrc, 256, 0, st>>>((const FineDev*)dfd.p, (const InstDev*)dfi.p,
+            k_transfer<float><<<(unsigned)nsrc, 256, 0, st>>>((const FineDev*)dfd.p, (InstDev*)dfi.p,
                                                              (const double*)ttf.p, (const InstDev*)dci.p, (float*)ttc.p,
-                                                             (uint8_t*)flags.p, (int*)ntr.p, nd);
+                                                             (uint8_t*)flags.p, (int*)ntr.p, (SrcPt*)xl.p, cap, nd);
         CK(hipGetLastError());
     } else {
         for (int64_t s = 0; s < nsrc; s++) {

@@ -6,6 +6,7 @@
 #include <stdint.h>
 #include <stdio.h>
 
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <string>
@@ -30,6 +31,7 @@ struct InstDev {
     double dphrad;       // spherical 3D: dphi in radians
     int32_t kind;        // EIK_GRID*
     int32_t pad;
+    double tstop;        // adjoint: nodes with tt <= tstop form the system (+inf on coarse grids)
 };
 
 // Refined box around one source (createfinegrid, src/EikSolv/eikmainfwd.jl:950-1104)
@@ -42,6 +44,23 @@ struct FineDev {
     int32_t ncorn;       // source-box corners on the fine grid
     int32_t corn[8][3];
     int32_t cinst;       // coarse instance of the same source
+};
+
+// A source (init) point of one instance: coarse source-box corner, transferred coarse node, or
+// fine-grid corner.  node is the linear index inside the instance's grid.
+struct SrcPt {
+    int32_t inst;
+    int32_t aux;    // transferred points: fine-grid node of the same position; -1 otherwise
+    int64_t node;
+};
+
+// Taper box of one source (smoothgradaroundsrc!, src/EikSolv/utils.jl:121-338)
+struct TaperDev {
+    int32_t lo[3], hi[3];  // index box (may reach outside the grid)
+    double src[3];
+    double rmax;
+    int32_t on;            // radius > 0
+    int32_t pad;
 };
 
 // One source-box corner to initialise (sourceboxloctt!, eikmainfwd.jl:1195-1322)
@@ -81,6 +100,25 @@ __host__ __device__ __forceinline__ double distance2points(int kind, int nd, con
     double s = 0.0;
     for (int d = 0; d < nd; d++) s += (a[d] - b[d]) * (a[d] - b[d]);
     return sqrt(s);
+}
+
+// mark the tile of node g and the face-neighbour tiles whose stencil reaches it
+__device__ __forceinline__ void seed_tiles(const InstDev& I, int nd, const int* g, uint8_t* flags) {
+    const int TX = nd == 2 ? 16 : 8, TY = TX, TZ = nd == 2 ? 1 : 8;
+    const int tx = g[0] / TX, ty = g[1] / TY, tz = nd == 3 ? g[2] / TZ : 0;
+    const int64_t t0 = I.t[0], t01 = (int64_t)I.t[0] * I.t[1];
+    const int64_t tl = tx + t0 * ty + t01 * tz;
+    uint8_t* fl = flags + I.tile_base;
+    fl[tl] = 1;
+    const int lx = g[0] - tx * TX, ly = g[1] - ty * TY, lz = nd == 3 ? g[2] - tz * TZ : 0;
+    if (lx < 2 && tx > 0) fl[tl - 1] = 1;
+    if (lx >= TX - 2 && tx < I.t[0] - 1) fl[tl + 1] = 1;
+    if (ly < 2 && ty > 0) fl[tl - t0] = 1;
+    if (ly >= TY - 2 && ty < I.t[1] - 1) fl[tl + t0] = 1;
+    if (nd == 3) {
+        if (lz < 2 && tz > 0) fl[tl - t01] = 1;
+        if (lz >= TZ - 2 && tz < I.t[2] - 1) fl[tl + t01] = 1;
+    }
 }
 
 // per-node stencil spacing (eikforward2D.jl:115-119, eikforward3D.jl:168-172)
@@ -218,6 +256,7 @@ struct eik_ctx {
     hipEvent_t ev[2] = {nullptr, nullptr};   // phase timer
     hipEvent_t kev[2] = {nullptr, nullptr};  // per-launch kernel timer
     double stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int xcap = 1;  // capacity per source of the transferred-point list
     std::mutex mu;
     DevBuf& buf(const char* name) { return bufs[name]; }
     int fail(int code, const char* fmt, ...) {
@@ -282,6 +321,12 @@ void build_fine(const eik_grid& g, const double* src, int df, int noderadius, Fi
 int forward_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, int prec, const eik_params* P);
 int forward_picks(eik_ctx* ctx, eik_model* M, const SourceSet& S, int prec, double* picks, double* ttgrid);
 int model_slowness(eik_model* M);
+int gradient_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, const eik_params* P, int whichgrad,
+                   const double* pickobs, const double* stdobs, double* gradvel, double* gradsrc, double* misf);
+// tile-flag compaction (eik_fim.hip), also used by the adjoint rounds
+__global__ void k_compact(uint8_t* __restrict__ flags, int64_t ntiles, int64_t* __restrict__ list,
+                          unsigned* __restrict__ count);
+
 struct FimStats {
     int64_t rounds = 0;        // solver rounds (one launch each)
     double activations = 0;    // tile activations (blocks launched)

@@ -51,7 +51,7 @@ class Context:
         out = np.zeros(8)
         _lib.load().eik_last_stats(self.handle, _lib.ptr(out), 8)
         return {"total_ms": out[0], "fine_ms": out[1], "coarse_ms": out[2], "adjoint_ms": out[3],
-                "tile_sweeps": out[4], "rounds": out[5]}
+                "tile_activations": out[4], "rounds": out[5], "_kernel_ms": out[6], "tile_sweeps": out[7]}
 
 
 _default_ctx = None
