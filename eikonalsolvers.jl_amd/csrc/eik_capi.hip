@@ -149,6 +149,8 @@ static int check_grid(eik_ctx* ctx, const eik_grid* g) {
     if ((k2 && g->ndim != 2) || (k3 && g->ndim != 3)) return ctx->fail(EIK_EASSERT, "Ndim==length(grd.grsize)");
     for (int d = 0; d < g->ndim; d++) {
         if (g->n[d] < 1) return ctx->fail(EIK_EASSERT, "all(grsize.>0)");
+        // receiver interpolation needs a cell along every axis (the reference indexes i+1)
+        if (g->n[d] < 2) return ctx->fail(EIK_EASSERT, "every grid axis needs at least 2 nodes");
         if (g->n[d] > (int64_t)1 << 30) return ctx->fail(EIK_EINVAL, "grid too large");
         if (!(g->d[d] > 0.0) && !(d > 0 && (g->kind == EIK_GRID2D_CART || g->kind == EIK_GRID3D_CART)))
             return ctx->fail(EIK_EASSERT, "grid spacing must be > 0");
