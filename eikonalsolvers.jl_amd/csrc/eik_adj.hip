@@ -54,16 +54,35 @@ __device__ __forceinline__ double tabs(T v) {
     return fabs((double)v);
 }
 
+// pop key of node l (init nodes pop first)
+template <typename T>
+__device__ __forceinline__ Key<T> node_key(const T* tt, const T* kk, const T* ss, const uint8_t* dd, int64_t l) {
+    const T raw = tt[l];
+    if (signbit(raw)) return Key<T>{-tinf<T>(), -tinf<T>(), 0};
+    if (dd[l]) return Key<T>{kk[l], ss[l], (int)dd[l]};
+    return Key<T>{raw, -tinf<T>(), 0};
+}
+
+// the node popped no later than the refinement stop (always true on coarse grids)
+template <typename T>
+__device__ __forceinline__ bool before_stop(const InstDev& I, const Key<T>& k) {
+    const Key<double> kd{(double)k.K, (double)k.S, k.D};
+    return !klt(Key<double>{I.tstopK, I.tstopS, I.tstopD}, kd);
+}
+
 // Codes and G_d of every node: replay of the final stencil evaluation (eikmainfwd.jl:794-809)
 // and 2*(D_d t)_i accumulated over the stencil in acceptance order (far node first).
 template <int ND, typename T>
-__global__ void __launch_bounds__(kAB) k_adj_prep(const InstDev* __restrict__ inst, const T* __restrict__ ttpool,
+__global__ void __launch_bounds__(kAB) k_adj_prep(const InstDev* __restrict__ inst, TTPools pools,
                                                   const T* __restrict__ slowpool, uint16_t* __restrict__ code_out,
                                                   double* __restrict__ G, int64_t gstride) {
 #pragma clang fp contract(off)
     const InstDev I = inst[blockIdx.y];
     const int64_t n = (int64_t)I.n[0] * I.n[1] * I.n[2];
-    const T* tt = ttpool + I.tt_off;
+    const T* tt = (const T*)pools.tt + I.tt_off;
+    const T* kk = (const T*)pools.k + I.tt_off;
+    const T* ss = (const T*)pools.s + I.tt_off;
+    const uint8_t* dd = pools.d + I.tt_off;
     const T* slow = slowpool + I.s_off;
     const int64_t nx = I.n[0], nxy = (int64_t)I.n[0] * I.n[1];
     for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < n; l += (int64_t)gridDim.x * blockDim.x) {
@@ -72,8 +91,9 @@ __global__ void __launch_bounds__(kAB) k_adj_prep(const InstDev* __restrict__ in
         const double t0 = tabs(raw);
         uint16_t pc = 0;
         double Gd[3] = {0.0, 0.0, 0.0};
-        if (!signbit(raw) && t0 <= I.tstop) {
+        if (!signbit(raw) && t0 < EIK_INF && before_stop(I, node_key(tt, kk, ss, dd, l))) {
             T v1[2 * ND], v2[2 * ND];
+            Key<T> k1[2 * ND], k2[2 * ND];
             unsigned f1 = 0, f2 = 0, ok1 = 0, ok2 = 0;
 #pragma unroll
             for (int ax = 0; ax < ND; ax++) {
@@ -87,12 +107,21 @@ __global__ void __launch_bounds__(kAB) k_adj_prep(const InstDev* __restrict__ in
 #pragma unroll
                     for (int d = 0; d < ND; d++) in2 = in2 && nb[d] >= 1 && nb[d] <= I.n[d] - 2;
                     T a = tinf<T>(), b = tinf<T>();
-                    if (in1) a = tt[nb[0] + nx * nb[1] + nxy * nb[2]];
+                    Key<T> ka{tinf<T>(), -tinf<T>(), 0}, kb{tinf<T>(), -tinf<T>(), 0};
+                    if (in1) {
+                        const int64_t q1 = nb[0] + nx * nb[1] + nxy * nb[2];
+                        a = tt[q1];
+                        ka = node_key(tt, kk, ss, dd, q1);
+                    }
                     if (in2) {
                         int nb2[3] = {g[0], g[1], g[2]};
                         nb2[ax] += (s == 0) ? 2 : -2;
-                        b = tt[nb2[0] + nx * nb2[1] + nxy * nb2[2]];
+                        const int64_t q2 = nb2[0] + nx * nb2[1] + nxy * nb2[2];
+                        b = tt[q2];
+                        kb = node_key(tt, kk, ss, dd, q2);
                     }
+                    k1[k] = ka;
+                    k2[k] = kb;
                     ok1 |= (unsigned)in1 << k;
                     ok2 |= (unsigned)in2 << k;
                     f1 |= (unsigned)signbit(a) << k;
@@ -104,7 +133,8 @@ __global__ void __launch_bounds__(kAB) k_adj_prep(const InstDev* __restrict__ in
             T dh[3];
             node_spacing<T>(I, g, dh);
             int code[3] = {0, 0, 0};
-            (void)replay<ND, T>(v1, v2, f1, f2, ok1, ok2, slow[l], dh, code);
+            Key<T> kz;
+            (void)replay<ND, T>(v1, v2, k1, k2, f1, f2, ok1, ok2, slow[l], dh, code, &kz);
 #pragma unroll
             for (int d = 0; d < ND; d++) {
                 const int c = code[d];
@@ -407,10 +437,6 @@ __global__ void k_srcterms(const SrcPt* __restrict__ pts, int npts, const InstDe
     dpsidus[q] = -b[I.tt_off + l];
 }
 
-template __global__ void k_adj_prep<2, double>(const InstDev*, const double*, const double*, uint16_t*, double*, int64_t);
-template __global__ void k_adj_prep<3, double>(const InstDev*, const double*, const double*, uint16_t*, double*, int64_t);
-template __global__ void k_adj_prep<2, float>(const InstDev*, const float*, const float*, uint16_t*, double*, int64_t);
-template __global__ void k_adj_prep<3, float>(const InstDev*, const float*, const float*, uint16_t*, double*, int64_t);
 
 }  // namespace eik
 
@@ -484,7 +510,7 @@ __global__ void k_fine_rhs(const SrcPt* __restrict__ xp, int nxp, const InstDev*
 
 // z_p of the fine grid: 2*mu/w^3 (T2Wb) at accepted non-source nodes, dtau/dw * (dgi + H_A^T q)
 // (T2Wa + T1Wa) at the fine source-box corners
-__global__ void k_fine_z(const InstDev* __restrict__ finst, const FineDev* __restrict__ fine,
+__global__ void k_fine_z(const InstDev* __restrict__ finst, const FineDev* __restrict__ fine, TTPools fp,
                          const double* __restrict__ ttf, const double* __restrict__ mu, const double* __restrict__ dgi,
                          const double* __restrict__ haq, const double* __restrict__ velc,
                          const double* __restrict__ srcxyz, int nd, int kind, double* __restrict__ z) {
@@ -506,7 +532,8 @@ __global__ void k_fine_z(const InstDev* __restrict__ finst, const FineDev* __res
             const double dist = distance2points(kind, nd, xyz, srcxyz + f * nd);
             const double dd = -dist * (1.0 / (w * w));
             zv = dd * (dgi[I.tt_off + l] + haq[I.tt_off + l]);
-        } else if (fabs(raw) <= I.tstop) {
+        } else if (raw < EIK_INF && before_stop(I, node_key((const double*)fp.tt + I.tt_off, (const double*)fp.k + I.tt_off,
+                                                             (const double*)fp.s + I.tt_off, fp.d + I.tt_off, l))) {
             zv = 2.0 * mu[I.tt_off + l] / (w * w * w);
         }
         z[I.tt_off + l] = zv;
@@ -516,7 +543,7 @@ __global__ void k_fine_z(const InstDev* __restrict__ finst, const FineDev* __res
 // coarse source point a gathers z over the accepted fine nodes whose nearest coarse node is a
 // (dwq_dva keeps only coarse nodes that are transferred points, eikmaingrad.jl:717-733)
 __global__ void k_fine_gather(const SrcPt* __restrict__ xp, int nxp, const InstDev* __restrict__ finst,
-                              const FineDev* __restrict__ fine, const double* __restrict__ ttf,
+                              const FineDev* __restrict__ fine, TTPools fp, const double* __restrict__ ttf,
                               const double* __restrict__ z, double* __restrict__ t2, double* __restrict__ t1, int nd) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nxp) return;
@@ -538,7 +565,10 @@ __global__ void k_fine_gather(const SrcPt* __restrict__ xp, int nxp, const InstD
                 if (nn_coarse(i, F.df) != w[0] || nn_coarse(j, F.df) != w[1]) continue;
                 if (nd == 3 && nn_coarse(k, F.df) != w[2]) continue;
                 const int64_t l = i + (int64_t)I.n[0] * (j + (int64_t)I.n[1] * k);
-                if (!(fabs(ttf[I.tt_off + l]) <= I.tstop)) continue;
+                if (!(fabs(ttf[I.tt_off + l]) < EIK_INF)) continue;
+                if (!before_stop(I, node_key((const double*)fp.tt + I.tt_off, (const double*)fp.k + I.tt_off,
+                                             (const double*)fp.s + I.tt_off, fp.d + I.tt_off, l)))
+                    continue;
                 acc += z[I.tt_off + l];
             }
     t2[q] = acc;
@@ -749,13 +779,13 @@ static int adj_run(eik_ctx* ctx, int nd, const InstDev* d_inst, int ninst, int64
 }
 
 template <typename T>
-static void launch_prep(int nd, hipStream_t st, const InstDev* inst, int ninst, const void* tt, const void* slow,
-                        uint16_t* code, double* G, int64_t gstride) {
+static void launch_prep(int nd, hipStream_t st, const InstDev* inst, int ninst, const TTPools& pools,
+                        const void* slow, uint16_t* code, double* G, int64_t gstride) {
     dim3 grid(256, (unsigned)ninst);
     if (nd == 2)
-        k_adj_prep<2, T><<<grid, kAB, 0, st>>>(inst, (const T*)tt, (const T*)slow, code, G, gstride);
+        k_adj_prep<2, T><<<grid, kAB, 0, st>>>(inst, pools, (const T*)slow, code, G, gstride);
     else
-        k_adj_prep<3, T><<<grid, kAB, 0, st>>>(inst, (const T*)tt, (const T*)slow, code, G, gstride);
+        k_adj_prep<3, T><<<grid, kAB, 0, st>>>(inst, pools, (const T*)slow, code, G, gstride);
 }
 
 static void taper_box(const eik_grid& g, const double* src, int R, TaperDev& T) {
@@ -807,10 +837,11 @@ int gradient_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, const eik_par
         return ctx->fail(EIK_ENOMEM, "out of device memory (adjoint pools)");
     CK(hipMemsetAsync(lam_c.p, 0, (size_t)P_c * sizeof(double), st));
     CK(hipMemsetAsync(b_c.p, 0, (size_t)P_c * sizeof(double), st));
+    const TTPools cpools{ttc, ctx->buf("key_K").p, ctx->buf("key_S").p, (uint8_t*)ctx->buf("key_D").p};
     if (prec == EIK_PREC_FP64)
-        launch_prep<double>(nd, st, dci, (int)nsrc, ttc, slow, (uint16_t*)code_c.p, (double*)G_c.p, P_c);
+        launch_prep<double>(nd, st, dci, (int)nsrc, cpools, slow, (uint16_t*)code_c.p, (double*)G_c.p, P_c);
     else
-        launch_prep<float>(nd, st, dci, (int)nsrc, ttc, slow, (uint16_t*)code_c.p, (double*)G_c.p, P_c);
+        launch_prep<float>(nd, st, dci, (int)nsrc, cpools, slow, (uint16_t*)code_c.p, (double*)G_c.p, P_c);
     CK(hipGetLastError());
 
     // receivers: rhs, misfit
@@ -940,7 +971,8 @@ int gradient_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, const eik_par
         CK(hipMemsetAsync(haq.p, 0, (size_t)P_f * sizeof(double), st));
         CK(hipMemsetAsync(dgi.p, 0, (size_t)P_f * sizeof(double), st));
         CK(hipMemcpyAsync(dfp.p, fpts.data(), nfp * sizeof(SrcPt), hipMemcpyHostToDevice, st));
-        launch_prep<double>(nd, st, dfi, (int)nsrc, ttf, slf, (uint16_t*)code_f.p, (double*)G_f.p, P_f);
+        const TTPools fpools{(void*)ttf, ctx->buf("keyf_K").p, ctx->buf("keyf_S").p, (uint8_t*)ctx->buf("keyf_D").p};
+        launch_prep<double>(nd, st, dfi, (int)nsrc, fpools, slf, (uint16_t*)code_f.p, (double*)G_f.p, P_f);
         if (npts > 0)
             k_fine_rhs<<<nb(npts, 64), 64, 0, st>>>((const SrcPt*)dpts.p, npts, dfi, ttf, (const double*)dup.p,
                                                     (const double*)dus.p, (double*)b_f.p, (double*)haq.p, flags, nd);
@@ -961,10 +993,10 @@ int gradient_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, const eik_par
                                                       (double*)dgi.p);
         }
         dim3 gz(256, (unsigned)nsrc);
-        k_fine_z<<<gz, 256, 0, st>>>(dfi, dfd, ttf, (const double*)lam_f.p, (const double*)dgi.p,
+        k_fine_z<<<gz, 256, 0, st>>>(dfi, dfd, fpools, ttf, (const double*)lam_f.p, (const double*)dgi.p,
                                      (const double*)haq.p, M->d_vel, (const double*)dsrc.p, nd, kind, (double*)zf.p);
         if (npts > 0)
-            k_fine_gather<<<nb(npts, 64), 64, 0, st>>>((const SrcPt*)dpts.p, npts, dfi, dfd, ttf, (const double*)zf.p,
+            k_fine_gather<<<nb(npts, 64), 64, 0, st>>>((const SrcPt*)dpts.p, npts, dfi, dfd, fpools, ttf, (const double*)zf.p,
                                                        (double*)dt2.p, (double*)dt1.p, nd);
         CK(hipGetLastError());
         if (gradsrc && whichgrad != EIK_GRADVEL) {

@@ -18,7 +18,9 @@ void make_inst(const eik_grid& g, InstDev& I) {
     I.kind = g.kind;
     I.dthrad = deg2rad(g.d[1]);
     I.dphrad = deg2rad(g.d[2]);
-    I.tstop = INFINITY;
+    I.tstopK = INFINITY;
+    I.tstopS = -INFINITY;
+    I.tstopD = 0;
 }
 
 static inline double gcoord(const eik_grid& g, int d, int64_t i) {

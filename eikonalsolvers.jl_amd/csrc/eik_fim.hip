@@ -52,10 +52,16 @@ __device__ __forceinline__ bool is_fixed(T raw) {
 template <int ND, typename T>
 __global__ void __launch_bounds__(kBlock) k_fim_round(const int64_t* __restrict__ list,
                                                       const InstDev* __restrict__ inst, int ninst,
-                                                      T* __restrict__ ttpool, const T* __restrict__ slowpool,
-                                                      uint8_t* __restrict__ flags, unsigned* __restrict__ visits) {
+                                                      T* __restrict__ ttpool, T* __restrict__ kpool,
+                                                      T* __restrict__ spool, uint8_t* __restrict__ dpool,
+                                                      const T* __restrict__ slowpool, uint8_t* __restrict__ flags,
+                                                      unsigned* __restrict__ visits) {
     using S = TileShape<ND>;
-    __shared__ T lds[S::LX * S::LY * S::LZ];
+    constexpr int NL = S::LX * S::LY * S::LZ;
+    __shared__ T s_t[NL];   // value, sign bit = init node
+    __shared__ T s_k[NL];   // pop key K
+    __shared__ T s_s[NL];   // pop key S
+    __shared__ uint8_t s_d[NL];  // pop key D
     __shared__ int s_faces;
     const int tid = threadIdx.x;
     const int64_t gid = list[blockIdx.x];
@@ -67,22 +73,38 @@ __global__ void __launch_bounds__(kBlock) k_fim_round(const int64_t* __restrict_
     const int tiz = (ND == 3) ? (int)(tl / ((int64_t)I.t[0] * I.t[1])) : 0;
     const int ox = tix * S::TX, oy = tiy * S::TY, oz = tiz * S::TZ;
     T* __restrict__ tt = ttpool + I.tt_off;
+    T* __restrict__ kk = kpool + I.tt_off;
+    T* __restrict__ ss = spool + I.tt_off;
+    uint8_t* __restrict__ dd = dpool + I.tt_off;
     const T* __restrict__ slow = slowpool + I.s_off;
     const int64_t nx = I.n[0], nxy = (int64_t)I.n[0] * I.n[1];
 
     if (tid == 0) s_faces = 0;
     // stage tile + face halos (edges/corners of the halo are never read by the axis stencil)
-    for (int c = tid; c < S::LX * S::LY * S::LZ; c += kBlock) {
+    for (int c = tid; c < NL; c += kBlock) {
         const int lx = c % S::LX, ly = (c / S::LX) % S::LY, lz = (ND == 3) ? c / (S::LX * S::LY) : 0;
         int nout = (lx < S::H || lx >= S::H + S::TX) + (ly < S::H || ly >= S::H + S::TY);
         if (ND == 3) nout += (lz < S::H || lz >= S::H + S::TZ);
-        T v = tinf<T>();
+        T v = tinf<T>(), kv = tinf<T>(), sv = -tinf<T>();
+        uint8_t dv = 0;
         if (nout <= 1) {
             const int gx = ox + lx - S::H, gy = oy + ly - S::H, gz = (ND == 3) ? oz + lz - S::H : 0;
-            if (gx >= 0 && gx < I.n[0] && gy >= 0 && gy < I.n[1] && (ND == 2 || (gz >= 0 && gz < I.n[2])))
-                v = tt[gx + nx * gy + nxy * gz];
+            if (gx >= 0 && gx < I.n[0] && gy >= 0 && gy < I.n[1] && (ND == 2 || (gz >= 0 && gz < I.n[2]))) {
+                const int64_t l = gx + nx * gy + nxy * gz;
+                v = tt[l];
+                dv = dd[l];
+                if (dv) {
+                    kv = kk[l];
+                    sv = ss[l];
+                } else {
+                    kv = fabs(v);
+                }
+            }
         }
-        lds[c] = v;
+        s_t[c] = v;
+        s_k[c] = kv;
+        s_s[c] = sv;
+        s_d[c] = dv;
     }
 
     // per-node static data
@@ -91,6 +113,7 @@ __global__ void __launch_bounds__(kBlock) k_fim_round(const int64_t* __restrict_
     bool act[NP];
     unsigned ok1[NP], ok2[NP];
     T sl[NP], dh[NP][3], v0[NP];
+    Key<T> key0[NP];
     int gxyz[NP][3];
 #pragma unroll
     for (int p = 0; p < NP; p++) {
@@ -130,7 +153,8 @@ __global__ void __launch_bounds__(kBlock) k_fim_round(const int64_t* __restrict_
     __syncthreads();
 #pragma unroll
     for (int p = 0; p < NP; p++) {
-        v0[p] = lds[lidx[p]];
+        v0[p] = s_t[lidx[p]];
+        key0[p] = Key<T>{s_k[lidx[p]], s_s[lidx[p]], (int)s_d[lidx[p]]};
         if (act[p] && is_fixed(v0[p])) act[p] = false;
     }
 
@@ -144,6 +168,7 @@ __global__ void __launch_bounds__(kBlock) k_fim_round(const int64_t* __restrict_
         for (int p = 0; p < NP; p++) {
             if (!act[p]) continue;
             T v1[2 * ND], v2[2 * ND];
+            Key<T> k1[2 * ND], k2[2 * ND];
             unsigned f1 = 0u, f2 = 0u;
             const int li = lidx[p];
 #pragma unroll
@@ -151,21 +176,37 @@ __global__ void __launch_bounds__(kBlock) k_fim_round(const int64_t* __restrict_
 #pragma unroll
                 for (int l = 0; l < 2; l++) {
                     const int k = 2 * ax + l;
-                    const int sgn = (l == 0) ? 1 : -1;
+                    const int o1 = li + ((l == 0) ? 1 : -1) * strides[ax];
+                    const int o2 = li + ((l == 0) ? 2 : -2) * strides[ax];
                     T a = tinf<T>(), b = tinf<T>();
-                    if ((ok1[p] >> k) & 1u) a = lds[li + sgn * strides[ax]];
-                    if ((ok2[p] >> k) & 1u) b = lds[li + 2 * sgn * strides[ax]];
+                    Key<T> ka{tinf<T>(), -tinf<T>(), 0}, kb{tinf<T>(), -tinf<T>(), 0};
+                    if ((ok1[p] >> k) & 1u) {
+                        a = s_t[o1];
+                        ka = Key<T>{s_k[o1], s_s[o1], (int)s_d[o1]};
+                    }
+                    if ((ok2[p] >> k) & 1u) {
+                        b = s_t[o2];
+                        kb = Key<T>{s_k[o2], s_s[o2], (int)s_d[o2]};
+                    }
                     f1 |= (unsigned)is_fixed(a) << k;
                     f2 |= (unsigned)is_fixed(b) << k;
                     v1[k] = fabs(a);
                     v2[k] = fabs(b);
+                    k1[k] = ka;
+                    k2[k] = kb;
                 }
             }
             int code[3];
-            const T nv = replay<ND, T>(v1, v2, f1, f2, ok1[p], ok2[p], sl[p], dh[p], code);
-            const T cur = lds[li];
-            if (nv == nv && nv != cur && !(nv >= tinf<T>() && cur >= tinf<T>())) {
-                lds[li] = nv;
+            Key<T> nk;
+            const T nv = replay<ND, T>(v1, v2, k1, k2, f1, f2, ok1[p], ok2[p], sl[p], dh[p], code, &nk);
+            const T cur = s_t[li];
+            const bool vchg = nv == nv && nv != cur && !(nv >= tinf<T>() && cur >= tinf<T>());
+            const bool kchg = nv < tinf<T>() && (nk.K != s_k[li] || nk.S != s_s[li] || nk.D != (int)s_d[li]);
+            if (vchg || kchg) {
+                s_t[li] = nv;
+                s_k[li] = nk.K;
+                s_s[li] = nk.S;
+                s_d[li] = (uint8_t)nk.D;
                 any = 1;
             }
         }
@@ -179,10 +220,18 @@ __global__ void __launch_bounds__(kBlock) k_fim_round(const int64_t* __restrict_
 #pragma unroll
     for (int p = 0; p < NP; p++) {
         if (!act[p]) continue;
-        const T v = lds[lidx[p]];
-        if (v != v0[p]) {
+        const int li = lidx[p];
+        const T v = s_t[li];
+        const Key<T> kn{s_k[li], s_s[li], (int)s_d[li]};
+        if (v != v0[p] || !keq(kn, key0[p])) {
             const int gx = gxyz[p][0], gy = gxyz[p][1], gz = gxyz[p][2];
-            tt[gx + nx * gy + nxy * gz] = v;
+            const int64_t l = gx + nx * gy + nxy * gz;
+            tt[l] = v;
+            dd[l] = (uint8_t)kn.D;
+            if (kn.D) {
+                kk[l] = kn.K;
+                ss[l] = kn.S;
+            }
             const int lx = gx - ox, ly = gy - oy, lz = gz - oz;
             if (lx < 2) faces |= 1;
             if (lx >= S::TX - 2) faces |= 2;
@@ -296,22 +345,25 @@ __global__ void k_fine_setup(const FineDev* __restrict__ fine, int nfine, const 
 // the fine box stops at the first accepted node on a box face that is not a model border, once
 // at least one coarse-coincident node has been transferred; the coarse init set is every
 // coarse-coincident fine node accepted up to and including that node.  With the converged fine
-// solution: t_c = min tt over coarse-coincident nodes, t_stop = min tt over face nodes with
-// tt >= t_c, transfer = coarse-coincident nodes with tt <= t_stop.  One block per source; the
-// transferred points are listed in coarse-window order (xlist, cap entries per source) and
-// t_stop is stored in the fine instance for the adjoint.
+// solution and its pop keys: k_c = min key over coarse-coincident nodes, k_stop = min key over
+// face nodes with key >= k_c, transfer = coarse-coincident nodes with key <= k_stop.  One block
+// per source; the transferred points are listed in coarse-window order (xlist, cap entries per
+// source) and k_stop is stored in the fine instance for the adjoint.
 template <typename T>
 __global__ void __launch_bounds__(256) k_transfer(const FineDev* __restrict__ fine, InstDev* __restrict__ finst,
-                                                  const double* __restrict__ ttf, const InstDev* __restrict__ cinst,
-                                                  T* __restrict__ ttc, uint8_t* __restrict__ cflags,
-                                                  int* __restrict__ ntransfer, SrcPt* __restrict__ xlist, int cap,
-                                                  int nd) {
+                                                  TTPools fp, const InstDev* __restrict__ cinst, T* __restrict__ ttc,
+                                                  uint8_t* __restrict__ cflags, int* __restrict__ ntransfer,
+                                                  SrcPt* __restrict__ xlist, int cap, int nd) {
     const int f = blockIdx.x;
     const FineDev F = fine[f];
     const InstDev I = finst[f];
     const InstDev C = cinst[F.cinst];
     const int64_t n = (int64_t)I.n[0] * I.n[1] * I.n[2];
-    __shared__ double red[256];
+    const double* ttf = (const double*)fp.tt + I.tt_off;
+    const double* kf = (const double*)fp.k + I.tt_off;
+    const double* sf = (const double*)fp.s + I.tt_off;
+    const uint8_t* df = fp.d + I.tt_off;
+    __shared__ Key<double> red[256];
     auto coord = [&](int64_t l, int* g) {
         g[0] = (int)(l % I.n[0]);
         g[1] = (int)((l / I.n[0]) % I.n[1]);
@@ -322,40 +374,53 @@ __global__ void __launch_bounds__(256) k_transfer(const FineDev* __restrict__ fi
             if (g[d] % F.df != 0) return false;
         return true;
     };
-    auto blockmin = [&](double v) {
+    // pop key of a fine node; init nodes pop before every other node
+    auto keyof = [&](int64_t l) {
+        const double raw = ttf[l];
+        if (signbit(raw)) return Key<double>{-EIK_INF, -EIK_INF, 0};
+        if (df[l]) return Key<double>{kf[l], sf[l], (int)df[l]};
+        return Key<double>{raw, -EIK_INF, 0};
+    };
+    auto blockmin = [&](Key<double> v) {
         red[threadIdx.x] = v;
         __syncthreads();
         for (int s = 128; s > 0; s >>= 1) {
-            if (threadIdx.x < s) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + s]);
+            if (threadIdx.x < s && klt(red[threadIdx.x + s], red[threadIdx.x])) red[threadIdx.x] = red[threadIdx.x + s];
             __syncthreads();
         }
-        const double r = red[0];
+        const Key<double> r = red[0];
         __syncthreads();
         return r;
     };
-    double m = EIK_INF;
+    const Key<double> kinf{EIK_INF, EIK_INF, 0};
+    Key<double> m = kinf;
     for (int64_t l = threadIdx.x; l < n; l += 256) {
         int g[3];
         coord(l, g);
-        if (coincident(g)) m = fmin(m, fabs(ttf[I.tt_off + l]));
+        if (coincident(g)) {
+            const Key<double> k = keyof(l);
+            if (klt(k, m)) m = k;
+        }
     }
-    const double tc = blockmin(m);
-    m = EIK_INF;
+    const Key<double> kc = blockmin(m);
+    m = kinf;
     for (int64_t l = threadIdx.x; l < n; l += 256) {
         int g[3];
         coord(l, g);
-        const double raw = ttf[I.tt_off + l];
-        if (signbit(raw)) continue;  // init nodes are never popped
+        if (signbit(ttf[l]) || !(ttf[l] < EIK_INF)) continue;  // init nodes are never popped
         bool face = false;
         for (int d = 0; d < nd; d++) {
             if (g[d] == 0 && !F.outmin[d]) face = true;
             if (g[d] == I.n[d] - 1 && !F.outmax[d]) face = true;
         }
-        if (face && raw >= tc) m = fmin(m, raw);
+        const Key<double> k = keyof(l);
+        if (face && !klt(k, kc) && klt(k, m)) m = k;
     }
-    const double tstop = blockmin(m);
+    const Key<double> ks = blockmin(m);
     if (threadIdx.x != 0) return;
-    finst[f].tstop = tstop;
+    finst[f].tstopK = ks.K;
+    finst[f].tstopS = ks.S;
+    finst[f].tstopD = ks.D;
     int nw[3] = {1, 1, 1};
     for (int d = 0; d < nd; d++) nw[d] = (I.n[d] - 1) / F.df + 1;
     int cnt = 0;
@@ -364,8 +429,8 @@ __global__ void __launch_bounds__(256) k_transfer(const FineDev* __restrict__ fi
             for (int wx = 0; wx < nw[0]; wx++) {
                 const int64_t lf = (int64_t)wx * F.df +
                                    (int64_t)I.n[0] * ((int64_t)wy * F.df + (int64_t)I.n[1] * ((int64_t)wz * F.df));
-                const double t = fabs(ttf[I.tt_off + lf]);
-                if (!(t <= tstop)) continue;
+                if (!(fabs(ttf[lf]) < EIK_INF) || klt(ks, keyof(lf))) continue;
+                const double t = fabs(ttf[lf]);
                 const int cg[3] = {F.i1c[0] + wx, F.i1c[1] + wy, F.i1c[2] + wz};
                 const int64_t lc = cg[0] + (int64_t)C.n[0] * (cg[1] + (int64_t)C.n[1] * cg[2]);
                 ttc[C.tt_off + lc] = -(T)t;
@@ -400,7 +465,7 @@ __global__ void k_export(const T* __restrict__ src, double* __restrict__ dst, in
 static unsigned nblocks(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 template <int ND, typename T>
-static int fim_run_t(eik_ctx* ctx, const InstDev* d_inst, int ninst, int64_t ntiles, T* ttpool,
+static int fim_run_t(eik_ctx* ctx, const InstDev* d_inst, int ninst, int64_t ntiles, const TTPools& P,
                      const T* slowpool, FimStats* fs) {
     DevBuf& list = ctx->buf("fim_list");
     if (list.ensure((size_t)ntiles * sizeof(int64_t))) return ctx->fail(EIK_ENOMEM, "out of device memory (tile list)");
@@ -427,8 +492,8 @@ static int fim_run_t(eik_ctx* ctx, const InstDev* d_inst, int ninst, int64_t nti
         if (++st.rounds > maxrounds) return ctx->fail(EIK_ENOCONV, "traveltime solve did not converge");
         st.activations += nact;
         CK(hipEventRecord(ctx->kev[0], ctx->stream));
-        k_fim_round<ND, T><<<nact, kBlock, 0, ctx->stream>>>((const int64_t*)list.p, d_inst, ninst, ttpool, slowpool,
-                                                               flags, d_cnt + 1);
+        k_fim_round<ND, T><<<nact, kBlock, 0, ctx->stream>>>((const int64_t*)list.p, d_inst, ninst, (T*)P.tt, (T*)P.k,
+                                                               (T*)P.s, P.d, slowpool, flags, d_cnt + 1);
         CK(hipGetLastError());
         CK(hipEventRecord(ctx->kev[1], ctx->stream));
         pending = true;
@@ -440,15 +505,13 @@ static int fim_run_t(eik_ctx* ctx, const InstDev* d_inst, int ninst, int64_t nti
     return EIK_OK;
 }
 
-int fim_run(eik_ctx* ctx, int nd, int prec, const InstDev* d_inst, int ninst, int64_t ntiles, void* ttpool,
+int fim_run(eik_ctx* ctx, int nd, int prec, const InstDev* d_inst, int ninst, int64_t ntiles, const TTPools& pools,
             const void* slowpool, FimStats* fs) {
     if (nd == 2)
-        return prec == EIK_PREC_FP64
-                   ? fim_run_t<2, double>(ctx, d_inst, ninst, ntiles, (double*)ttpool, (const double*)slowpool, fs)
-                   : fim_run_t<2, float>(ctx, d_inst, ninst, ntiles, (float*)ttpool, (const float*)slowpool, fs);
-    return prec == EIK_PREC_FP64
-               ? fim_run_t<3, double>(ctx, d_inst, ninst, ntiles, (double*)ttpool, (const double*)slowpool, fs)
-               : fim_run_t<3, float>(ctx, d_inst, ninst, ntiles, (float*)ttpool, (const float*)slowpool, fs);
+        return prec == EIK_PREC_FP64 ? fim_run_t<2, double>(ctx, d_inst, ninst, ntiles, pools, (const double*)slowpool, fs)
+                                     : fim_run_t<2, float>(ctx, d_inst, ninst, ntiles, pools, (const float*)slowpool, fs);
+    return prec == EIK_PREC_FP64 ? fim_run_t<3, double>(ctx, d_inst, ninst, ntiles, pools, (const double*)slowpool, fs)
+                                 : fim_run_t<3, float>(ctx, d_inst, ninst, ntiles, pools, (const float*)slowpool, fs);
 }
 
 // ----------------------------------------------------------------- forward pass of one call
@@ -477,7 +540,14 @@ int forward_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, int prec, cons
     const int64_t ntiles_c = nsrc * tiles_per;
     const size_t esz = prec == EIK_PREC_FP64 ? sizeof(double) : sizeof(float);
     DevBuf& ttc = ctx->buf("tt_coarse");
-    if (ttc.ensure((size_t)nsrc * N * esz)) return ctx->fail(EIK_ENOMEM, "out of device memory (traveltime grids)");
+    DevBuf& kc = ctx->buf("key_K");
+    DevBuf& sc = ctx->buf("key_S");
+    DevBuf& dc = ctx->buf("key_D");
+    if (ttc.ensure((size_t)nsrc * N * esz) || kc.ensure((size_t)nsrc * N * esz) || sc.ensure((size_t)nsrc * N * esz) ||
+        dc.ensure((size_t)nsrc * N))
+        return ctx->fail(EIK_ENOMEM, "out of device memory (traveltime grids)");
+    CK(hipMemsetAsync(dc.p, 0, (size_t)nsrc * N, st));
+    const TTPools cpools{ttc.p, kc.p, sc.p, (uint8_t*)dc.p};
     DevBuf& dci = ctx->buf("inst_coarse");
     if (dci.ensure(cinst.size() * sizeof(InstDev))) return ctx->fail(EIK_ENOMEM, "out of device memory");
     CK(hipMemcpyAsync(dci.p, cinst.data(), cinst.size() * sizeof(InstDev), hipMemcpyHostToDevice, st));
@@ -516,9 +586,16 @@ int forward_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, int prec, cons
         DevBuf& dfd = ctx->buf("fine_desc");
         DevBuf& ttf = ctx->buf("tt_fine");
         DevBuf& slf = ctx->buf("slow_fine");
+        DevBuf& kf = ctx->buf("keyf_K");
+        DevBuf& sf = ctx->buf("keyf_S");
+        DevBuf& df = ctx->buf("keyf_D");
         if (dfi.ensure(finst.size() * sizeof(InstDev)) || dfd.ensure(fine.size() * sizeof(FineDev)) ||
-            ttf.ensure((size_t)nfine_pts * sizeof(double)) || slf.ensure((size_t)nfine_pts * sizeof(double)))
+            ttf.ensure((size_t)nfine_pts * sizeof(double)) || slf.ensure((size_t)nfine_pts * sizeof(double)) ||
+            kf.ensure((size_t)nfine_pts * sizeof(double)) || sf.ensure((size_t)nfine_pts * sizeof(double)) ||
+            df.ensure((size_t)nfine_pts))
             return ctx->fail(EIK_ENOMEM, "out of device memory (refined grids)");
+        CK(hipMemsetAsync(df.p, 0, (size_t)nfine_pts, st));
+        const TTPools fpools{ttf.p, kf.p, sf.p, (uint8_t*)df.p};
         CK(hipMemcpyAsync(dfi.p, finst.data(), finst.size() * sizeof(InstDev), hipMemcpyHostToDevice, st));
         CK(hipMemcpyAsync(dfd.p, fine.data(), fine.size() * sizeof(FineDev), hipMemcpyHostToDevice, st));
         dim3 gs(64, (unsigned)nsrc);
@@ -542,7 +619,7 @@ int forward_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, int prec, cons
         CK(hipGetLastError());
         tm.mark();
         FimStats fst{};
-        int rc = fim_run(ctx, nd, EIK_PREC_FP64, (const InstDev*)dfi.p, (int)nsrc, ntiles_f, ttf.p, slf.p, &fst);
+        int rc = fim_run(ctx, nd, EIK_PREC_FP64, (const InstDev*)dfi.p, (int)nsrc, ntiles_f, fpools, slf.p, &fst);
         if (rc) return rc;
         ctx->stats[1] = tm.mark();
         DevBuf& ntr = ctx->buf("ntransfer");
@@ -557,14 +634,13 @@ int forward_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, int prec, cons
         if (ntr.ensure((size_t)nsrc * sizeof(int)) || xl.ensure((size_t)nsrc * cap * sizeof(SrcPt)))
             return ctx->fail(EIK_ENOMEM, "out of device memory");
         if (prec == EIK_PREC_FP64)
-            k_transfer<double><<<(unsigned)nsrc, 256, 0, st>>>((const FineDev*)dfd.p, (InstDev*)dfi.p,
-                                                              (const double*)ttf.p, (const InstDev*)dci.p,
-                                                              (double*)ttc.p, (uint8_t*)flags.p, (int*)ntr.p,
-                                                              (SrcPt*)xl.p, cap, nd);
+            k_transfer<double><<<(unsigned)nsrc, 256, 0, st>>>((const FineDev*)dfd.p, (InstDev*)dfi.p, fpools,
+                                                              (const InstDev*)dci.p, (double*)ttc.p, (uint8_t*)flags.p,
+                                                              (int*)ntr.p, (SrcPt*)xl.p, cap, nd);
         else
-            k_transfer<float><<<(unsigned)nsrc, 256, 0, st>>>((const FineDev*)dfd.p, (InstDev*)dfi.p,
-                                                             (const double*)ttf.p, (const InstDev*)dci.p, (float*)ttc.p,
-                                                             (uint8_t*)flags.p, (int*)ntr.p, (SrcPt*)xl.p, cap, nd);
+            k_transfer<float><<<(unsigned)nsrc, 256, 0, st>>>((const FineDev*)dfd.p, (InstDev*)dfi.p, fpools,
+                                                             (const InstDev*)dci.p, (float*)ttc.p, (uint8_t*)flags.p,
+                                                             (int*)ntr.p, (SrcPt*)xl.p, cap, nd);
         CK(hipGetLastError());
     } else {
         for (int64_t s = 0; s < nsrc; s++) {
@@ -594,7 +670,7 @@ int forward_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, int prec, cons
     }
     const void* slow = prec == EIK_PREC_FP64 ? (const void*)M->d_slow64 : (const void*)M->d_slow32;
     FimStats cst{};
-    int rc = fim_run(ctx, nd, prec, (const InstDev*)dci.p, (int)nsrc, ntiles_c, ttc.p, slow, &cst);
+    int rc = fim_run(ctx, nd, prec, (const InstDev*)dci.p, (int)nsrc, ntiles_c, cpools, slow, &cst);
     if (rc) return rc;
     ctx->stats[2] = tm.mark();
     ctx->stats[4] = cst.activations;

@@ -31,7 +31,12 @@ struct InstDev {
     double dphrad;       // spherical 3D: dphi in radians
     int32_t kind;        // EIK_GRID*
     int32_t pad;
-    double tstop;        // adjoint: nodes with tt <= tstop form the system (+inf on coarse grids)
+    // adjoint: nodes that pop no later than the refinement stop form the system; the stop's pop
+    // key (K, S, D) -- (+inf, -inf, 0) on coarse grids
+    double tstopK;
+    double tstopS;
+    int32_t tstopD;
+    int32_t pad2;
 };
 
 // Refined box around one source (createfinegrid, src/EikSolv/eikmainfwd.jl:950-1104)
@@ -333,7 +338,14 @@ struct FimStats {
     double sweeps = 0;         // local relaxation sweeps summed over activations
     double kernel_ms = 0;      // summed duration of the round launches
 };
-int fim_run(eik_ctx* ctx, int nd, int prec, const InstDev* d_inst, int ninst, int64_t ntiles, void* ttpool,
+// traveltime pool plus its pop-key pools (K, S in the pool's precision, D as bytes)
+struct TTPools {
+    void* tt;
+    void* k;
+    void* s;
+    uint8_t* d;
+};
+int fim_run(eik_ctx* ctx, int nd, int prec, const InstDev* d_inst, int ninst, int64_t ntiles, const TTPools& pools,
             const void* slowpool, FimStats* fs);
 
 }  // namespace eik

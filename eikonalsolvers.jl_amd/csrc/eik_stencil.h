@@ -8,12 +8,12 @@
 // were accepted.  replay() re-enacts that sequence locally from the current neighbour values:
 //   1. if any adjacent node is a source/init node (accepted before the march starts), evaluate
 //      the stencil with only those accepted (initial narrow band, eikmainfwd.jl:590-621);
-//   2. take the other adjacent neighbours in increasing traveltime order (equal values form one
-//      batch, eikmainfwd.jl:647-666); while the next one is smaller than the current value it
-//      would be accepted first, so re-evaluate with every node of traveltime <= its value
+//   2. take the other adjacent neighbours in the heap's pop order (their pop keys, below; equal
+//      keys form one batch, eikmainfwd.jl:647-666); while the next one pops before this node it
+//      would be accepted first, so re-evaluate with every node that pops no later than it
 //      counted as accepted (this also decides whether the 2-away nodes of the 2nd-order stencil
 //      were accepted at that moment);
-//   3. stop at the first neighbour that is not smaller: the heap would pop this node first.
+//   3. stop at the first neighbour that does not pop first: the heap would pop this node first.
 // At the fixed point of this update over the whole grid every node holds the value the heap
 // FMM gives it; in FP64 the arithmetic below follows the reference's operation order so the
 // results agree bit for bit.
@@ -33,9 +33,8 @@ namespace eik {
 
 // ---------------------------------------------------------------- FP64 (reference arithmetic)
 template <int ND>
-__device__ __forceinline__ double eval64(const double* v1, const double* v2, unsigned f1, unsigned f2,
-                                         unsigned ok1, unsigned ok2, double thr, double s,
-                                         const double* dh, int* code) {
+__device__ __forceinline__ double eval64(const double* v1, const double* v2, unsigned acc1, unsigned acc2,
+                                         double s, const double* dh, int* code) {
 #pragma clang fp contract(off)
     double alpha = 0.0, beta = 0.0, gamma = -(s * s);
 #pragma unroll
@@ -48,14 +47,14 @@ __device__ __forceinline__ double eval64(const double* v1, const double* v2, uns
         for (int l = 0; l < 2; l++) {
             const int k = 2 * ax + l;
             const int sh = (l == 0) ? 1 : -1;
-            const bool a1 = ((ok1 >> k) & 1u) && (((f1 >> k) & 1u) || v1[k] <= thr);
+            const bool a1 = (acc1 >> k) & 1u;
             if (a1) {
                 const double tv1 = v1[k];
                 if (tv1 < cv1) {
                     cv1 = tv1;
                     use1 = true;
                     code[ax] = -sh;
-                    const bool a2 = ((ok2 >> k) & 1u) && (((f2 >> k) & 1u) || v2[k] <= thr);
+                    const bool a2 = (acc2 >> k) & 1u;
                     if (a2) {
                         const double tv2 = v2[k];
                         if (tv2 <= cv1) {
@@ -99,7 +98,7 @@ __device__ __forceinline__ double eval64(const double* v1, const double* v2, uns
 #pragma unroll
             for (int l = 0; l < 2; l++) {
                 const int k = 2 * ax + l;
-                const bool a1 = ((ok1 >> k) & 1u) && (((f1 >> k) & 1u) || v1[k] <= thr);
+                const bool a1 = (acc1 >> k) & 1u;
                 if (a1 && v1[k] < cv1) {
                     cv1 = v1[k];
                     use1 = true;
@@ -124,7 +123,7 @@ __device__ __forceinline__ double eval64(const double* v1, const double* v2, uns
 #pragma unroll
             for (int l = 0; l < 2; l++) {
                 const int k = 2 * ax + l;
-                const bool a1 = ((ok1 >> k) & 1u) && (((f1 >> k) & 1u) || v1[k] <= thr);
+                const bool a1 = (acc1 >> k) & 1u;
                 if (a1 && v1[k] < cvv[ax]) {
                     cvv[ax] = v1[k];
                     cc[ax] = (l == 0) ? -1 : 1;
@@ -151,9 +150,8 @@ __device__ __forceinline__ double eval64(const double* v1, const double* v2, uns
 // differences of nearby traveltimes, so FP32 keeps ~7 digits of the increment instead of losing
 // them to the cancellation in beta^2 - 4 alpha gamma (both terms ~ (alpha T)^2).
 template <int ND>
-__device__ __forceinline__ float eval32(const float* v1, const float* v2, unsigned f1, unsigned f2,
-                                        unsigned ok1, unsigned ok2, float thr, float s, const float* dh,
-                                        int* code) {
+__device__ __forceinline__ float eval32(const float* v1, const float* v2, unsigned acc1, unsigned acc2, float s,
+                                        const float* dh, int* code) {
     float t1a[3], t2a[3], al[3];
     int mode[3];  // 0 unused, 1 first order, 2 second order
 #pragma unroll
@@ -165,14 +163,14 @@ __device__ __forceinline__ float eval32(const float* v1, const float* v2, unsign
         for (int l = 0; l < 2; l++) {
             const int k = 2 * ax + l;
             const int sh = (l == 0) ? 1 : -1;
-            const bool a1 = ((ok1 >> k) & 1u) && (((f1 >> k) & 1u) || v1[k] <= thr);
+            const bool a1 = (acc1 >> k) & 1u;
             if (a1) {
                 const float tv1 = v1[k];
                 if (tv1 < cv1) {
                     cv1 = tv1;
                     use1 = true;
                     code[ax] = -sh;
-                    const bool a2 = ((ok2 >> k) & 1u) && (((f2 >> k) & 1u) || v2[k] <= thr);
+                    const bool a2 = (acc2 >> k) & 1u;
                     if (a2) {
                         const float tv2 = v2[k];
                         if (tv2 <= cv1) {
@@ -224,7 +222,7 @@ __device__ __forceinline__ float eval32(const float* v1, const float* v2, unsign
 #pragma unroll
             for (int l = 0; l < 2; l++) {
                 const int k = 2 * ax + l;
-                const bool a1 = ((ok1 >> k) & 1u) && (((f1 >> k) & 1u) || v1[k] <= thr);
+                const bool a1 = (acc1 >> k) & 1u;
                 if (a1 && v1[k] < c1[ax]) {
                     c1[ax] = v1[k];
                     u[ax] = true;
@@ -257,33 +255,55 @@ __device__ __forceinline__ float eval32(const float* v1, const float* v2, unsign
     return r + (B + sqrtf(disc)) / A;
 }
 
-// ---------------------------------------------------------------- local FMM replay
+// ---------------------------------------------------------------- heap pop keys
+// The heap pops nodes in value order except after a non-causal 2nd-order update: a node whose
+// new value falls below the value of the neighbour whose acceptance triggered it is popped right
+// after that neighbour, and such nodes can trigger further ones ("cascade").  The pop order is
+// reproduced by the key (K, S, D): K = level (for an ordinary node its own value), S = sub-level
+// inside a cascade (-inf for ordinary nodes), D = cascade depth.  A node whose replay ends below
+// the key of its trigger c gets (K_c, max(t, S_c), D_c + 1); otherwise (t, -inf, 0).  Keys compare
+// lexicographically; equal keys pop in one batch.
+template <typename T>
+struct Key {
+    T K, S;
+    int D;
+};
+template <typename T>
+__device__ __forceinline__ bool klt(const Key<T>& a, const Key<T>& b) {
+    if (a.K != b.K) return a.K < b.K;
+    if (a.S != b.S) return a.S < b.S;
+    return a.D < b.D;
+}
+template <typename T>
+__device__ __forceinline__ bool keq(const Key<T>& a, const Key<T>& b) {
+    return a.K == b.K && a.S == b.S && a.D == b.D;
+}
+// A cascade can never be deeper than this; a longer chain can only come from two nodes feeding
+// each other stale keys during the iteration, and ignoring such a candidate breaks the cycle.
+constexpr int kMaxCascade = 16;
+
 template <int ND, typename T>
-__device__ __forceinline__ T eval_any(const T* v1, const T* v2, unsigned f1, unsigned f2, unsigned ok1,
-                                      unsigned ok2, T thr, T s, const T* dh, int* code);
+__device__ __forceinline__ T eval_any(const T* v1, const T* v2, unsigned acc1, unsigned acc2, T s, const T* dh,
+                                      int* code);
 template <>
-__device__ __forceinline__ double eval_any<2, double>(const double* v1, const double* v2, unsigned f1, unsigned f2,
-                                                      unsigned ok1, unsigned ok2, double thr, double s,
-                                                      const double* dh, int* code) {
-    return eval64<2>(v1, v2, f1, f2, ok1, ok2, thr, s, dh, code);
+__device__ __forceinline__ double eval_any<2, double>(const double* v1, const double* v2, unsigned acc1, unsigned acc2,
+                                                      double s, const double* dh, int* code) {
+    return eval64<2>(v1, v2, acc1, acc2, s, dh, code);
 }
 template <>
-__device__ __forceinline__ double eval_any<3, double>(const double* v1, const double* v2, unsigned f1, unsigned f2,
-                                                      unsigned ok1, unsigned ok2, double thr, double s,
-                                                      const double* dh, int* code) {
-    return eval64<3>(v1, v2, f1, f2, ok1, ok2, thr, s, dh, code);
+__device__ __forceinline__ double eval_any<3, double>(const double* v1, const double* v2, unsigned acc1, unsigned acc2,
+                                                      double s, const double* dh, int* code) {
+    return eval64<3>(v1, v2, acc1, acc2, s, dh, code);
 }
 template <>
-__device__ __forceinline__ float eval_any<2, float>(const float* v1, const float* v2, unsigned f1, unsigned f2,
-                                                    unsigned ok1, unsigned ok2, float thr, float s, const float* dh,
-                                                    int* code) {
-    return eval32<2>(v1, v2, f1, f2, ok1, ok2, thr, s, dh, code);
+__device__ __forceinline__ float eval_any<2, float>(const float* v1, const float* v2, unsigned acc1, unsigned acc2,
+                                                    float s, const float* dh, int* code) {
+    return eval32<2>(v1, v2, acc1, acc2, s, dh, code);
 }
 template <>
-__device__ __forceinline__ float eval_any<3, float>(const float* v1, const float* v2, unsigned f1, unsigned f2,
-                                                    unsigned ok1, unsigned ok2, float thr, float s, const float* dh,
-                                                    int* code) {
-    return eval32<3>(v1, v2, f1, f2, ok1, ok2, thr, s, dh, code);
+__device__ __forceinline__ float eval_any<3, float>(const float* v1, const float* v2, unsigned acc1, unsigned acc2,
+                                                    float s, const float* dh, int* code) {
+    return eval32<3>(v1, v2, acc1, acc2, s, dh, code);
 }
 
 template <typename T>
@@ -297,12 +317,14 @@ __device__ __forceinline__ float tinf<float>() {
     return EIK_INFF;
 }
 
-// Returns the value the heap FMM would give the node (see the file comment); code receives the
-// derivative codes of the last stencil evaluation (codeDeriv, eikmainfwd.jl:794-809).
+// Local replay of the heap (see the file comment).  v1/v2: adjacent / 2-away values; k1/k2 their
+// keys; f1/f2 fixed (init) bits; ok1/ok2 grid-boundary bits.  Returns the value and sets *kout to
+// the node's pop key and code to the derivative codes of the last stencil evaluation.
 template <int ND, typename T>
-__device__ __forceinline__ T replay(const T* v1, const T* v2, unsigned f1, unsigned f2, unsigned ok1, unsigned ok2,
-                                    T s, const T* dh, int* code) {
-    T c[2 * ND];
+__device__ __forceinline__ T replay(const T* v1, const T* v2, const Key<T>* k1, const Key<T>* k2, unsigned f1,
+                                    unsigned f2, unsigned ok1, unsigned ok2, T s, const T* dh, int* code,
+                                    Key<T>* kout) {
+    int ord[2 * ND];
     int nc = 0;
     bool haveinit = false;
 #pragma unroll
@@ -310,26 +332,42 @@ __device__ __forceinline__ T replay(const T* v1, const T* v2, unsigned f1, unsig
         if (!((ok1 >> k) & 1u)) continue;
         if ((f1 >> k) & 1u) {
             haveinit = true;
-        } else if (v1[k] < tinf<T>()) {
-            // insertion into the sorted candidate list
-            T x = v1[k];
+        } else if (v1[k] < tinf<T>() && k1[k].D < kMaxCascade) {
             int j = nc++;
-            while (j > 0 && c[j - 1] > x) {
-                c[j] = c[j - 1];
+            while (j > 0 && klt(k1[k], k1[ord[j - 1]])) {
+                ord[j] = ord[j - 1];
                 j--;
             }
-            c[j] = x;
+            ord[j] = k;
         }
     }
     T cur = tinf<T>();
+    Key<T> kz{tinf<T>(), -tinf<T>(), 0};
 #pragma unroll
     for (int d = 0; d < ND; d++) code[d] = 0;
-    if (haveinit) cur = eval_any<ND, T>(v1, v2, f1, f2, ok1, ok2, -tinf<T>(), s, dh, code);
-    for (int i = 0; i < nc; i++) {
-        if (i + 1 < nc && c[i + 1] == c[i]) continue;  // one batch of equal values
-        if (!(c[i] < cur)) break;
-        cur = eval_any<ND, T>(v1, v2, f1, f2, ok1, ok2, c[i], s, dh, code);
+    if (haveinit) {
+        cur = eval_any<ND, T>(v1, v2, f1 & ok1, f2 & ok2, s, dh, code);
+        kz = Key<T>{cur, -tinf<T>(), 0};
     }
+    for (int i = 0; i < nc; i++) {
+        const Key<T> c = k1[ord[i]];
+        if (i + 1 < nc && keq(k1[ord[i + 1]], c)) continue;  // one batch of equal keys
+        if (!klt(c, kz)) break;
+        unsigned acc1 = 0u, acc2 = 0u;
+#pragma unroll
+        for (int k = 0; k < 2 * ND; k++) {
+            const bool a1 = ((ok1 >> k) & 1u) && (((f1 >> k) & 1u) || (v1[k] < tinf<T>() && !klt(c, k1[k])));
+            const bool a2 = ((ok2 >> k) & 1u) && (((f2 >> k) & 1u) || (v2[k] < tinf<T>() && !klt(c, k2[k])));
+            acc1 |= (unsigned)a1 << k;
+            acc2 |= (unsigned)a2 << k;
+        }
+        cur = eval_any<ND, T>(v1, v2, acc1, acc2, s, dh, code);
+        if (cur >= c.K)
+            kz = Key<T>{cur, -tinf<T>(), 0};
+        else
+            kz = Key<T>{c.K, cur > c.S ? cur : c.S, c.D + 1};
+    }
+    *kout = kz;
     return cur;
 }
 
