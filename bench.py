@@ -60,9 +60,9 @@ def main() -> int:
 
     import eikonalsolvers as eik
     from eikonalsolvers.grids import ExtraParams
-    from eikonalsolvers.synthetic import c2_3d
+    from eikonalsolvers.synthetic import c2_3d, c2_observed
 
-    grd, vel, src, recs, _velobs = c2_3d()
+    grd, vel, src, recs, velobs = c2_3d()
     nsrc = src.shape[0]
     model = eik.EikModel(vel, grd)
 
@@ -81,7 +81,24 @@ def main() -> int:
             "achieved_gbs": (visits * BYTES_PER_VISIT[prec] / 1e9) / (kern_ms / 1e3) if kern_ms > 0 else None,
         }
 
-    # parity spot check of the timed configuration against the oracle (same inputs)
+    # gradient evaluations: observed picks from the second model plus N(0, 0.01^2) noise
+    obsmodel = eik.EikModel(velobs, grd)
+    pobs, sobs = c2_observed(obsmodel.traveltime(src, recs, extraparams=ExtraParams()), nsrc)
+    obsmodel.close()
+    grad = {}
+    for prec in ("fp64", "mixed"):
+        ep = ExtraParams(precision=prec)
+        for _ in range(args.warmup):
+            model.gradient(src, recs, pobs, sobs, "gradvelandsrcloc", extraparams=ep)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            gv, gs, misf = model.gradient(src, recs, pobs, sobs, "gradvelandsrcloc", extraparams=ep)
+        dt = time.perf_counter() - t0
+        grad[prec] = {"evals_per_s": nsrc * args.steps / dt, "ms_per_call": 1e3 * dt / args.steps, "gv": gv, "gs": gs,
+                      "misf": misf}
+    g64, gmx = grad["fp64"], grad["mixed"]
+    grad_rel_l2 = float(np.linalg.norm(gmx["gv"] - g64["gv"]) / np.linalg.norm(g64["gv"]))
+
     out = {
         "metric": METRIC,
         "value": round(res["fp64"]["solves_per_s"], 3),
@@ -100,6 +117,10 @@ def main() -> int:
                    "nsrc": nsrc, "grid": list(grd.grsize), "nrec_per_src": 64},
         "mixed_precision": {"solves_per_s": round(res["mixed"]["solves_per_s"], 3),
                             "ms_per_step": round(res["mixed"]["ms_per_step"], 3), "dtype": "f32 stencil / f64 near-source"},
+        "gradient": {"evals_per_s_fp64": round(g64["evals_per_s"], 3), "ms_per_call_fp64": round(g64["ms_per_call"], 3),
+                     "evals_per_s_mixed": round(gmx["evals_per_s"], 3), "ms_per_call_mixed": round(gmx["ms_per_call"], 3),
+                     "whichgrad": "gradvelandsrcloc", "misfit_fp64": g64["misf"],
+                     "mixed_vs_fp64_gradvel_rel_l2": grad_rel_l2},
     }
     f = res["fp64"]
     out["roofline"] = {
