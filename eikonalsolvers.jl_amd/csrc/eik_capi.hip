@@ -254,14 +254,32 @@ static int load_sources(eik_ctx* ctx, eik_model* M, int64_t nsrc, const double* 
     return EIK_OK;
 }
 
+// Mixed precision keeps the solve near the source in FP64.  Without refinement that region is an
+// FP64 box of +-kNearSourceRadius nodes at the coarse spacing, solved with the refinement machinery
+// at downscalefactor 1: the nodes it hands to the FP32 solve are those the heap accepts before the
+// front reaches the box faces, whose values do not depend on the box, so the coarse problem is
+// the reference's unrefined one.
+static const int32_t kNearSourceRadius = 4;
+static eik_params effective_params(const eik_params* p) {
+    eik_params e = *p;
+    if (e.precision == EIK_PREC_MIXED && !e.refinearoundsrc) {
+        e.refinearoundsrc = 1;
+        e.downscalefactor = 1;
+        e.noderadius = kNearSourceRadius;
+    }
+    return e;
+}
+
 int eik_traveltime(eik_ctx* ctx, eik_model* model, int64_t nsrc, const double* coordsrc, const int64_t* rec_off,
-                   const double* coordrec, const eik_params* params, double* ttpicks, double* ttgrid) {
+                   const double* coordrec, const eik_params* params_in, double* ttpicks, double* ttgrid) {
     if (!ctx || !model) return EIK_EINVAL;
     std::lock_guard<std::mutex> lk(ctx->mu);
     (void)hipSetDevice(ctx->device);
     SourceSet S;
-    int rc = load_sources(ctx, model, nsrc, coordsrc, rec_off, coordrec, params, S);
+    int rc = load_sources(ctx, model, nsrc, coordsrc, rec_off, coordrec, params_in, S);
     if (rc) return rc;
+    const eik_params pe = effective_params(params_in);
+    const eik_params* params = &pe;
     if (S.nrec() > 0 && !ttpicks) return ctx->fail(EIK_EINVAL, "null picks buffer");
     if (nsrc == 0) return EIK_OK;
     for (double& x : ctx->stats) x = 0.0;
@@ -296,15 +314,17 @@ int eik_misfit(eik_ctx* ctx, eik_model* model, int64_t nsrc, const double* coord
 
 int eik_gradient(eik_ctx* ctx, eik_model* model, int64_t nsrc, const double* coordsrc, const int64_t* rec_off,
                  const double* coordrec, const double* pickobs, const double* stdobs, int32_t whichgrad,
-                 const eik_params* params, double* gradvel, double* gradsrc, double* misf) {
+                 const eik_params* params_in, double* gradvel, double* gradsrc, double* misf) {
     if (!ctx || !model || !misf) return EIK_EINVAL;
     std::lock_guard<std::mutex> lk(ctx->mu);
     (void)hipSetDevice(ctx->device);
     if (whichgrad < EIK_GRADVEL || whichgrad > EIK_GRADVELANDSRCLOC)
         return ctx->fail(EIK_EASSERT, "Symbol 'whichgrad' not in (:gradvel, :gradsrcloc, :gradvelandsrcloc)");
     SourceSet S;
-    int rc = load_sources(ctx, model, nsrc, coordsrc, rec_off, coordrec, params, S);
+    int rc = load_sources(ctx, model, nsrc, coordsrc, rec_off, coordrec, params_in, S);
     if (rc) return rc;
+    const eik_params pe = effective_params(params_in);
+    const eik_params* params = &pe;
     const int kind = model->g.kind;
     if (kind == EIK_GRID3D_SPHERE)
         return ctx->fail(EIK_EINVAL, "createsparsederivativematrices!(): Not yet implemented for Grid3DSphere.");
