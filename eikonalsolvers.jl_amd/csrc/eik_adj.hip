@@ -130,11 +130,12 @@ __global__ void __launch_bounds__(kAB) k_adj_prep(const InstDev* __restrict__ in
                     v2[k] = fabs(b);
                 }
             }
-            T dh[3];
-            node_spacing<T>(I, g, dh);
+            Metric<T> mt;
+            node_spacing<T>(I, g, mt.dh);
+            metric_coefs(mt, ND);
             int code[3] = {0, 0, 0};
             Key<T> kz;
-            (void)replay<ND, T>(v1, v2, k1, k2, f1, f2, ok1, ok2, slow[l], dh, code, &kz);
+            (void)replay<ND, T>(v1, v2, k1, k2, f1, f2, ok1, ok2, slow[l], mt, code, &kz);
 #pragma unroll
             for (int d = 0; d < ND; d++) {
                 const int c = code[d];

@@ -117,12 +117,17 @@ int eik_ctx_create(int32_t device, eik_ctx** out) {
     eik_ctx* ctx = new eik_ctx();
     ctx->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipMalloc(&ctx->d_small, 64) != hipSuccess || hipHostMalloc(&ctx->h_small, 64, hipHostMallocDefault) != hipSuccess ||
-        hipEventCreate(&ctx->ev[0]) != hipSuccess || hipEventCreate(&ctx->ev[1]) != hipSuccess ||
-        hipEventCreate(&ctx->kev[0]) != hipSuccess || hipEventCreate(&ctx->kev[1]) != hipSuccess) {
-        delete ctx;
+        hipMalloc(&ctx->d_small, kSmallBytes) != hipSuccess ||
+        hipHostMalloc(&ctx->h_small, kSmallBytes, hipHostMallocDefault) != hipSuccess ||
+        hipEventCreate(&ctx->ev[0]) != hipSuccess || hipEventCreate(&ctx->ev[1]) != hipSuccess) {
+        eik_ctx_destroy(ctx);
         return EIK_EDEVICE;
     }
+    for (auto& e : ctx->kev)
+        if (hipEventCreate(&e) != hipSuccess) {
+            eik_ctx_destroy(ctx);
+            return EIK_EDEVICE;
+        }
     *out = ctx;
     return EIK_OK;
 }
@@ -134,10 +139,10 @@ void eik_ctx_destroy(eik_ctx* ctx) {
     for (auto& kv : ctx->bufs) kv.second.release();
     if (ctx->d_small) (void)hipFree(ctx->d_small);
     if (ctx->h_small) (void)hipHostFree(ctx->h_small);
-    for (int i = 0; i < 2; i++) {
+    for (int i = 0; i < 2; i++)
         if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
-        if (ctx->kev[i]) (void)hipEventDestroy(ctx->kev[i]);
-    }
+    for (auto& e : ctx->kev)
+        if (e) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -162,8 +167,12 @@ static int check_grid(eik_ctx* ctx, const eik_grid* g) {
 
 static int update_vel_locked(eik_model* M, const double* vel) {
     eik_ctx* ctx = M->ctx;
-    for (int64_t i = 0; i < M->N; i++)
+    double sm = 0.0;
+    for (int64_t i = 0; i < M->N; i++) {
         if (!(vel[i] > 0.0)) return ctx->fail(EIK_EASSERT, "all(vel.>0.0)");
+        sm += 1.0 / vel[i];
+    }
+    M->smean = sm / (double)M->N;
     (void)hipSetDevice(ctx->device);
     CK(hipMemcpyAsync(M->d_vel, vel, M->N * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
     if (model_slowness(M)) return ctx->fail(EIK_EDEVICE, "slowness kernel failed");
@@ -347,7 +356,7 @@ int eik_gradient(eik_ctx* ctx, eik_model* model, int64_t nsrc, const double* coo
 
 int eik_last_stats(const eik_ctx* ctx, double* out, int32_t nout) {
     if (!ctx || !out) return EIK_EINVAL;
-    for (int i = 0; i < nout && i < 8; i++) out[i] = ctx->stats[i];
+    for (int i = 0; i < nout && i < 12; i++) out[i] = ctx->stats[i];
     return EIK_OK;
 }
 

@@ -13,23 +13,26 @@
 namespace eik {
 
 // ================================================================= tile geometry
+// A tile is relaxed by one 256-thread block: 16x16 nodes in 2D (one per thread), 8x8x8 in 3D (two
+// per thread), staged in LDS with a 2-deep halo on every face (the 2nd-order stencil reads two
+// nodes along each axis; edges and corners of the halo are never read).
 template <int ND>
 struct TileShape;
 template <>
 struct TileShape<2> {
     static constexpr int TX = 16, TY = 16, TZ = 1, H = 2;
     static constexpr int LX = TX + 2 * H, LY = TY + 2 * H, LZ = 1;
-    static constexpr int NODES_PER_THREAD = 1;
-    static constexpr int MAXIT = 96;
+    static constexpr int NODES = TX * TY;
+    static constexpr int MAXIT = 128;
 };
 template <>
 struct TileShape<3> {
     static constexpr int TX = 8, TY = 8, TZ = 8, H = 2;
     static constexpr int LX = TX + 2 * H, LY = TY + 2 * H, LZ = TZ + 2 * H;
-    static constexpr int NODES_PER_THREAD = 2;
-    static constexpr int MAXIT = 64;
+    static constexpr int NODES = TX * TY * TZ;
+    static constexpr int MAXIT = 96;
 };
-constexpr int kBlock = 256;
+constexpr int kTileThreads = 256;
 
 __device__ __forceinline__ int find_inst(const InstDev* __restrict__ inst, int ninst, int64_t gid) {
     int lo = 0, hi = ninst - 1;
@@ -48,218 +51,399 @@ __device__ __forceinline__ bool is_fixed(T raw) {
     return signbit(raw);
 }
 
-// one round: every listed tile is relaxed to local convergence in LDS
+// A pop key K as a float no larger than it (changes and pending values are compared against node
+// keys conservatively, so rounding must never move them up).
+__device__ __forceinline__ float key_down(double k) { return __double2float_rd(k); }
+__device__ __forceinline__ float key_down(float k) { return k; }
+
+// One round: every listed tile is relaxed in LDS until nothing within the band is left.  Each
+// node carries a pending trigger key (+inf: nothing pending):
+//   at entry: 0 for every node if the tile was seeded or did not converge last time; else the
+//            tile's pending value, for nodes whose key reaches it and that lie within two of a
+//            face whose halo changed (or, after a band cut, that have a reached neighbour);
+//   later:   the smallest key (before or after the change) of a stencil node changed in the
+//            previous sweep, if it does not exceed the node's own key.
+// A change whose old and new keys both exceed a node's key leaves that node's heap replay as it
+// is (the replay stops before reaching such a neighbour), which is why the key filter is exact.
+// A sweep updates the nodes whose trigger lies within the instance's band (lowest pending tile
+// value + band width), compacted into a work list so the waves stay full; nodes beyond it stay
+// pending and the tile is queued again with their smallest trigger.  Faces whose 2-deep layer
+// changed activate the neighbour tile with that face's bit and the smallest changed key as its
+// pending value.  The list length is read on the device; blocks stride over it.
 template <int ND, typename T>
-__global__ void __launch_bounds__(kBlock) k_fim_round(const int64_t* __restrict__ list,
-                                                      const InstDev* __restrict__ inst, int ninst,
-                                                      T* __restrict__ ttpool, T* __restrict__ kpool,
-                                                      T* __restrict__ spool, uint8_t* __restrict__ dpool,
-                                                      const T* __restrict__ slowpool, uint8_t* __restrict__ flags,
-                                                      unsigned* __restrict__ visits) {
+__global__ void __launch_bounds__(kTileThreads) k_fim_round(
+    const int64_t* __restrict__ list, const uint2* __restrict__ info, unsigned* __restrict__ count,
+    const InstDev* __restrict__ inst, int ninst, const unsigned* __restrict__ wmin, T* __restrict__ ttpool,
+    T* __restrict__ kpool, T* __restrict__ spool, uint8_t* __restrict__ dpool, const T* __restrict__ slowpool,
+    unsigned* __restrict__ flags, unsigned* __restrict__ pend, unsigned long long* __restrict__ counters) {
     using S = TileShape<ND>;
     constexpr int NL = S::LX * S::LY * S::LZ;
-    __shared__ T s_t[NL];   // value, sign bit = init node
-    __shared__ T s_k[NL];   // pop key K
-    __shared__ T s_s[NL];   // pop key S
+    constexpr int NB = kTileThreads;
+    constexpr int NN = S::NODES;
+    constexpr int NPT = NN / NB;
+    constexpr int NW = NB / 64;
+    __shared__ T s_t[NL];        // value, sign bit = init node
+    __shared__ T s_k[NL];        // pop key K (the value itself outside cascades)
+    __shared__ T s_s[NL];        // pop key S (cascade nodes)
     __shared__ uint8_t s_d[NL];  // pop key D
-    __shared__ int s_faces;
+    __shared__ float s_chg[2][NL];  // smaller of old/new key of a node changed in a sweep, +inf if not
+    __shared__ T s_sl[NN];
+    __shared__ uint16_t s_work[NN];
+    __shared__ unsigned s_wc[NW];
+    __shared__ unsigned s_face[8];  // per face (and whole tile) smallest changed key; [7] deferred
+    __shared__ unsigned s_nupd, s_nchg;  // s_nchg: value changes << 16 | key-only changes
     const int tid = threadIdx.x;
-    const int64_t gid = list[blockIdx.x];
-    const int ii = find_inst(inst, ninst, gid);
-    const InstDev I = inst[ii];
-    const int64_t tl = gid - I.tile_base;
-    const int tix = (int)(tl % I.t[0]);
-    const int tiy = (int)((tl / I.t[0]) % I.t[1]);
-    const int tiz = (ND == 3) ? (int)(tl / ((int64_t)I.t[0] * I.t[1])) : 0;
-    const int ox = tix * S::TX, oy = tiy * S::TY, oz = tiz * S::TZ;
-    T* __restrict__ tt = ttpool + I.tt_off;
-    T* __restrict__ kk = kpool + I.tt_off;
-    T* __restrict__ ss = spool + I.tt_off;
-    uint8_t* __restrict__ dd = dpool + I.tt_off;
-    const T* __restrict__ slow = slowpool + I.s_off;
-    const int64_t nx = I.n[0], nxy = (int64_t)I.n[0] * I.n[1];
+    const int lane = tid & 63, wave = tid >> 6;
+    __shared__ unsigned s_bi;
+    const unsigned nlist = count[kCntList];
+    unsigned long long tot_upd = 0ull, tot_chg = 0ull, tot_act = 0ull, tot_kchg = 0ull;
+    if (blockIdx.x == 0 && tid == 0 && nlist) atomicAdd(&count[kCntRounds], 1u);
+    for (;;) {
+        if (tid == 0) s_bi = atomicAdd(&count[kCntCursor], 1u);
+        __syncthreads();
+        const unsigned bi = s_bi;
+        if (bi >= nlist) break;
+        const int64_t gid = list[bi];
+        const uint2 inf0 = info[bi];
+        const float pend0 = __uint_as_float(inf0.x);
+        const unsigned fbits = inf0.y;
+        const int ii = find_inst(inst, ninst, gid);
+        const InstDev I = inst[ii];
+        const float lim = __uint_as_float(wmin[ii]) + (float)I.nband;
+        const int64_t tl = gid - I.tile_base;
+        const int tix = (int)(tl % I.t[0]);
+        const int tiy = (int)((tl / I.t[0]) % I.t[1]);
+        const int tiz = (ND == 3) ? (int)(tl / ((int64_t)I.t[0] * I.t[1])) : 0;
+        const int ox = tix * S::TX, oy = tiy * S::TY, oz = tiz * S::TZ;
+        T* __restrict__ tt = ttpool + I.tt_off;
+        T* __restrict__ kk = kpool + I.tt_off;
+        T* __restrict__ ss = spool + I.tt_off;
+        uint8_t* __restrict__ dd = dpool + I.tt_off;
+        const T* __restrict__ slow = slowpool + I.s_off;
+        const int64_t nx = I.n[0], nxy = (int64_t)I.n[0] * I.n[1];
 
-    if (tid == 0) s_faces = 0;
-    // stage tile + face halos (edges/corners of the halo are never read by the axis stencil)
-    for (int c = tid; c < NL; c += kBlock) {
-        const int lx = c % S::LX, ly = (c / S::LX) % S::LY, lz = (ND == 3) ? c / (S::LX * S::LY) : 0;
-        int nout = (lx < S::H || lx >= S::H + S::TX) + (ly < S::H || ly >= S::H + S::TY);
-        if (ND == 3) nout += (lz < S::H || lz >= S::H + S::TZ);
-        T v = tinf<T>(), kv = tinf<T>(), sv = -tinf<T>();
-        uint8_t dv = 0;
-        if (nout <= 1) {
-            const int gx = ox + lx - S::H, gy = oy + ly - S::H, gz = (ND == 3) ? oz + lz - S::H : 0;
-            if (gx >= 0 && gx < I.n[0] && gy >= 0 && gy < I.n[1] && (ND == 2 || (gz >= 0 && gz < I.n[2]))) {
-                const int64_t l = gx + nx * gy + nxy * gz;
-                v = tt[l];
-                dv = dd[l];
-                if (dv) {
-                    kv = kk[l];
-                    sv = ss[l];
-                } else {
-                    kv = fabs(v);
-                }
-            }
+        if (tid < 8) s_face[tid] = 0x7f800000u;
+        if (tid == 0) {
+            s_nupd = 0u;
+            s_nchg = 0u;
         }
-        s_t[c] = v;
-        s_k[c] = kv;
-        s_s[c] = sv;
-        s_d[c] = dv;
-    }
-
-    // per-node static data
-    constexpr int NP = S::NODES_PER_THREAD;
-    int lidx[NP];
-    bool act[NP];
-    unsigned ok1[NP], ok2[NP];
-    T sl[NP], dh[NP][3], v0[NP];
-    Key<T> key0[NP];
-    int gxyz[NP][3];
-#pragma unroll
-    for (int p = 0; p < NP; p++) {
-        const int lx = tid % S::TX;
-        const int ly = (tid / S::TX) % S::TY;
-        const int lz = (ND == 3) ? (tid / (S::TX * S::TY)) + p * (S::TZ / NP) : 0;
-        const int gx = ox + lx, gy = oy + ly, gz = oz + lz;
-        gxyz[p][0] = gx;
-        gxyz[p][1] = gy;
-        gxyz[p][2] = gz;
-        lidx[p] = (lx + S::H) + S::LX * ((ly + S::H) + S::LY * (ND == 3 ? lz + S::H : 0));
-        const bool inside = gx < I.n[0] && gy < I.n[1] && (ND == 2 || gz < I.n[2]);
-        act[p] = inside;
-        ok1[p] = ok2[p] = 0u;
-        sl[p] = T(0);
-        if (inside) {
-            const int g[3] = {gx, gy, gz};
-#pragma unroll
-            for (int ax = 0; ax < ND; ax++) {
-#pragma unroll
-                for (int l = 0; l < 2; l++) {
-                    const int k = 2 * ax + l;
-                    int nb[3] = {g[0], g[1], g[2]};
-                    nb[ax] += (l == 0) ? 1 : -1;
-                    const bool in1 = nb[ax] >= 0 && nb[ax] <= I.n[ax] - 1;
-                    bool in2 = in1;
-#pragma unroll
-                    for (int d = 0; d < ND; d++) in2 = in2 && nb[d] >= 1 && nb[d] <= I.n[d] - 2;
-                    ok1[p] |= (unsigned)in1 << k;
-                    ok2[p] |= (unsigned)in2 << k;
-                }
-            }
-            sl[p] = slow[gx + nx * gy + nxy * gz];
-            node_spacing<T>(I, g, dh[p]);
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int p = 0; p < NP; p++) {
-        v0[p] = s_t[lidx[p]];
-        key0[p] = Key<T>{s_k[lidx[p]], s_s[lidx[p]], (int)s_d[lidx[p]]};
-        if (act[p] && is_fixed(v0[p])) act[p] = false;
-    }
-
-    constexpr int stx = 1, sty = S::LX, stz = S::LX * S::LY;
-    const int strides[3] = {stx, sty, stz};
-    int it = 0;
-    unsigned nvis = 0;
-    for (; it < S::MAXIT; it++) {
-        int any = 0;
-#pragma unroll
-        for (int p = 0; p < NP; p++) {
-            if (!act[p]) continue;
-            T v1[2 * ND], v2[2 * ND];
-            Key<T> k1[2 * ND], k2[2 * ND];
-            unsigned f1 = 0u, f2 = 0u;
-            const int li = lidx[p];
-#pragma unroll
-            for (int ax = 0; ax < ND; ax++) {
-#pragma unroll
-                for (int l = 0; l < 2; l++) {
-                    const int k = 2 * ax + l;
-                    const int o1 = li + ((l == 0) ? 1 : -1) * strides[ax];
-                    const int o2 = li + ((l == 0) ? 2 : -2) * strides[ax];
-                    T a = tinf<T>(), b = tinf<T>();
-                    Key<T> ka{tinf<T>(), -tinf<T>(), 0}, kb{tinf<T>(), -tinf<T>(), 0};
-                    if ((ok1[p] >> k) & 1u) {
-                        a = s_t[o1];
-                        ka = Key<T>{s_k[o1], s_s[o1], (int)s_d[o1]};
+        for (int c = tid; c < NL; c += NB) {
+            const int lx = c % S::LX, ly = (c / S::LX) % S::LY, lz = (ND == 3) ? c / (S::LX * S::LY) : 0;
+            int nout = (lx < S::H || lx >= S::H + S::TX) + (ly < S::H || ly >= S::H + S::TY);
+            if (ND == 3) nout += (lz < S::H || lz >= S::H + S::TZ);
+            T v = tinf<T>(), kv = tinf<T>(), sv = -tinf<T>();
+            uint8_t dv = 0;
+            if (nout <= 1) {
+                const int gx = ox + lx - S::H, gy = oy + ly - S::H, gz = (ND == 3) ? oz + lz - S::H : 0;
+                if (gx >= 0 && gx < I.n[0] && gy >= 0 && gy < I.n[1] && (ND == 2 || (gz >= 0 && gz < I.n[2]))) {
+                    const int64_t l = gx + nx * gy + nxy * gz;
+                    v = tt[l];
+                    dv = dd[l];
+                    if (dv) {
+                        kv = kk[l];
+                        sv = ss[l];
+                    } else {
+                        kv = fabs(v);
                     }
-                    if ((ok2[p] >> k) & 1u) {
-                        b = s_t[o2];
-                        kb = Key<T>{s_k[o2], s_s[o2], (int)s_d[o2]};
-                    }
-                    f1 |= (unsigned)is_fixed(a) << k;
-                    f2 |= (unsigned)is_fixed(b) << k;
-                    v1[k] = fabs(a);
-                    v2[k] = fabs(b);
-                    k1[k] = ka;
-                    k2[k] = kb;
                 }
             }
-            int code[3];
-            Key<T> nk;
-            const T nv = replay<ND, T>(v1, v2, k1, k2, f1, f2, ok1[p], ok2[p], sl[p], dh[p], code, &nk);
-            const T cur = s_t[li];
-            const bool vchg = nv == nv && nv != cur && !(nv >= tinf<T>() && cur >= tinf<T>());
-            const bool kchg = nv < tinf<T>() && (nk.K != s_k[li] || nk.S != s_s[li] || nk.D != (int)s_d[li]);
-            if (vchg || kchg) {
-                s_t[li] = nv;
-                s_k[li] = nk.K;
-                s_s[li] = nk.S;
-                s_d[li] = (uint8_t)nk.D;
-                any = 1;
-            }
+            s_t[c] = v;
+            s_k[c] = kv;
+            s_s[c] = sv;
+            s_d[c] = dv;
+            s_chg[0][c] = EIK_INFF;
+            s_chg[1][c] = EIK_INFF;
         }
-        nvis++;
-        if (!__syncthreads_or(any)) break;
-    }
-    const bool converged = it < S::MAXIT;
+        auto node_pos = [&](int n, int& lx, int& ly, int& lz) {
+            lx = n % S::TX;
+            ly = (n / S::TX) % S::TY;
+            lz = (ND == 3) ? n / (S::TX * S::TY) : 0;
+        };
+        auto lds_index = [&](int lx, int ly, int lz) {
+            return (lx + S::H) + S::LX * ((ly + S::H) + S::LY * (ND == 3 ? lz + S::H : 0));
+        };
+        // this thread's nodes: LDS index, whether it is solved for, whether it is near a changed face
+        int li[NPT];
+        bool act[NPT];
+        bool near[NPT];
+        float ptr[NPT];  // pending trigger key
+    #pragma unroll
+        for (int j = 0; j < NPT; j++) {
+            const int n = tid + j * NB;
+            int lx, ly, lz;
+            node_pos(n, lx, ly, lz);
+            li[j] = lds_index(lx, ly, lz);
+            const int g[3] = {ox + lx, oy + ly, oz + lz};
+            const bool inside = g[0] < I.n[0] && g[1] < I.n[1] && (ND == 2 || g[2] < I.n[2]);
+            s_sl[n] = inside ? slow[g[0] + nx * g[1] + nxy * g[2]] : T(0);
+            unsigned m = (unsigned)(lx < 2) | (unsigned)(lx >= S::TX - 2) << 1 | (unsigned)(ly < 2) << 2 |
+                         (unsigned)(ly >= S::TY - 2) << 3;
+            if (ND == 3) m |= (unsigned)(lz < 2) << 4 | (unsigned)(lz >= S::TZ - 2) << 5;
+            near[j] = (m & fbits) != 0u;
+            act[j] = inside;
+        }
+        __syncthreads();
+        constexpr int strides[3] = {1, S::LX, S::LX * S::LY};
+    #pragma unroll
+        for (int j = 0; j < NPT; j++) {
+            act[j] = act[j] && !is_fixed(s_t[li[j]]);
+            ptr[j] = EIK_INFF;
+            if (!act[j]) continue;
+            if (fbits & kFullBit) {
+                ptr[j] = 0.f;
+                continue;
+            }
+            bool reached = false;
+    #pragma unroll
+            for (int ax = 0; ax < ND; ax++)
+                reached = reached | (fabs(s_t[li[j] + strides[ax]]) < tinf<T>()) | (fabs(s_t[li[j] - strides[ax]]) < tinf<T>());
+            const bool cand = near[j] | ((fbits & kKeyBit) != 0u && reached);
+            if (cand && (T)pend0 <= s_k[li[j]]) ptr[j] = pend0;
+        }
 
-    // write back changed nodes and collect the faces whose 2-deep layer changed
-    int faces = 0;
-#pragma unroll
-    for (int p = 0; p < NP; p++) {
-        if (!act[p]) continue;
-        const int li = lidx[p];
-        const T v = s_t[li];
-        const Key<T> kn{s_k[li], s_s[li], (int)s_d[li]};
-        if (v != v0[p] || !keq(kn, key0[p])) {
-            const int gx = gxyz[p][0], gy = gxyz[p][1], gz = gxyz[p][2];
-            const int64_t l = gx + nx * gy + nxy * gz;
+        // Cartesian grids share one metric; spherical ones compute it per node
+        const bool cart = I.kind == EIK_GRID2D_CART || I.kind == EIK_GRID3D_CART;
+        Metric<T> mcart;
+    #pragma unroll
+        for (int ax = 0; ax < 3; ax++) mcart.dh[ax] = (T)I.h[0];
+        metric_coefs(mcart, ND);
+        unsigned nupd = 0u, nchg = 0u;
+        int it = 0;
+        for (; it < S::MAXIT; it++) {
+            const int cb = it & 1, pb = cb ^ 1;
+            // 1. collect this thread's triggers; the nodes within the band need an update
+            bool need[NPT];
+    #pragma unroll
+            for (int j = 0; j < NPT; j++) {
+                if (act[j] && it > 0) {
+                    const T kx = s_k[li[j]];
+                    float tr = ptr[j];
+    #pragma unroll
+                    for (int ax = 0; ax < ND; ax++) {
+                        const int st = strides[ax];
+                        const float c[4] = {s_chg[pb][li[j] + st], s_chg[pb][li[j] - st], s_chg[pb][li[j] + 2 * st],
+                                            s_chg[pb][li[j] - 2 * st]};
+    #pragma unroll
+                        for (int q = 0; q < 4; q++) tr = ((T)c[q] <= kx) ? fminf(tr, c[q]) : tr;
+                    }
+                    ptr[j] = tr;
+                }
+                need[j] = act[j] & (ptr[j] < EIK_INFF) & (ptr[j] <= lim);
+                ptr[j] = need[j] ? EIK_INFF : ptr[j];
+            }
+            // 2. compact the work list (wave ballots, then a prefix over the waves)
+            unsigned long long msk[NPT];
+            unsigned wtot = 0u;
+    #pragma unroll
+            for (int j = 0; j < NPT; j++) {
+                msk[j] = __ballot(need[j]);
+                wtot += (unsigned)__popcll(msk[j]);
+            }
+            if (lane == 0) s_wc[wave] = wtot;
+            __syncthreads();
+    #pragma unroll
+            for (int j = 0; j < NPT; j++) s_chg[pb][li[j]] = EIK_INFF;
+            unsigned base = 0u, total = 0u;
+    #pragma unroll
+            for (int w = 0; w < NW; w++) {
+                const unsigned c = s_wc[w];
+                base += (w < wave) ? c : 0u;
+                total += c;
+            }
+            if (total == 0u) break;
+            const unsigned long long below = (1ull << lane) - 1ull;
+    #pragma unroll
+            for (int j = 0; j < NPT; j++) {
+                if (need[j]) s_work[base + (unsigned)__popcll(msk[j] & below)] = (uint16_t)(tid + j * NB);
+                base += (unsigned)__popcll(msk[j]);
+            }
+            __syncthreads();
+            // 3. update the listed nodes
+            for (unsigned w = tid; w < total; w += NB) {
+                const int n = s_work[w];
+                int lx, ly, lz;
+                node_pos(n, lx, ly, lz);
+                const int lc = lds_index(lx, ly, lz);
+                const int g[3] = {ox + lx, oy + ly, oz + lz};
+                unsigned ok1 = 0u, ok2 = 0u;
+    #pragma unroll
+                for (int ax = 0; ax < ND; ax++) {
+    #pragma unroll
+                    for (int l = 0; l < 2; l++) {
+                        const int k = 2 * ax + l;
+                        int nb[3] = {g[0], g[1], g[2]};
+                        nb[ax] += (l == 0) ? 1 : -1;
+                        const bool in1 = nb[ax] >= 0 && nb[ax] <= I.n[ax] - 1;
+                        bool in2 = in1;
+    #pragma unroll
+                        for (int d = 0; d < ND; d++) in2 = in2 && nb[d] >= 1 && nb[d] <= I.n[d] - 2;
+                        ok1 |= (unsigned)in1 << k;
+                        ok2 |= (unsigned)in2 << k;
+                    }
+                }
+                Metric<T> mt = mcart;
+                if (!cart) {
+                    node_spacing<T>(I, g, mt.dh);
+                    metric_coefs(mt, ND);
+                }
+                T v1[2 * ND], v2[2 * ND];
+                Key<T> k1[2 * ND], k2[2 * ND];
+                unsigned f1 = 0u, f2 = 0u;
+    #pragma unroll
+                for (int ax = 0; ax < ND; ax++) {
+    #pragma unroll
+                    for (int l = 0; l < 2; l++) {
+                        const int k = 2 * ax + l;
+                        const int o1 = lc + ((l == 0) ? 1 : -1) * strides[ax];
+                        const int o2 = lc + ((l == 0) ? 2 : -2) * strides[ax];
+                        T a = tinf<T>(), b = tinf<T>();
+                        Key<T> ka{tinf<T>(), -tinf<T>(), 0}, kb{tinf<T>(), -tinf<T>(), 0};
+                        if ((ok1 >> k) & 1u) {
+                            a = s_t[o1];
+                            ka = Key<T>{s_k[o1], s_s[o1], (int)s_d[o1]};
+                        }
+                        if ((ok2 >> k) & 1u) {
+                            b = s_t[o2];
+                            kb = Key<T>{s_k[o2], s_s[o2], (int)s_d[o2]};
+                        }
+                        f1 |= (unsigned)is_fixed(a) << k;
+                        f2 |= (unsigned)is_fixed(b) << k;
+                        v1[k] = fabs(a);
+                        v2[k] = fabs(b);
+                        k1[k] = ka;
+                        k2[k] = kb;
+                    }
+                }
+                int code[3];
+                Key<T> nk;
+                const T nv = replay<ND, T>(v1, v2, k1, k2, f1, f2, ok1, ok2, s_sl[n], mt, code, &nk);
+                nupd++;
+                const T cur = s_t[lc];
+                const T kold = s_k[lc];
+                const bool vchg = nv == nv && nv != cur && !(nv >= tinf<T>() && cur >= tinf<T>());
+                const bool kchg = nv < tinf<T>() && (nk.K != kold || nk.S != s_s[lc] || nk.D != (int)s_d[lc]);
+                if (vchg || kchg) {
+                    nchg += vchg ? 0x10000u : 1u;
+                    s_t[lc] = nv;
+                    s_k[lc] = nk.K;
+                    s_s[lc] = nk.S;
+                    s_d[lc] = (uint8_t)nk.D;
+                    s_chg[cb][lc] = key_down(kold < nk.K ? kold : nk.K);
+                }
+            }
+            __syncthreads();
+        }
+        const bool converged = it < S::MAXIT;
+
+        // write back changed nodes; smallest changed key (old or new) per face; deferred triggers
+        if (nupd) {
+            atomicAdd(&s_nupd, nupd);
+            atomicAdd(&s_nchg, nchg);
+        }
+    #pragma unroll
+        for (int j = 0; j < NPT; j++)
+            if (ptr[j] < EIK_INFF) atomicMin(&s_face[7], __float_as_uint(ptr[j]));
+    #pragma unroll
+        for (int j = 0; j < NPT; j++) {
+            if (!act[j]) continue;
+            const int n = tid + j * NB;
+            int lx, ly, lz;
+            node_pos(n, lx, ly, lz);
+            const int64_t l = (ox + lx) + nx * (oy + ly) + nxy * (oz + lz);
+            const T v = s_t[li[j]];
+            const Key<T> kn{s_k[li[j]], s_s[li[j]], (int)s_d[li[j]]};
+            const T v0 = tt[l];
+            const uint8_t d0 = dd[l];
+            const Key<T> ko = d0 ? Key<T>{kk[l], ss[l], (int)d0} : Key<T>{fabs(v0), -tinf<T>(), 0};
+            if (v == v0 && keq(kn, ko)) continue;
             tt[l] = v;
             dd[l] = (uint8_t)kn.D;
             if (kn.D) {
                 kk[l] = kn.K;
                 ss[l] = kn.S;
             }
-            const int lx = gx - ox, ly = gy - oy, lz = gz - oz;
-            if (lx < 2) faces |= 1;
-            if (lx >= S::TX - 2) faces |= 2;
-            if (ly < 2) faces |= 4;
-            if (ly >= S::TY - 2) faces |= 8;
+            const unsigned pv = __float_as_uint(key_down(ko.K < kn.K ? ko.K : kn.K));
+            if (lx < 2) atomicMin(&s_face[0], pv);
+            if (lx >= S::TX - 2) atomicMin(&s_face[1], pv);
+            if (ly < 2) atomicMin(&s_face[2], pv);
+            if (ly >= S::TY - 2) atomicMin(&s_face[3], pv);
             if (ND == 3) {
-                if (lz < 2) faces |= 16;
-                if (lz >= S::TZ - 2) faces |= 32;
+                if (lz < 2) atomicMin(&s_face[4], pv);
+                if (lz >= S::TZ - 2) atomicMin(&s_face[5], pv);
             }
+            atomicMin(&s_face[6], pv);
         }
-    }
-    if (faces) atomicOr(&s_faces, faces);
-    __syncthreads();
-    if (tid == 0) {
-        const int f = s_faces;
-        uint8_t* fl = flags + I.tile_base;
-        const int64_t t0 = I.t[0], t01 = (int64_t)I.t[0] * I.t[1];
-        if ((f & 1) && tix > 0) fl[tl - 1] = 1;
-        if ((f & 2) && tix < I.t[0] - 1) fl[tl + 1] = 1;
-        if ((f & 4) && tiy > 0) fl[tl - t0] = 1;
-        if ((f & 8) && tiy < I.t[1] - 1) fl[tl + t0] = 1;
-        if (ND == 3) {
-            if ((f & 16) && tiz > 0) fl[tl - t01] = 1;
-            if ((f & 32) && tiz < I.t[2] - 1) fl[tl + t01] = 1;
+        __syncthreads();
+        if (tid == 0) {
+            unsigned* fl = flags + I.tile_base;
+            unsigned* pd = pend + I.tile_base;
+            const int64_t t0 = I.t[0], t01 = (int64_t)I.t[0] * I.t[1];
+            auto act_tile = [&](int64_t t, unsigned bit, unsigned v) {
+                atomicOr(&fl[t], bit);
+                atomicMin(&pd[t], v);
+            };
+            // the x- neighbour sees this tile across its x+ face, and so on
+            if (s_face[0] != 0x7f800000u && tix > 0) act_tile(tl - 1, 1u << 1, s_face[0]);
+            if (s_face[1] != 0x7f800000u && tix < I.t[0] - 1) act_tile(tl + 1, 1u << 0, s_face[1]);
+            if (s_face[2] != 0x7f800000u && tiy > 0) act_tile(tl - t0, 1u << 3, s_face[2]);
+            if (s_face[3] != 0x7f800000u && tiy < I.t[1] - 1) act_tile(tl + t0, 1u << 2, s_face[3]);
+            if (ND == 3) {
+                if (s_face[4] != 0x7f800000u && tiz > 0) act_tile(tl - t01, 1u << 5, s_face[4]);
+                if (s_face[5] != 0x7f800000u && tiz < I.t[2] - 1) act_tile(tl + t01, 1u << 4, s_face[5]);
+            }
+            if (!converged)
+                act_tile(tl, kFullBit, s_face[6] == 0x7f800000u ? 0u : s_face[6]);
+            else if (s_face[7] != 0x7f800000u)
+                act_tile(tl, kKeyBit, s_face[7]);
+            atomicAdd(&count[kCntSweeps], (unsigned)it);
+            tot_upd += s_nupd;
+            tot_chg += (s_nchg >> 16) + (s_nchg & 0xffffu);
+            tot_kchg += s_nchg & 0xffffu;
+            tot_act++;
         }
-        if (!converged) fl[tl] = 1;
-        if (visits) atomicAdd(visits, nvis);
+        __syncthreads();  // LDS and s_bi are reused by the next tile
     }
+    if (tid == 0 && tot_act) {
+        atomicAdd(&counters[0], tot_upd);
+        atomicAdd(&counters[1], tot_chg);
+        atomicAdd(&counters[2], tot_act);
+        atomicAdd(&counters[3], tot_kchg);
+    }
+}
+
+// Band scheduling over the active tiles.  Pass 1: lowest pending value per instance.
+__global__ void k_band_min(const unsigned* __restrict__ flags, const unsigned* __restrict__ pend, int64_t ntiles,
+                           const InstDev* __restrict__ inst, int ninst, unsigned* __restrict__ wmin,
+                           unsigned* __restrict__ nflagged) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool on = t < ntiles && flags[t] != 0u;
+    const unsigned long long m = __ballot(on);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(nflagged, (unsigned)__popcll(m));
+    if (!on) return;
+    atomicMin(&wmin[find_inst(inst, ninst, t)], pend[t]);
+}
+
+// Pass 2: list the active tiles whose pending value is within the instance's band of its lowest,
+// with their pending value and face bits, and clear them; the others wait for a later round.
+__global__ void k_band_select(unsigned* __restrict__ flags, unsigned* __restrict__ pend, int64_t ntiles,
+                              const InstDev* __restrict__ inst, int ninst, const unsigned* __restrict__ wmin,
+                              int64_t* __restrict__ list, uint2* __restrict__ info, unsigned* __restrict__ count) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntiles || !flags[t]) return;
+    const int i = find_inst(inst, ninst, t);
+    const float lim = __uint_as_float(wmin[i]) + (float)inst[i].band;
+    if (__uint_as_float(pend[t]) <= lim) {
+        const unsigned pos = atomicAdd(count, 1u);
+        list[pos] = t;
+        info[pos] = make_uint2(pend[t], flags[t]);
+        flags[t] = 0u;
+        pend[t] = 0x7f800000u;
+    }
+}
+
+__global__ void k_fill_u32(unsigned* __restrict__ p, int64_t n, unsigned v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
 // gather the set flags into a list (order irrelevant) and clear them
@@ -296,7 +480,7 @@ __global__ void k_slowness(const double* __restrict__ vel, double* __restrict__ 
 template <typename T>
 __global__ void k_corners(const CornerDev* __restrict__ cs, int ncs, const InstDev* __restrict__ inst,
                           T* __restrict__ ttpool, const double* __restrict__ velc, const FineDev* __restrict__ fine,
-                          uint8_t* __restrict__ flags, int nd, int kind) {
+                          unsigned* __restrict__ flags, unsigned* __restrict__ pend, int nd, int kind) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= ncs) return;
     const CornerDev C = cs[q];
@@ -316,7 +500,7 @@ __global__ void k_corners(const CornerDev* __restrict__ cs, int ncs, const InstD
     for (int d = 0; d < nd; d++) xyz[d] = fma((double)g[d], I.h[d], I.o[d]);
     const double dist = distance2points(kind, nd, xyz, C.src);
     ttpool[I.tt_off + l] = -(T)(dist / v);
-    seed_tiles(I, nd, g, flags);
+    seed_tiles_fwd(I, nd, g, flags, pend, dist / v);
 }
 
 // nearest-neighbour fine slowness of every refined box (createfinegrid, eikmainfwd.jl:1035-1056)
@@ -352,8 +536,9 @@ __global__ void k_fine_setup(const FineDev* __restrict__ fine, int nfine, const 
 template <typename T>
 __global__ void __launch_bounds__(256) k_transfer(const FineDev* __restrict__ fine, InstDev* __restrict__ finst,
                                                   TTPools fp, const InstDev* __restrict__ cinst, T* __restrict__ ttc,
-                                                  uint8_t* __restrict__ cflags, int* __restrict__ ntransfer,
-                                                  SrcPt* __restrict__ xlist, int cap, int nd) {
+                                                  unsigned* __restrict__ cflags, unsigned* __restrict__ cpend,
+                                                  int* __restrict__ ntransfer, SrcPt* __restrict__ xlist, int cap,
+                                                  int nd) {
     const int f = blockIdx.x;
     const FineDev F = fine[f];
     const InstDev I = finst[f];
@@ -434,7 +619,7 @@ __global__ void __launch_bounds__(256) k_transfer(const FineDev* __restrict__ fi
                 const int cg[3] = {F.i1c[0] + wx, F.i1c[1] + wy, F.i1c[2] + wz};
                 const int64_t lc = cg[0] + (int64_t)C.n[0] * (cg[1] + (int64_t)C.n[1] * cg[2]);
                 ttc[C.tt_off + lc] = -(T)t;
-                seed_tiles(C, nd, cg, cflags);
+                seed_tiles_fwd(C, nd, cg, cflags, cpend, t);
                 if (cnt < cap) xlist[(int64_t)f * cap + cnt] = SrcPt{F.cinst, (int32_t)lf, lc};
                 cnt++;
             }
@@ -464,44 +649,82 @@ __global__ void k_export(const T* __restrict__ src, double* __restrict__ dst, in
 
 static unsigned nblocks(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
+// Per-round reset: lowest pending value per instance, list length, tile cursor, flagged count.
+__global__ void k_round_reset(unsigned* __restrict__ wmin, int ninst, unsigned* __restrict__ cnt) {
+    for (int i = threadIdx.x; i < ninst; i += blockDim.x) wmin[i] = 0x7f800000u;
+    if (threadIdx.x == 0) {
+        cnt[kCntList] = 0u;
+        cnt[kCntCursor] = 0u;
+        cnt[kCntFlagged] = 0u;
+    }
+}
+
+// Rounds are enqueued in batches of kRoundBatch without host synchronisation: each round resets
+// the per-instance minimum, selects the tiles within the band, and runs the round kernel on a
+// fixed grid of resident blocks that take tiles from the device-side list.  After a batch the
+// host reads how many tiles are still flagged; none means converged.
 template <int ND, typename T>
 static int fim_run_t(eik_ctx* ctx, const InstDev* d_inst, int ninst, int64_t ntiles, const TTPools& P,
                      const T* slowpool, FimStats* fs) {
     DevBuf& list = ctx->buf("fim_list");
-    if (list.ensure((size_t)ntiles * sizeof(int64_t))) return ctx->fail(EIK_ENOMEM, "out of device memory (tile list)");
-    uint8_t* flags = (uint8_t*)ctx->buf("fim_flags").p;
+    DevBuf& linfo = ctx->buf("fim_list_info");
+    DevBuf& wm = ctx->buf("fim_wmin");
+    if (list.ensure((size_t)ntiles * sizeof(int64_t)) || linfo.ensure((size_t)ntiles * sizeof(uint2)) ||
+        wm.ensure((size_t)ninst * sizeof(unsigned)))
+        return ctx->fail(EIK_ENOMEM, "out of device memory (tile list)");
+    unsigned* flags = (unsigned*)ctx->buf("fim_tflags").p;
+    unsigned* pend = (unsigned*)ctx->buf("fim_pend").p;
     unsigned* d_cnt = (unsigned*)ctx->d_small;
     unsigned* h_cnt = (unsigned*)ctx->h_small;
-    FimStats st{};
-    CK(hipMemsetAsync(d_cnt + 1, 0, sizeof(unsigned), ctx->stream));
-    const int64_t maxrounds = 50000;
-    bool pending = false;  // a timed round launch whose events are not read yet
-    for (;;) {
-        CK(hipMemsetAsync(d_cnt, 0, sizeof(unsigned), ctx->stream));
-        k_compact<<<nblocks(ntiles, 256), 256, 0, ctx->stream>>>(flags, ntiles, (int64_t*)list.p, d_cnt);
-        CK(hipMemcpyAsync(h_cnt, d_cnt, sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
-        CK(hipStreamSynchronize(ctx->stream));
-        if (pending) {
-            float ms = 0.f;
-            CK(hipEventElapsedTime(&ms, ctx->kev[0], ctx->kev[1]));
-            st.kernel_ms += ms;
-            pending = false;
-        }
-        const unsigned nact = h_cnt[0];
-        if (nact == 0) break;
-        if (++st.rounds > maxrounds) return ctx->fail(EIK_ENOCONV, "traveltime solve did not converge");
-        st.activations += nact;
-        CK(hipEventRecord(ctx->kev[0], ctx->stream));
-        k_fim_round<ND, T><<<nact, kBlock, 0, ctx->stream>>>((const int64_t*)list.p, d_inst, ninst, (T*)P.tt, (T*)P.k,
-                                                               (T*)P.s, P.d, slowpool, flags, d_cnt + 1);
-        CK(hipGetLastError());
-        CK(hipEventRecord(ctx->kev[1], ctx->stream));
-        pending = true;
+    unsigned long long* d_c64 = (unsigned long long*)(d_cnt + kCnt64);
+    hipStream_t st = ctx->stream;
+    static int grid = 0;  // resident blocks of this instantiation on the device
+    if (grid == 0) {
+        int ncu = 0, occ = 0;
+        CK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device));
+        CK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_fim_round<ND, T>, kTileThreads, 0));
+        grid = std::max(1, ncu * std::max(1, occ));
     }
-    CK(hipMemcpyAsync(h_cnt + 1, d_cnt + 1, sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
-    CK(hipStreamSynchronize(ctx->stream));
-    st.sweeps = (double)h_cnt[1];
-    if (fs) *fs = st;
+    const int g = (int)std::min<int64_t>(grid, ntiles);
+    FimStats stt{};
+    CK(hipMemsetAsync(d_cnt, 0, kSmallBytes, st));
+    const int64_t maxbatches = 50000 / kRoundBatch;
+    for (int64_t batch = 0;; batch++) {
+        if (batch >= maxbatches) return ctx->fail(EIK_ENOCONV, "traveltime solve did not converge");
+        for (int r = 0; r < kRoundBatch; r++) {
+            k_round_reset<<<1, 256, 0, st>>>((unsigned*)wm.p, ninst, d_cnt);
+            k_band_min<<<nblocks(ntiles, 256), 256, 0, st>>>(flags, pend, ntiles, d_inst, ninst, (unsigned*)wm.p,
+                                                            d_cnt + kCntFlagged);
+            k_band_select<<<nblocks(ntiles, 256), 256, 0, st>>>(flags, pend, ntiles, d_inst, ninst,
+                                                               (const unsigned*)wm.p, (int64_t*)list.p,
+                                                               (uint2*)linfo.p, d_cnt + kCntList);
+            CK(hipEventRecord(ctx->kev[2 * r], st));
+            k_fim_round<ND, T><<<g, kTileThreads, 0, st>>>((const int64_t*)list.p, (const uint2*)linfo.p, d_cnt,
+                                                          d_inst, ninst, (const unsigned*)wm.p, (T*)P.tt, (T*)P.k,
+                                                          (T*)P.s, P.d, slowpool, flags, pend, d_c64);
+            CK(hipGetLastError());
+            CK(hipEventRecord(ctx->kev[2 * r + 1], st));
+        }
+        k_round_reset<<<1, 256, 0, st>>>((unsigned*)wm.p, ninst, d_cnt);
+        k_band_min<<<nblocks(ntiles, 256), 256, 0, st>>>(flags, pend, ntiles, d_inst, ninst, (unsigned*)wm.p,
+                                                        d_cnt + kCntFlagged);
+        CK(hipMemcpyAsync(h_cnt, d_cnt, kSmallBytes, hipMemcpyDeviceToHost, st));
+        CK(hipStreamSynchronize(st));
+        for (int r = 0; r < kRoundBatch; r++) {
+            float ms = 0.f;
+            CK(hipEventElapsedTime(&ms, ctx->kev[2 * r], ctx->kev[2 * r + 1]));
+            stt.kernel_ms += ms;
+        }
+        if (h_cnt[kCntFlagged] == 0u) break;
+    }
+    const unsigned long long* h_c64 = (const unsigned long long*)(h_cnt + kCnt64);
+    stt.rounds = (int64_t)h_cnt[kCntRounds];
+    stt.sweeps = (double)h_cnt[kCntSweeps];
+    stt.updates = (double)h_c64[0];
+    stt.changed = (double)h_c64[1];
+    stt.activations = (double)h_c64[2];
+    stt.keychanged = (double)h_c64[3];
+    if (fs) *fs = stt;
     return EIK_OK;
 }
 
@@ -515,6 +738,36 @@ int fim_run(eik_ctx* ctx, int nd, int prec, const InstDev* d_inst, int ninst, in
 }
 
 // ----------------------------------------------------------------- forward pass of one call
+// Band width of the tile scheduler, in tile crossing times (tile edge x spacing x mean slowness).
+// A round runs only the active tiles whose pending traveltime lies within the band of the lowest
+// pending one, so tiles far ahead of the front are not relaxed against values that will still
+// change.  EIK_BAND overrides the multiplier (0 or less: no band, every active tile runs).
+static double band_mult() {
+    static const double m = [] {
+        const char* e = getenv("EIK_BAND");
+        return e ? atof(e) : 0.5;
+    }();
+    return m;
+}
+
+// EIK_NODEBAND: the same for the nodes inside a tile (default 0: a listed tile relaxes all of
+// its nodes; measured faster on the 201x201x101 workload, where cut tiles were queued again too
+// often).
+static double nodeband_mult() {
+    static const double m = [] {
+        const char* e = getenv("EIK_NODEBAND");
+        return e ? atof(e) : 0.0;
+    }();
+    return m;
+}
+
+static void set_band(InstDev& I, int nd, double smean) {
+    const double m = band_mult(), mn = nodeband_mult();
+    const int edge = nd == 2 ? 16 : 8;
+    I.band = m > 0.0 ? m * edge * I.h[0] * smean : INFINITY;
+    I.nband = mn > 0.0 ? mn * edge * I.h[0] * smean : INFINITY;
+}
+
 int forward_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, int prec, const eik_params* P) {
     const int nd = M->g.ndim;
     const int kind = M->g.kind;
@@ -536,6 +789,7 @@ int forward_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, int prec, cons
         I.tt_off = s * N;
         I.s_off = 0;
         I.tile_base = s * tiles_per;
+        set_band(I, nd, M->smean);
     }
     const int64_t ntiles_c = nsrc * tiles_per;
     const size_t esz = prec == EIK_PREC_FP64 ? sizeof(double) : sizeof(float);
@@ -565,14 +819,21 @@ int forward_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, int prec, cons
             finst[s].tt_off = nfine_pts;
             finst[s].s_off = nfine_pts;
             finst[s].tile_base = ntiles_f;
+            set_band(finst[s], nd, M->smean);
             nfine_pts += (int64_t)finst[s].n[0] * finst[s].n[1] * finst[s].n[2];
             ntiles_f += (int64_t)finst[s].t[0] * finst[s].t[1] * finst[s].t[2];
         }
         maxt = std::max(maxt, ntiles_f);
     }
-    DevBuf& flags = ctx->buf("fim_flags");
-    if (flags.ensure((size_t)maxt)) return ctx->fail(EIK_ENOMEM, "out of device memory (tile flags)");
-    CK(hipMemsetAsync(flags.p, 0, (size_t)maxt, st));
+    DevBuf& aflags = ctx->buf("fim_flags");  // adjoint rounds
+    if (aflags.ensure((size_t)maxt)) return ctx->fail(EIK_ENOMEM, "out of device memory (tile flags)");
+    CK(hipMemsetAsync(aflags.p, 0, (size_t)maxt, st));
+    DevBuf& flags = ctx->buf("fim_tflags");
+    if (flags.ensure((size_t)maxt * sizeof(unsigned))) return ctx->fail(EIK_ENOMEM, "out of device memory (tile flags)");
+    CK(hipMemsetAsync(flags.p, 0, (size_t)maxt * sizeof(unsigned), st));
+    DevBuf& pend = ctx->buf("fim_pend");
+    if (pend.ensure((size_t)maxt * sizeof(unsigned))) return ctx->fail(EIK_ENOMEM, "out of device memory (tile flags)");
+    k_fill_u32<<<nblocks(maxt, 256), 256, 0, st>>>((unsigned*)pend.p, maxt, 0x7f800000u);
 
     // coarse grids start at +inf
     if (prec == EIK_PREC_FP64)
@@ -615,7 +876,7 @@ int forward_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, int prec, cons
         CK(hipMemcpyAsync(dcs.p, corners.data(), corners.size() * sizeof(CornerDev), hipMemcpyHostToDevice, st));
         k_corners<double><<<nblocks((int64_t)corners.size(), 64), 64, 0, st>>>(
             (const CornerDev*)dcs.p, (int)corners.size(), (const InstDev*)dfi.p, (double*)ttf.p, M->d_vel,
-            (const FineDev*)dfd.p, (uint8_t*)flags.p, nd, kind);
+            (const FineDev*)dfd.p, (unsigned*)flags.p, (unsigned*)pend.p, nd, kind);
         CK(hipGetLastError());
         tm.mark();
         FimStats fst{};
@@ -635,12 +896,12 @@ int forward_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, int prec, cons
             return ctx->fail(EIK_ENOMEM, "out of device memory");
         if (prec == EIK_PREC_FP64)
             k_transfer<double><<<(unsigned)nsrc, 256, 0, st>>>((const FineDev*)dfd.p, (InstDev*)dfi.p, fpools,
-                                                              (const InstDev*)dci.p, (double*)ttc.p, (uint8_t*)flags.p,
-                                                              (int*)ntr.p, (SrcPt*)xl.p, cap, nd);
+                                                              (const InstDev*)dci.p, (double*)ttc.p, (unsigned*)flags.p,
+                                                              (unsigned*)pend.p, (int*)ntr.p, (SrcPt*)xl.p, cap, nd);
         else
             k_transfer<float><<<(unsigned)nsrc, 256, 0, st>>>((const FineDev*)dfd.p, (InstDev*)dfi.p, fpools,
-                                                             (const InstDev*)dci.p, (float*)ttc.p, (uint8_t*)flags.p,
-                                                             (int*)ntr.p, (SrcPt*)xl.p, cap, nd);
+                                                             (const InstDev*)dci.p, (float*)ttc.p, (unsigned*)flags.p,
+                                                             (unsigned*)pend.p, (int*)ntr.p, (SrcPt*)xl.p, cap, nd);
         CK(hipGetLastError());
     } else {
         for (int64_t s = 0; s < nsrc; s++) {
@@ -660,11 +921,11 @@ int forward_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, int prec, cons
         if (prec == EIK_PREC_FP64)
             k_corners<double><<<nblocks((int64_t)corners.size(), 64), 64, 0, st>>>(
                 (const CornerDev*)dcs.p, (int)corners.size(), (const InstDev*)dci.p, (double*)ttc.p, M->d_vel, nullptr,
-                (uint8_t*)flags.p, nd, kind);
+                (unsigned*)flags.p, (unsigned*)pend.p, nd, kind);
         else
             k_corners<float><<<nblocks((int64_t)corners.size(), 64), 64, 0, st>>>(
                 (const CornerDev*)dcs.p, (int)corners.size(), (const InstDev*)dci.p, (float*)ttc.p, M->d_vel, nullptr,
-                (uint8_t*)flags.p, nd, kind);
+                (unsigned*)flags.p, (unsigned*)pend.p, nd, kind);
         CK(hipGetLastError());
         tm.mark();
     }
@@ -677,6 +938,9 @@ int forward_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, int prec, cons
     ctx->stats[5] = (double)cst.rounds;
     ctx->stats[6] = cst.kernel_ms;
     ctx->stats[7] = cst.sweeps;
+    ctx->stats[8] = cst.updates;
+    ctx->stats[9] = cst.changed;
+    ctx->stats[10] = cst.keychanged;
     return EIK_OK;
 }
 

@@ -37,6 +37,8 @@ struct InstDev {
     double tstopS;
     int32_t tstopD;
     int32_t pad2;
+    double band;         // band scheduler: tiles within this traveltime of the lowest pending one run
+    double nband;        // the same for the nodes updated inside a tile
 };
 
 // Refined box around one source (createfinegrid, src/EikSolv/eikmainfwd.jl:950-1104)
@@ -107,7 +109,8 @@ __host__ __device__ __forceinline__ double distance2points(int kind, int nd, con
     return sqrt(s);
 }
 
-// mark the tile of node g and the face-neighbour tiles whose stencil reaches it
+// mark the tile of node g and the face-neighbour tiles whose stencil reaches it (adjoint rounds:
+// one byte per tile)
 __device__ __forceinline__ void seed_tiles(const InstDev& I, int nd, const int* g, uint8_t* flags) {
     const int TX = nd == 2 ? 16 : 8, TY = TX, TZ = nd == 2 ? 1 : 8;
     const int tx = g[0] / TX, ty = g[1] / TY, tz = nd == 3 ? g[2] / TZ : 0;
@@ -123,6 +126,39 @@ __device__ __forceinline__ void seed_tiles(const InstDev& I, int nd, const int* 
     if (nd == 3) {
         if (lz < 2 && tz > 0) fl[tl - t01] = 1;
         if (lz >= TZ - 2 && tz < I.t[2] - 1) fl[tl + t01] = 1;
+    }
+}
+
+// Forward tile flags (one word per tile): bits 0-5 = faces (x-, x+, y-, y+, z-, z+) whose 2-deep
+// halo changed since the tile last ran, kFullBit = relax every node.  The pending value is the
+// smallest changed key, as float bits rounded down.
+constexpr unsigned kFullBit = 0x40u;
+// kKeyBit: nodes were left pending at the band limit; relax every node whose key reaches the
+// pending value and that has a reached neighbour.
+constexpr unsigned kKeyBit = 0x80u;
+
+// Seed node g with value val: its tile and the face neighbours whose stencil reaches it relax
+// every node in their next round.
+__device__ __forceinline__ void seed_tiles_fwd(const InstDev& I, int nd, const int* g, unsigned* flags,
+                                               unsigned* pend, double val) {
+    const int TX = nd == 2 ? 16 : 8, TY = TX, TZ = nd == 2 ? 1 : 8;
+    const int tx = g[0] / TX, ty = g[1] / TY, tz = nd == 3 ? g[2] / TZ : 0;
+    const int64_t t0 = I.t[0], t01 = (int64_t)I.t[0] * I.t[1];
+    const int64_t tl = tx + t0 * ty + t01 * tz;
+    const unsigned pv = __float_as_uint(__double2float_rd(fabs(val)));
+    auto mark = [&](int64_t t) {
+        atomicOr(&flags[I.tile_base + t], kFullBit);
+        atomicMin(&pend[I.tile_base + t], pv);
+    };
+    mark(tl);
+    const int lx = g[0] - tx * TX, ly = g[1] - ty * TY, lz = nd == 3 ? g[2] - tz * TZ : 0;
+    if (lx < 2 && tx > 0) mark(tl - 1);
+    if (lx >= TX - 2 && tx < I.t[0] - 1) mark(tl + 1);
+    if (ly < 2 && ty > 0) mark(tl - t0);
+    if (ly >= TY - 2 && ty < I.t[1] - 1) mark(tl + t0);
+    if (nd == 3) {
+        if (lz < 2 && tz > 0) mark(tl - t01);
+        if (lz >= TZ - 2 && tz < I.t[2] - 1) mark(tl + t01);
     }
 }
 
@@ -251,6 +287,13 @@ struct DevBuf {
     }
 };
 
+// solver rounds enqueued between two host checks for convergence
+constexpr int kRoundBatch = 8;
+// device counters (ctx->d_small, 32-bit words): list length, list cursor, flagged tiles, sweeps,
+// non-empty rounds; from word kCnt64 on, 64-bit node updates, changed updates, tile activations
+constexpr size_t kSmallBytes = 256;
+constexpr int kCntList = 0, kCntCursor = 1, kCntFlagged = 2, kCntSweeps = 3, kCntRounds = 4, kCnt64 = 8;
+
 struct eik_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -259,8 +302,8 @@ struct eik_ctx {
     void* d_small = nullptr;  // device counters
     void* h_small = nullptr;  // pinned host mirror of the counters
     hipEvent_t ev[2] = {nullptr, nullptr};   // phase timer
-    hipEvent_t kev[2] = {nullptr, nullptr};  // per-launch kernel timer
-    double stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    hipEvent_t kev[2 * kRoundBatch] = {};    // round-kernel timers (start, end) of one round batch
+    double stats[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     int xcap = 1;  // capacity per source of the transferred-point list
     std::mutex mu;
     DevBuf& buf(const char* name) { return bufs[name]; }
@@ -282,6 +325,7 @@ struct eik_model {
     double* d_vel = nullptr;
     double* d_slow64 = nullptr;
     float* d_slow32 = nullptr;
+    double smean = 1.0;  // mean slowness (band scheduler scale)
 };
 
 #define CK(x)                                                                                    \
@@ -336,6 +380,9 @@ struct FimStats {
     int64_t rounds = 0;        // solver rounds (one launch each)
     double activations = 0;    // tile activations (blocks launched)
     double sweeps = 0;         // local relaxation sweeps summed over activations
+    double updates = 0;        // node updates (stencil replays) summed over activations
+    double changed = 0;        // node updates that changed the node
+    double keychanged = 0;     // of those, the ones that changed only the pop key
     double kernel_ms = 0;      // summed duration of the round launches
 };
 // traveltime pool plus its pop-key pools (K, S in the pool's precision, D as bytes)

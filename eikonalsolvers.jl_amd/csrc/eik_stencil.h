@@ -31,117 +31,120 @@
 
 namespace eik {
 
+template <typename T>
+__device__ __forceinline__ T tinf();
+template <>
+__device__ __forceinline__ double tinf<double>() {
+    return EIK_INF;
+}
+template <>
+__device__ __forceinline__ float tinf<float>() {
+    return EIK_INFF;
+}
+
+
+// Per-node stencil metric: spacing per axis and the quadratic coefficients of the 1st-order
+// (1/h^2) and 2nd-order (9/(4 h^2)) one-sided differences, computed once per node update rather
+// than once per stencil evaluation.
+template <typename T>
+struct Metric {
+    T dh[3], c1[3], c2[3];
+};
+__device__ __forceinline__ void metric_coefs(Metric<double>& m, int nd) {
+#pragma clang fp contract(off)
+    for (int ax = 0; ax < nd; ax++) {
+        const double h = m.dh[ax];
+        m.c1[ax] = 1.0 / (h * h);
+        m.c2[ax] = 9.0 / (4.0 * (h * h));
+    }
+}
+__device__ __forceinline__ void metric_coefs(Metric<float>& m, int nd) {
+    for (int ax = 0; ax < nd; ax++) {
+        const float ih2 = 1.0f / (m.dh[ax] * m.dh[ax]);
+        m.c1[ax] = ih2;
+        m.c2[ax] = 2.25f * ih2;
+    }
+}
+
+// Neighbour choice along one axis, as calcttpt_2ndord! scans it (+ side first, strict <): cv1 =
+// the smaller accepted adjacent value, 2nd order if that side's 2-away node is accepted and not
+// larger.  When the - side wins without a usable 2-away node the + side's 2nd-order choice is
+// kept (the reference does not reset it).  Written with selects only, so a wave never diverges.
+template <typename T>
+struct AxisPick {
+    T cv1, cv2;
+    bool use1, use2;
+    int codeA, codeB;  // derivative code of the mixed-order and of the all-1st-order stencil
+};
+template <typename T>
+__device__ __forceinline__ AxisPick<T> pick_axis(const T* v1, const T* v2, unsigned acc1, unsigned acc2, int ax) {
+    const int kp = 2 * ax, km = 2 * ax + 1;
+    AxisPick<T> p;
+    // + side (cv1 starts at +inf)
+    const bool p1 = (bool)((acc1 >> kp) & 1u) & (v1[kp] < tinf<T>());
+    const T c1p = p1 ? v1[kp] : tinf<T>();
+    const bool p2 = p1 & (bool)((acc2 >> kp) & 1u);
+    const bool u2p = p2 & (v2[kp] <= c1p);
+    // - side
+    const bool q1 = (bool)((acc1 >> km) & 1u) & (v1[km] < c1p);
+    const bool q2 = q1 & (bool)((acc2 >> km) & 1u);
+    p.cv1 = q1 ? v1[km] : c1p;
+    const bool u2m = q2 & (v2[km] <= p.cv1);
+    p.use1 = p1 | q1;
+    p.use2 = q2 ? u2m : u2p;
+    p.cv2 = q2 ? (u2m ? v2[km] : tinf<T>()) : (u2p ? v2[kp] : tinf<T>());
+    p.codeA = q1 ? (u2m ? 2 : 1) : (p1 ? (u2p ? -2 : -1) : 0);
+    p.codeB = q1 ? 1 : (p1 ? -1 : 0);
+    return p;
+}
+
 // ---------------------------------------------------------------- FP64 (reference arithmetic)
+// calcttpt_2ndord! with the reference's operation order: the mixed-order quadratic, its
+// all-1st-order fallback when the discriminant is negative, then the 1-D fallback t1 + h s.
+// Both quadratics are accumulated in one pass and one square root and one division are taken
+// on the one selected; max(t1, t2) is always t1 here (alpha > 0, rounding is monotone).
 template <int ND>
 __device__ __forceinline__ double eval64(const double* v1, const double* v2, unsigned acc1, unsigned acc2,
-                                         double s, const double* dh, int* code) {
+                                         double s, const Metric<double>& m, int* code) {
 #pragma clang fp contract(off)
     double alpha = 0.0, beta = 0.0, gamma = -(s * s);
-#pragma unroll
-    for (int d = 0; d < ND; d++) code[d] = 0;
+    double alpha1 = 0.0, beta1 = 0.0, gamma1 = -(s * s);
+    double cmin = EIK_INF, hmin = m.dh[0];
+    int idc = 0;
+    int cA[3] = {0, 0, 0}, cB[3] = {0, 0, 0};
 #pragma unroll
     for (int ax = 0; ax < ND; ax++) {
-        bool use1 = false, use2 = false;
-        double cv1 = EIK_INF, cv2 = EIK_INF;
-#pragma unroll
-        for (int l = 0; l < 2; l++) {
-            const int k = 2 * ax + l;
-            const int sh = (l == 0) ? 1 : -1;
-            const bool a1 = (acc1 >> k) & 1u;
-            if (a1) {
-                const double tv1 = v1[k];
-                if (tv1 < cv1) {
-                    cv1 = tv1;
-                    use1 = true;
-                    code[ax] = -sh;
-                    const bool a2 = (acc2 >> k) & 1u;
-                    if (a2) {
-                        const double tv2 = v2[k];
-                        if (tv2 <= cv1) {
-                            cv2 = tv2;
-                            use2 = true;
-                            code[ax] = -2 * sh;
-                        } else {
-                            cv2 = EIK_INF;
-                            use2 = false;
-                            code[ax] = -sh;
-                        }
-                    }
-                }
-            }
-        }
-        const double h = dh[ax];
-        if (use2 && use1) {
-            const double tmpa2 = 1.0 / 3.0 * (4.0 * cv1 - cv2);
-            const double ca = 9.0 / (4.0 * (h * h));
-            alpha += ca;
-            beta += (-2.0 * ca * tmpa2);
-            gamma += ca * (tmpa2 * tmpa2);
-        } else if (use1) {
-            const double ca = 1.0 / (h * h);
-            alpha += ca;
-            beta += (-2.0 * ca * cv1);
-            gamma += ca * (cv1 * cv1);
-        }
+        const AxisPick<double> p = pick_axis<double>(v1, v2, acc1, acc2, ax);
+        cA[ax] = p.codeA;
+        cB[ax] = p.codeB;
+        const bool sec = p.use1 && p.use2;
+        const double tmpa2 = 1.0 / 3.0 * (4.0 * p.cv1 - p.cv2);
+        const double ca = sec ? m.c2[ax] : m.c1[ax];
+        const double tv = sec ? tmpa2 : p.cv1;
+        alpha = p.use1 ? alpha + ca : alpha;
+        beta = p.use1 ? beta + (-2.0 * ca * tv) : beta;
+        gamma = p.use1 ? gamma + ca * (tv * tv) : gamma;
+        const double cb = m.c1[ax];
+        alpha1 = p.use1 ? alpha1 + cb : alpha1;
+        beta1 = p.use1 ? beta1 + (-2.0 * cb * p.cv1) : beta1;
+        gamma1 = p.use1 ? gamma1 + cb * (p.cv1 * p.cv1) : gamma1;
+        // 1-D fallback: first axis with the smallest adjacent value
+        const bool better = (ax == 0) || p.cv1 < cmin;
+        idc = better ? ax : idc;
+        cmin = better ? p.cv1 : cmin;
+        hmin = better ? m.dh[ax] : hmin;
     }
-    double sqarg = beta * beta - 4.0 * alpha * gamma;
-    if (sqarg < 0.0) {  // all axes 1st order (eikforward2D.jl:265-337)
+    const double sqA = beta * beta - 4.0 * alpha * gamma;
+    const double sqB = beta1 * beta1 - 4.0 * alpha1 * gamma1;
+    const bool useA = !(sqA < 0.0);
+    const bool useB = !useA && !(sqB < 0.0);
+    const double a = useA ? alpha : alpha1;
+    const double b = useA ? beta : beta1;
+    const double q = useA ? sqA : sqB;
 #pragma unroll
-        for (int d = 0; d < ND; d++) code[d] = 0;
-        alpha = 0.0;
-        beta = 0.0;
-        gamma = -(s * s);
-#pragma unroll
-        for (int ax = 0; ax < ND; ax++) {
-            bool use1 = false;
-            double cv1 = EIK_INF;
-#pragma unroll
-            for (int l = 0; l < 2; l++) {
-                const int k = 2 * ax + l;
-                const bool a1 = (acc1 >> k) & 1u;
-                if (a1 && v1[k] < cv1) {
-                    cv1 = v1[k];
-                    use1 = true;
-                    code[ax] = (l == 0) ? -1 : 1;
-                }
-            }
-            if (use1) {
-                const double h = dh[ax];
-                const double ca = 1.0 / (h * h);
-                alpha += ca;
-                beta += (-2.0 * ca * cv1);
-                gamma += ca * (cv1 * cv1);
-            }
-        }
-        sqarg = beta * beta - 4.0 * alpha * gamma;
-    }
-    if (sqarg < 0.0) {  // one dimension (eikforward2D.jl:340-421)
-        double cvv[3] = {EIK_INF, EIK_INF, EIK_INF};
-        int cc[3] = {0, 0, 0};
-#pragma unroll
-        for (int ax = 0; ax < ND; ax++) {
-#pragma unroll
-            for (int l = 0; l < 2; l++) {
-                const int k = 2 * ax + l;
-                const bool a1 = (acc1 >> k) & 1u;
-                if (a1 && v1[k] < cvv[ax]) {
-                    cvv[ax] = v1[k];
-                    cc[ax] = (l == 0) ? -1 : 1;
-                }
-            }
-        }
-        int idc = 0;
-#pragma unroll
-        for (int ax = 1; ax < ND; ax++)
-            if (cvv[ax] < cvv[idc]) idc = ax;
-#pragma unroll
-        for (int d = 0; d < ND; d++) code[d] = (d == idc) ? cc[d] : 0;
-        return (dh[idc] * s) + cvv[idc];
-    }
-    const double tmpsq = sqrt(sqarg);
-    const double t1 = (-beta + tmpsq) / (2.0 * alpha);
-    const double t2 = (-beta - tmpsq) / (2.0 * alpha);
-    return fmax(t1, t2);
+    for (int d = 0; d < ND; d++) code[d] = useA ? cA[d] : (useB ? cB[d] : (d == idc ? cB[d] : 0));
+    if (useA || useB) return (-b + sqrt(q)) / (2.0 * a);
+    return (hmin * s) + cmin;
 }
 
 // ---------------------------------------------------------------- FP32 (shifted quadratic)
@@ -151,108 +154,48 @@ __device__ __forceinline__ double eval64(const double* v1, const double* v2, uns
 // them to the cancellation in beta^2 - 4 alpha gamma (both terms ~ (alpha T)^2).
 template <int ND>
 __device__ __forceinline__ float eval32(const float* v1, const float* v2, unsigned acc1, unsigned acc2, float s,
-                                        const float* dh, int* code) {
-    float t1a[3], t2a[3], al[3];
-    int mode[3];  // 0 unused, 1 first order, 2 second order
-#pragma unroll
-    for (int ax = 0; ax < ND; ax++) {
-        bool use1 = false, use2 = false;
-        float cv1 = EIK_INFF, cv2 = EIK_INFF;
-        code[ax] = 0;
-#pragma unroll
-        for (int l = 0; l < 2; l++) {
-            const int k = 2 * ax + l;
-            const int sh = (l == 0) ? 1 : -1;
-            const bool a1 = (acc1 >> k) & 1u;
-            if (a1) {
-                const float tv1 = v1[k];
-                if (tv1 < cv1) {
-                    cv1 = tv1;
-                    use1 = true;
-                    code[ax] = -sh;
-                    const bool a2 = (acc2 >> k) & 1u;
-                    if (a2) {
-                        const float tv2 = v2[k];
-                        if (tv2 <= cv1) {
-                            cv2 = tv2;
-                            use2 = true;
-                            code[ax] = -2 * sh;
-                        } else {
-                            cv2 = EIK_INFF;
-                            use2 = false;
-                            code[ax] = -sh;
-                        }
-                    }
-                }
-            }
-        }
-        const float ih2 = 1.0f / (dh[ax] * dh[ax]);
-        mode[ax] = (use1 && use2) ? 2 : (use1 ? 1 : 0);
-        t1a[ax] = cv1;
-        t2a[ax] = cv2;
-        al[ax] = (mode[ax] == 2) ? 2.25f * ih2 : ih2;
-    }
+                                        const Metric<float>& m, int* code) {
+    AxisPick<float> p[3];
     float r = EIK_INFF;
 #pragma unroll
-    for (int ax = 0; ax < ND; ax++)
-        if (mode[ax]) r = fminf(r, t1a[ax]);
+    for (int ax = 0; ax < ND; ax++) {
+        p[ax] = pick_axis<float>(v1, v2, acc1, acc2, ax);
+        r = p[ax].use1 ? fminf(r, p[ax].cv1) : r;
+    }
     float A = 0.f, B = 0.f, C = -(s * s);
+    float A1 = 0.f, B1 = 0.f, C1 = -(s * s);
+    float cmin = EIK_INFF, hmin = m.dh[0];
+    int idc = 0;
 #pragma unroll
     for (int ax = 0; ax < ND; ax++) {
-        if (!mode[ax]) continue;
-        float d = t1a[ax] - r;
-        if (mode[ax] == 2) d += (t1a[ax] - t2a[ax]) * (1.0f / 3.0f);
-        A += al[ax];
-        B += al[ax] * d;
-        C += al[ax] * d * d;
+        const bool sec = p[ax].use1 && p[ax].use2;
+        const float al = sec ? m.c2[ax] : m.c1[ax];
+        const float d1 = p[ax].cv1 - r;
+        const float d = sec ? d1 + (p[ax].cv1 - p[ax].cv2) * (1.0f / 3.0f) : d1;
+        A = p[ax].use1 ? A + al : A;
+        B = p[ax].use1 ? B + al * d : B;
+        C = p[ax].use1 ? C + al * d * d : C;
+        const float ih2 = m.c1[ax];
+        A1 = p[ax].use1 ? A1 + ih2 : A1;
+        B1 = p[ax].use1 ? B1 + ih2 * d1 : B1;
+        C1 = p[ax].use1 ? C1 + ih2 * d1 * d1 : C1;
+        const bool better = (ax == 0) || p[ax].cv1 < cmin;
+        idc = better ? ax : idc;
+        cmin = better ? p[ax].cv1 : cmin;
+        hmin = better ? m.dh[ax] : hmin;
     }
-    float disc = B * B - A * C;
-    if (disc < 0.f) {  // all axes 1st order
-        A = 0.f;
-        B = 0.f;
-        C = -(s * s);
-        r = EIK_INFF;
-        float c1[3];
-        bool u[3];
+    const float discA = B * B - A * C;
+    const float discB = B1 * B1 - A1 * C1;
+    const bool useA = !(discA < 0.f);
+    const bool useB = !useA && !(discB < 0.f);
 #pragma unroll
-        for (int ax = 0; ax < ND; ax++) {
-            c1[ax] = EIK_INFF;
-            u[ax] = false;
-            code[ax] = 0;
-#pragma unroll
-            for (int l = 0; l < 2; l++) {
-                const int k = 2 * ax + l;
-                const bool a1 = (acc1 >> k) & 1u;
-                if (a1 && v1[k] < c1[ax]) {
-                    c1[ax] = v1[k];
-                    u[ax] = true;
-                    code[ax] = (l == 0) ? -1 : 1;
-                }
-            }
-            if (u[ax]) r = fminf(r, c1[ax]);
-        }
-#pragma unroll
-        for (int ax = 0; ax < ND; ax++) {
-            if (!u[ax]) continue;
-            const float ih2 = 1.0f / (dh[ax] * dh[ax]);
-            const float d = c1[ax] - r;
-            A += ih2;
-            B += ih2 * d;
-            C += ih2 * d * d;
-        }
-        disc = B * B - A * C;
-        if (disc < 0.f) {  // one dimension
-            int idc = 0;
-#pragma unroll
-            for (int ax = 1; ax < ND; ax++)
-                if (c1[ax] < c1[idc]) idc = ax;
-            const int keep = code[idc];
-#pragma unroll
-            for (int d = 0; d < ND; d++) code[d] = (d == idc) ? keep : 0;
-            return c1[idc] + dh[idc] * s;
-        }
-    }
-    return r + (B + sqrtf(disc)) / A;
+    for (int d = 0; d < ND; d++)
+        code[d] = useA ? p[d].codeA : (useB ? p[d].codeB : (d == idc ? p[d].codeB : 0));
+    const float a = useA ? A : A1;
+    const float b = useA ? B : B1;
+    const float q = useA ? discA : discB;
+    if (useA || useB) return r + (b + sqrtf(q)) / a;
+    return cmin + hmin * s;
 }
 
 // ---------------------------------------------------------------- heap pop keys
@@ -268,53 +211,28 @@ struct Key {
     T K, S;
     int D;
 };
+// (bitwise & and | keep the comparisons branch-free)
 template <typename T>
 __device__ __forceinline__ bool klt(const Key<T>& a, const Key<T>& b) {
-    if (a.K != b.K) return a.K < b.K;
-    if (a.S != b.S) return a.S < b.S;
-    return a.D < b.D;
+    return (a.K < b.K) | ((a.K == b.K) & ((a.S < b.S) | ((a.S == b.S) & (a.D < b.D))));
 }
 template <typename T>
 __device__ __forceinline__ bool keq(const Key<T>& a, const Key<T>& b) {
-    return a.K == b.K && a.S == b.S && a.D == b.D;
+    return (a.K == b.K) & (a.S == b.S) & (a.D == b.D);
 }
 // A cascade can never be deeper than this; a longer chain can only come from two nodes feeding
 // each other stale keys during the iteration, and ignoring such a candidate breaks the cycle.
 constexpr int kMaxCascade = 16;
 
-template <int ND, typename T>
-__device__ __forceinline__ T eval_any(const T* v1, const T* v2, unsigned acc1, unsigned acc2, T s, const T* dh,
-                                      int* code);
-template <>
-__device__ __forceinline__ double eval_any<2, double>(const double* v1, const double* v2, unsigned acc1, unsigned acc2,
-                                                      double s, const double* dh, int* code) {
-    return eval64<2>(v1, v2, acc1, acc2, s, dh, code);
+template <int ND>
+__device__ __forceinline__ double eval_any(const double* v1, const double* v2, unsigned acc1, unsigned acc2, double s,
+                                           const Metric<double>& m, int* code) {
+    return eval64<ND>(v1, v2, acc1, acc2, s, m, code);
 }
-template <>
-__device__ __forceinline__ double eval_any<3, double>(const double* v1, const double* v2, unsigned acc1, unsigned acc2,
-                                                      double s, const double* dh, int* code) {
-    return eval64<3>(v1, v2, acc1, acc2, s, dh, code);
-}
-template <>
-__device__ __forceinline__ float eval_any<2, float>(const float* v1, const float* v2, unsigned acc1, unsigned acc2,
-                                                    float s, const float* dh, int* code) {
-    return eval32<2>(v1, v2, acc1, acc2, s, dh, code);
-}
-template <>
-__device__ __forceinline__ float eval_any<3, float>(const float* v1, const float* v2, unsigned acc1, unsigned acc2,
-                                                    float s, const float* dh, int* code) {
-    return eval32<3>(v1, v2, acc1, acc2, s, dh, code);
-}
-
-template <typename T>
-__device__ __forceinline__ T tinf();
-template <>
-__device__ __forceinline__ double tinf<double>() {
-    return EIK_INF;
-}
-template <>
-__device__ __forceinline__ float tinf<float>() {
-    return EIK_INFF;
+template <int ND>
+__device__ __forceinline__ float eval_any(const float* v1, const float* v2, unsigned acc1, unsigned acc2, float s,
+                                          const Metric<float>& m, int* code) {
+    return eval32<ND>(v1, v2, acc1, acc2, s, m, code);
 }
 
 // Local replay of the heap (see the file comment).  v1/v2: adjacent / 2-away values; k1/k2 their
@@ -322,50 +240,54 @@ __device__ __forceinline__ float tinf<float>() {
 // the node's pop key and code to the derivative codes of the last stencil evaluation.
 template <int ND, typename T>
 __device__ __forceinline__ T replay(const T* v1, const T* v2, const Key<T>* k1, const Key<T>* k2, unsigned f1,
-                                    unsigned f2, unsigned ok1, unsigned ok2, T s, const T* dh, int* code,
+                                    unsigned f2, unsigned ok1, unsigned ok2, T s, const Metric<T>& m, int* code,
                                     Key<T>* kout) {
-    int ord[2 * ND];
-    int nc = 0;
+    // candidates: adjacent nodes inside the grid, not init, reached, cascade depth below the cap
+    unsigned cand = 0u;
     bool haveinit = false;
 #pragma unroll
     for (int k = 0; k < 2 * ND; k++) {
-        if (!((ok1 >> k) & 1u)) continue;
-        if ((f1 >> k) & 1u) {
-            haveinit = true;
-        } else if (v1[k] < tinf<T>() && k1[k].D < kMaxCascade) {
-            int j = nc++;
-            while (j > 0 && klt(k1[k], k1[ord[j - 1]])) {
-                ord[j] = ord[j - 1];
-                j--;
-            }
-            ord[j] = k;
-        }
+        const bool in = (ok1 >> k) & 1u, fx = (f1 >> k) & 1u;
+        haveinit |= in & fx;
+        cand |= (unsigned)(in & !fx & (v1[k] < tinf<T>()) & (k1[k].D < kMaxCascade)) << k;
     }
     T cur = tinf<T>();
     Key<T> kz{tinf<T>(), -tinf<T>(), 0};
 #pragma unroll
     for (int d = 0; d < ND; d++) code[d] = 0;
     if (haveinit) {
-        cur = eval_any<ND, T>(v1, v2, f1 & ok1, f2 & ok2, s, dh, code);
+        cur = eval_any<ND>(v1, v2, f1 & ok1, f2 & ok2, s, m, code);
         kz = Key<T>{cur, -tinf<T>(), 0};
     }
-    for (int i = 0; i < nc; i++) {
-        const Key<T> c = k1[ord[i]];
-        if (i + 1 < nc && keq(k1[ord[i + 1]], c)) continue;  // one batch of equal keys
-        if (!klt(c, kz)) break;
+    // distinct candidate keys in increasing order (equal keys pop as one batch); each step scans
+    // the 2*ND slots with static indices so everything stays in registers
+    Key<T> prev{-tinf<T>(), -tinf<T>(), -1};
+    for (int step = 0; step < 2 * ND && cand; step++) {
+        Key<T> c{tinf<T>(), tinf<T>(), 1 << 30};
+        bool found = false;
+#pragma unroll
+        for (int k = 0; k < 2 * ND; k++) {
+            const bool take = (bool)((cand >> k) & 1u) & klt(prev, k1[k]) & (!found | klt(k1[k], c));
+            c.K = take ? k1[k].K : c.K;
+            c.S = take ? k1[k].S : c.S;
+            c.D = take ? k1[k].D : c.D;
+            found |= take;
+        }
+        if (!found || !klt(c, kz)) break;
+        prev = c;
         unsigned acc1 = 0u, acc2 = 0u;
 #pragma unroll
         for (int k = 0; k < 2 * ND; k++) {
-            const bool a1 = ((ok1 >> k) & 1u) && (((f1 >> k) & 1u) || (v1[k] < tinf<T>() && !klt(c, k1[k])));
-            const bool a2 = ((ok2 >> k) & 1u) && (((f2 >> k) & 1u) || (v2[k] < tinf<T>() && !klt(c, k2[k])));
+            const bool a1 = (bool)((ok1 >> k) & 1u) & ((bool)((f1 >> k) & 1u) | ((v1[k] < tinf<T>()) & !klt(c, k1[k])));
+            const bool a2 = (bool)((ok2 >> k) & 1u) & ((bool)((f2 >> k) & 1u) | ((v2[k] < tinf<T>()) & !klt(c, k2[k])));
             acc1 |= (unsigned)a1 << k;
             acc2 |= (unsigned)a2 << k;
         }
-        cur = eval_any<ND, T>(v1, v2, acc1, acc2, s, dh, code);
-        if (cur >= c.K)
-            kz = Key<T>{cur, -tinf<T>(), 0};
-        else
-            kz = Key<T>{c.K, cur > c.S ? cur : c.S, c.D + 1};
+        cur = eval_any<ND>(v1, v2, acc1, acc2, s, m, code);
+        const bool ord = cur >= c.K;
+        kz.K = ord ? cur : c.K;
+        kz.S = ord ? -tinf<T>() : (cur > c.S ? cur : c.S);
+        kz.D = ord ? 0 : c.D + 1;
     }
     *kout = kz;
     return cur;

@@ -48,10 +48,11 @@ class Context:
             pass
 
     def last_stats(self):
-        out = np.zeros(8)
-        _lib.load().eik_last_stats(self.handle, _lib.ptr(out), 8)
+        out = np.zeros(12)
+        _lib.load().eik_last_stats(self.handle, _lib.ptr(out), 12)
         return {"total_ms": out[0], "fine_ms": out[1], "coarse_ms": out[2], "adjoint_ms": out[3],
-                "tile_activations": out[4], "rounds": out[5], "_kernel_ms": out[6], "tile_sweeps": out[7]}
+                "tile_activations": out[4], "rounds": out[5], "_kernel_ms": out[6], "tile_sweeps": out[7],
+                "node_updates": out[8], "changed_updates": out[9], "key_only_updates": out[10]}
 
 
 _default_ctx = None

@@ -106,7 +106,8 @@ int eik_misfit(eik_ctx* ctx, eik_model* model, int64_t nsrc, const double* coord
  * out[0] = total, [1] = near-source (fine grid) solve, [2] = coarse solve, [3] = adjoint,
  * [4] = tile activations of the coarse solve (node-visits = activations x tile nodes),
  * [5] = coarse solve rounds (launches), [6] = summed duration of those launches (ms, HIP events
- * on the library's stream), [7] = local relaxation sweeps summed over the activations. */
+ * on the library's stream), [7] = local relaxation sweeps summed over the activations,
+ * [8] = node updates (stencil replays) of the coarse solve, [9] = those that changed the node. */
 int eik_last_stats(const eik_ctx* ctx, double* out, int32_t nout);
 
 #ifdef __cplusplus
