@@ -33,16 +33,26 @@ BYTES_PER_VISIT = {"fp64": 24, "mixed": 12}  # slowness + tt read + tt write per
 def run_steps(model, src, recs, ep, steps, warmup):
     for _ in range(warmup):
         model.traveltime(src, recs, extraparams=ep)
-    kern_ms = acts = rounds = 0.0
+    tot = {"_kernel_ms": 0.0, "tile_activations": 0.0, "rounds": 0.0, "node_updates": 0.0, "changed_updates": 0.0,
+           "tile_sweeps": 0.0}
     t0 = time.perf_counter()
     for _ in range(steps):
         model.traveltime(src, recs, extraparams=ep)  # synchronous: returns after the device is idle
         st = model.ctx.last_stats()
-        kern_ms += st["_kernel_ms"]
-        acts += st["tile_activations"]
-        rounds += st["rounds"]
+        for k in tot:
+            tot[k] += st[k]
     dt = time.perf_counter() - t0
-    return dt, kern_ms, acts, rounds
+    return dt, tot
+
+
+def _traffic_from_profile():
+    """HBM bytes per k_fim_round<3,double> launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
+    (scaled by the same counters on a plain copy) and committed under profiles/; None if absent."""
+    p = os.path.join(ROOT, "profiles", "k_fim_round_3d_fp64_traffic.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        return json.load(f).get("hbm_bytes_per_launch")
 
 
 def main() -> int:
@@ -52,7 +62,20 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=0, help="sources timed on the CPU (0 = threads used)")
+    ap.add_argument("--legs", default="all", choices=["all", "fp64"],
+                    help="fp64: only the FP64 forward leg (for counter-collection runs)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run only a plain device copy of 512 MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        x = torch.empty(512 << 20, dtype=torch.uint8, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": x.numel(), "copy_probe_bytes_written": y.numel(), "launches": 5}))
+        return 0
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it "
                                    "(sources would shard by chunk, see DESIGN.md)", "n_gpus": args.gpus}))
@@ -67,19 +90,27 @@ def main() -> int:
     model = eik.EikModel(vel, grd)
 
     res = {}
-    for prec in ("fp64", "mixed"):
+    for prec in (("fp64", "mixed") if args.legs == "all" else ("fp64",)):
         ep = ExtraParams(precision=prec)
-        dt, kern_ms, acts, rounds = run_steps(model, src, recs, ep, args.steps, args.warmup)
-        visits = acts * TILE_NODES_3D
+        dt, tot = run_steps(model, src, recs, ep, args.steps, args.warmup)
+        kern_ms = tot["_kernel_ms"]
+        visits = tot["tile_activations"] * TILE_NODES_3D
+        nn = args.steps * nsrc * grd.npts
         res[prec] = {
             "solves_per_s": nsrc * args.steps / dt,
             "ms_per_step": 1e3 * dt / args.steps,
             "kernel_ms_per_step": kern_ms / args.steps,
-            "launches_per_step": rounds / args.steps,
+            "rounds_per_step": tot["rounds"] / args.steps,
             "node_visits_per_step": visits / args.steps,
-            "visits_per_node": visits / args.steps / (nsrc * grd.npts),
+            "visits_per_node": visits / nn,
+            "updates_per_node": tot["node_updates"] / nn,
+            "changes_per_node": tot["changed_updates"] / nn,
+            "sweeps_per_activation": tot["tile_sweeps"] / max(tot["tile_activations"], 1.0),
             "achieved_gbs": (visits * BYTES_PER_VISIT[prec] / 1e9) / (kern_ms / 1e3) if kern_ms > 0 else None,
         }
+    if args.legs == "fp64":
+        print(json.dumps({"legs": "fp64", **{k: v for k, v in res["fp64"].items()}}), flush=True)
+        return 0
 
     # gradient evaluations: observed picks from the second model plus N(0, 0.01^2) noise
     obsmodel = eik.EikModel(velobs, grd)
@@ -116,7 +147,9 @@ def main() -> int:
                                "refinearoundsrc=true (df=5, noderadius=3), eiktraveltime",
                    "nsrc": nsrc, "grid": list(grd.grsize), "nrec_per_src": 64},
         "mixed_precision": {"solves_per_s": round(res["mixed"]["solves_per_s"], 3),
-                            "ms_per_step": round(res["mixed"]["ms_per_step"], 3), "dtype": "f32 stencil / f64 near-source"},
+                            "ms_per_step": round(res["mixed"]["ms_per_step"], 3), "dtype": "f32 stencil / f64 near-source",
+                            "updates_per_node": round(res["mixed"]["updates_per_node"], 3),
+                            "kernel_ms_per_step": round(res["mixed"]["kernel_ms_per_step"], 3)},
         "gradient": {"evals_per_s_fp64": round(g64["evals_per_s"], 3), "ms_per_call_fp64": round(g64["ms_per_call"], 3),
                      "evals_per_s_mixed": round(gmx["evals_per_s"], 3), "ms_per_call_mixed": round(gmx["ms_per_call"], 3),
                      "whichgrad": "gradvelandsrcloc", "misfit_fp64": g64["misf"],
@@ -134,9 +167,15 @@ def main() -> int:
         "bytes_per_node_visit": BYTES_PER_VISIT["fp64"],
         "node_visits_per_step": f["node_visits_per_step"],
         "visits_per_node": round(f["visits_per_node"], 3),
-        "launches_per_step": f["launches_per_step"],
+        "updates_per_node": round(f["updates_per_node"], 3),
+        "changes_per_node": round(f["changes_per_node"], 3),
+        "sweeps_per_activation": round(f["sweeps_per_activation"], 3),
+        "rounds_per_step": f["rounds_per_step"],
         "kernel_ms_per_step": round(f["kernel_ms_per_step"], 3),
     }
+    tr = _traffic_from_profile()
+    if tr is not None:
+        out["roofline"]["traffic"] = tr
 
     if not args.no_cpu_baseline:
         from oracle import oracle as O
