@@ -197,6 +197,7 @@ template <>
 struct AdjTile<3> {
     static constexpr int TX = 8, TY = 8, TZ = 8, H = 2, LX = 12, LY = 12, LZ = 12, NP = 2, MAXIT = 96;
 };
+static_assert(AdjTile<2>::TX == adj_tile_edge(2) && AdjTile<3>::TX == adj_tile_edge(3), "adjoint tile edge");
 
 // one adjoint round over the listed tiles
 template <int ND>
@@ -217,15 +218,15 @@ __global__ void __launch_bounds__(kAB) k_adj_round(const int64_t* __restrict__ l
     int lo = 0, hi = ninst - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        if (inst[mid].tile_base <= gid)
+        if (inst[mid].atile_base <= gid)
             lo = mid;
         else
             hi = mid - 1;
     }
     const InstDev I = inst[lo];
-    const int64_t tl = gid - I.tile_base;
-    const int tix = (int)(tl % I.t[0]), tiy = (int)((tl / I.t[0]) % I.t[1]);
-    const int tiz = ND == 3 ? (int)(tl / ((int64_t)I.t[0] * I.t[1])) : 0;
+    const int64_t tl = gid - I.atile_base;
+    const int tix = (int)(tl % I.at[0]), tiy = (int)((tl / I.at[0]) % I.at[1]);
+    const int tiz = ND == 3 ? (int)(tl / ((int64_t)I.at[0] * I.at[1])) : 0;
     const int ox = tix * S::TX, oy = tiy * S::TY, oz = tiz * S::TZ;
     const int64_t nx = I.n[0], nxy = (int64_t)I.n[0] * I.n[1];
     const uint16_t* code = codep + I.tt_off;
@@ -348,15 +349,15 @@ __global__ void __launch_bounds__(kAB) k_adj_round(const int64_t* __restrict__ l
     __syncthreads();
     if (tid == 0) {
         const int f = s_faces;
-        uint8_t* fl = flags + I.tile_base;
-        const int64_t t0 = I.t[0], t01 = (int64_t)I.t[0] * I.t[1];
+        uint8_t* fl = flags + I.atile_base;
+        const int64_t t0 = I.at[0], t01 = (int64_t)I.at[0] * I.at[1];
         if ((f & 1) && tix > 0) fl[tl - 1] = 1;
-        if ((f & 2) && tix < I.t[0] - 1) fl[tl + 1] = 1;
+        if ((f & 2) && tix < I.at[0] - 1) fl[tl + 1] = 1;
         if ((f & 4) && tiy > 0) fl[tl - t0] = 1;
-        if ((f & 8) && tiy < I.t[1] - 1) fl[tl + t0] = 1;
+        if ((f & 8) && tiy < I.at[1] - 1) fl[tl + t0] = 1;
         if (ND == 3) {
             if ((f & 16) && tiz > 0) fl[tl - t01] = 1;
-            if ((f & 32) && tiz < I.t[2] - 1) fl[tl + t01] = 1;
+            if ((f & 32) && tiz < I.at[2] - 1) fl[tl + t01] = 1;
         }
         if (it >= S::MAXIT) fl[tl] = 1;
     }
@@ -826,7 +827,7 @@ int gradient_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, const eik_par
     const InstDev* dci = (const InstDev*)ctx->buf("inst_coarse").p;
     void* ttc = ctx->buf("tt_coarse").p;
     const void* slow = prec == EIK_PREC_FP64 ? (const void*)M->d_slow64 : (const void*)M->d_slow32;
-    const int TX = nd == 2 ? 16 : 8, TZ = nd == 2 ? 1 : 8;
+    const int TX = adj_tile_edge(nd), TZ = nd == 2 ? 1 : TX;
     const int64_t tiles_per = (int64_t)((M->g.n[0] + TX - 1) / TX) * ((M->g.n[1] + TX - 1) / TX) *
                               (nd == 3 ? (M->g.n[2] + TZ - 1) / TZ : 1);
     const int64_t P_c = nsrc * N;
@@ -949,7 +950,7 @@ int gradient_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, const eik_par
         int64_t P_f = 0, ntiles_f = 0;
         for (int64_t s = 0; s < nsrc; s++) {
             P_f = std::max(P_f, finst[s].tt_off + (int64_t)finst[s].n[0] * finst[s].n[1] * finst[s].n[2]);
-            ntiles_f = std::max(ntiles_f, finst[s].tile_base + (int64_t)finst[s].t[0] * finst[s].t[1] * finst[s].t[2]);
+            ntiles_f = std::max(ntiles_f, finst[s].atile_base + (int64_t)finst[s].at[0] * finst[s].at[1] * finst[s].at[2]);
             for (int k = 0; k < fdesc[s].ncorn; k++) {
                 const int* c = fdesc[s].corn[k];
                 fpts.push_back(SrcPt{(int32_t)s, -1, c[0] + (int64_t)finst[s].n[0] * (c[1] + (int64_t)finst[s].n[1] * c[2])});

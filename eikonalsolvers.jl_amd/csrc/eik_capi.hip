@@ -23,6 +23,14 @@ void make_inst(const eik_grid& g, InstDev& I) {
     I.tstopD = 0;
 }
 
+void set_tiles(InstDev& I, int nd) {
+    const int fe = fwd_tile_edge(nd), ae = adj_tile_edge(nd);
+    for (int d = 0; d < 3; d++) {
+        I.t[d] = d < nd ? (I.n[d] + fe - 1) / fe : 1;
+        I.at[d] = d < nd ? (I.n[d] + ae - 1) / ae : 1;
+    }
+}
+
 static inline double gcoord(const eik_grid& g, int d, int64_t i) {
     const double h = (g.kind == EIK_GRID2D_CART || g.kind == EIK_GRID3D_CART) ? g.d[0] : g.d[d];
     return fma((double)i, h, g.o[d]);
@@ -91,10 +99,7 @@ void build_fine(const eik_grid& g, const double* src, int df, int noderadius, Fi
     for (int d = 0; d < 3; d++) gf.d[d] = (cart ? g.d[0] : g.d[d]) / df;
     for (int d = 0; d < nd; d++) gf.o[d] = gcoord(g, d, i1[d]);
     make_inst(gf, I);
-    const int TX = nd == 2 ? 16 : 8, TZ = nd == 2 ? 1 : 8;
-    I.t[0] = (I.n[0] + TX - 1) / TX;
-    I.t[1] = (I.n[1] + TX - 1) / TX;
-    I.t[2] = nd == 3 ? (I.n[2] + TZ - 1) / TZ : 1;
+    set_tiles(I, nd);
     F.ncorn = find_enclosing_box(gf, src, F.corn);
 }
 

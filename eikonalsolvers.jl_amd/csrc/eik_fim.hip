@@ -13,26 +13,34 @@
 namespace eik {
 
 // ================================================================= tile geometry
-// A tile is relaxed by one 256-thread block: 16x16 nodes in 2D (one per thread), 8x8x8 in 3D (two
-// per thread), staged in LDS with a 2-deep halo on every face (the 2nd-order stencil reads two
-// nodes along each axis; edges and corners of the halo are never read).
+// A tile is relaxed by one wavefront, one node per lane: 8x8 nodes in 2D, 4x4x4 in 3D, staged in
+// LDS with a 2-deep halo on every face (the 2nd-order stencil reads two nodes along each axis;
+// edges and corners of the halo are never read).  Small tiles keep the relaxation close to the
+// front (fewer repeated updates) and let many independent tiles share a CU.
 template <int ND>
 struct TileShape;
 template <>
 struct TileShape<2> {
-    static constexpr int TX = 16, TY = 16, TZ = 1, H = 2;
+    static constexpr int TX = 8, TY = 8, TZ = 1, H = 2;
     static constexpr int LX = TX + 2 * H, LY = TY + 2 * H, LZ = 1;
     static constexpr int NODES = TX * TY;
-    static constexpr int MAXIT = 128;
+    static constexpr int MAXIT = 64;
 };
 template <>
 struct TileShape<3> {
-    static constexpr int TX = 8, TY = 8, TZ = 8, H = 2;
+    static constexpr int TX = 4, TY = 4, TZ = 4, H = 2;
     static constexpr int LX = TX + 2 * H, LY = TY + 2 * H, LZ = TZ + 2 * H;
     static constexpr int NODES = TX * TY * TZ;
-    static constexpr int MAXIT = 96;
+    static constexpr int MAXIT = 64;
 };
-constexpr int kTileThreads = 256;
+static_assert(TileShape<2>::TX == fwd_tile_edge(2) && TileShape<3>::TX == fwd_tile_edge(3), "tile edge");
+constexpr int kTileThreads = 64;
+// Waves per SIMD the register allocation targets.  The kernel is bound by VALU/SALU issue, not
+// by latency: forcing 3 waves (168 registers) on the FP32 3D kernel measured 8 % slower than 2.
+template <int ND, typename T>
+struct RoundOcc {
+    static constexpr int value = 2;
+};
 
 __device__ __forceinline__ int find_inst(const InstDev* __restrict__ inst, int ninst, int64_t gid) {
     int lo = 0, hi = ninst - 1;
@@ -71,7 +79,8 @@ __device__ __forceinline__ float key_down(float k) { return k; }
 // changed activate the neighbour tile with that face's bit and the smallest changed key as its
 // pending value.  The list length is read on the device; blocks stride over it.
 template <int ND, typename T>
-__global__ void __launch_bounds__(kTileThreads) k_fim_round(
+__global__ void __attribute__((amdgpu_flat_work_group_size(kTileThreads, kTileThreads),
+                               amdgpu_waves_per_eu(RoundOcc<ND, T>::value))) k_fim_round(
     const int64_t* __restrict__ list, const uint2* __restrict__ info, unsigned* __restrict__ count,
     const InstDev* __restrict__ inst, int ninst, const unsigned* __restrict__ wmin, T* __restrict__ ttpool,
     T* __restrict__ kpool, T* __restrict__ spool, uint8_t* __restrict__ dpool, const T* __restrict__ slowpool,
@@ -127,6 +136,7 @@ __global__ void __launch_bounds__(kTileThreads) k_fim_round(
             s_nupd = 0u;
             s_nchg = 0u;
         }
+#pragma unroll
         for (int c = tid; c < NL; c += NB) {
             const int lx = c % S::LX, ly = (c / S::LX) % S::LY, lz = (ND == 3) ? c / (S::LX * S::LY) : 0;
             int nout = (lx < S::H || lx >= S::H + S::TX) + (ly < S::H || ly >= S::H + S::TY);
@@ -167,6 +177,8 @@ __global__ void __launch_bounds__(kTileThreads) k_fim_round(
         bool act[NPT];
         bool near[NPT];
         float ptr[NPT];  // pending trigger key
+        T v0[NPT];       // staged value and key of the node (the global state at entry)
+        Key<T> k0[NPT];
     #pragma unroll
         for (int j = 0; j < NPT; j++) {
             const int n = tid + j * NB;
@@ -187,6 +199,8 @@ __global__ void __launch_bounds__(kTileThreads) k_fim_round(
     #pragma unroll
         for (int j = 0; j < NPT; j++) {
             act[j] = act[j] && !is_fixed(s_t[li[j]]);
+            v0[j] = s_t[li[j]];
+            k0[j] = Key<T>{s_k[li[j]], s_s[li[j]], (int)s_d[li[j]]};
             ptr[j] = EIK_INFF;
             if (!act[j]) continue;
             if (fbits & kFullBit) {
@@ -287,8 +301,7 @@ __global__ void __launch_bounds__(kTileThreads) k_fim_round(
                     metric_coefs(mt, ND);
                 }
                 T v1[2 * ND], v2[2 * ND];
-                Key<T> k1[2 * ND], k2[2 * ND];
-                unsigned f1 = 0u, f2 = 0u;
+                unsigned f1 = 0u, f2 = 0u, d1 = 0u, d2 = 0u, dcap = 0u;
     #pragma unroll
                 for (int ax = 0; ax < ND; ax++) {
     #pragma unroll
@@ -296,27 +309,28 @@ __global__ void __launch_bounds__(kTileThreads) k_fim_round(
                         const int k = 2 * ax + l;
                         const int o1 = lc + ((l == 0) ? 1 : -1) * strides[ax];
                         const int o2 = lc + ((l == 0) ? 2 : -2) * strides[ax];
-                        T a = tinf<T>(), b = tinf<T>();
-                        Key<T> ka{tinf<T>(), -tinf<T>(), 0}, kb{tinf<T>(), -tinf<T>(), 0};
-                        if ((ok1 >> k) & 1u) {
-                            a = s_t[o1];
-                            ka = Key<T>{s_k[o1], s_s[o1], (int)s_d[o1]};
-                        }
-                        if ((ok2 >> k) & 1u) {
-                            b = s_t[o2];
-                            kb = Key<T>{s_k[o2], s_s[o2], (int)s_d[o2]};
-                        }
+                        const bool in1 = (ok1 >> k) & 1u, in2 = (ok2 >> k) & 1u;
+                        const T a = in1 ? s_t[o1] : tinf<T>();
+                        const T b = in2 ? s_t[o2] : tinf<T>();
+                        const int da = in1 ? (int)s_d[o1] : 0, db = in2 ? (int)s_d[o2] : 0;
                         f1 |= (unsigned)is_fixed(a) << k;
                         f2 |= (unsigned)is_fixed(b) << k;
+                        d1 |= (unsigned)(da != 0) << k;
+                        d2 |= (unsigned)(db != 0) << k;
+                        dcap |= (unsigned)(da >= kMaxCascade) << k;
                         v1[k] = fabs(a);
                         v2[k] = fabs(b);
-                        k1[k] = ka;
-                        k2[k] = kb;
                     }
                 }
+                // pop keys straight from LDS (only the cascade path reads them)
+                auto keys = [&](int k, bool two) {
+                    const int ax = k >> 1, l = k & 1;
+                    const int o = lc + (l == 0 ? 1 : -1) * (two ? 2 : 1) * strides[ax];
+                    return Key<T>{s_k[o], s_s[o], (int)s_d[o]};
+                };
                 int code[3];
                 Key<T> nk;
-                const T nv = replay<ND, T>(v1, v2, k1, k2, f1, f2, ok1, ok2, s_sl[n], mt, code, &nk);
+                const T nv = replay<ND, T>(v1, v2, keys, d1, d2, dcap, f1, f2, ok1, ok2, s_sl[n], mt, code, &nk);
                 nupd++;
                 const T cur = s_t[lc];
                 const T kold = s_k[lc];
@@ -352,10 +366,8 @@ __global__ void __launch_bounds__(kTileThreads) k_fim_round(
             const int64_t l = (ox + lx) + nx * (oy + ly) + nxy * (oz + lz);
             const T v = s_t[li[j]];
             const Key<T> kn{s_k[li[j]], s_s[li[j]], (int)s_d[li[j]]};
-            const T v0 = tt[l];
-            const uint8_t d0 = dd[l];
-            const Key<T> ko = d0 ? Key<T>{kk[l], ss[l], (int)d0} : Key<T>{fabs(v0), -tinf<T>(), 0};
-            if (v == v0 && keq(kn, ko)) continue;
+            const Key<T> ko = k0[j];
+            if (v == v0[j] && keq(kn, ko)) continue;
             tt[l] = v;
             dd[l] = (uint8_t)kn.D;
             if (kn.D) {
@@ -411,31 +423,93 @@ __global__ void __launch_bounds__(kTileThreads) k_fim_round(
     }
 }
 
-// Band scheduling over the active tiles.  Pass 1: lowest pending value per instance.
-__global__ void k_band_min(const unsigned* __restrict__ flags, const unsigned* __restrict__ pend, int64_t ntiles,
-                           const InstDev* __restrict__ inst, int ninst, unsigned* __restrict__ wmin,
-                           unsigned* __restrict__ nflagged) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// Band scheduling over the active tiles.  Pass 1: lowest pending value per instance.  Blocks of
+// kScanThreads consecutive tiles almost always lie in one instance: the minimum and the flagged
+// count are then reduced in LDS and published with one atomic each (atomics on a few hot
+// addresses from every wave serialise); blocks that straddle instances reduce per wave.
+constexpr int kScanThreads = 1024;
+
+__global__ void __launch_bounds__(kScanThreads) k_band_min(const unsigned* __restrict__ flags,
+                                                         const unsigned* __restrict__ pend, int64_t ntiles,
+                                                         const InstDev* __restrict__ inst, int ninst,
+                                                         unsigned* __restrict__ wmin, unsigned* __restrict__ nflagged) {
+    __shared__ unsigned s_min, s_cnt;
+    const int64_t b0 = (int64_t)blockIdx.x * kScanThreads;
+    const int64_t t = b0 + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int ilo = find_inst(inst, ninst, b0);
+    const int ihi = find_inst(inst, ninst, (b0 + kScanThreads < ntiles ? b0 + kScanThreads : ntiles) - 1);
+    if (threadIdx.x == 0) {
+        s_min = 0x7f800000u;
+        s_cnt = 0u;
+    }
+    __syncthreads();
     const bool on = t < ntiles && flags[t] != 0u;
-    const unsigned long long m = __ballot(on);
-    if ((threadIdx.x & 63) == 0 && m) atomicAdd(nflagged, (unsigned)__popcll(m));
-    if (!on) return;
-    atomicMin(&wmin[find_inst(inst, ninst, t)], pend[t]);
+    unsigned long long m = __ballot(on);
+    if (lane == 0 && m) atomicAdd(&s_cnt, (unsigned)__popcll(m));
+    const int i = on ? (ilo == ihi ? ilo : find_inst(inst, ninst, t)) : -1;
+    const unsigned v = on ? pend[t] : 0x7f800000u;
+    if (ilo == ihi) {
+        unsigned r = v;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) r = min(r, (unsigned)__shfl_xor((int)r, o));
+        if (lane == 0 && m) atomicMin(&s_min, r);
+    } else {
+        while (m) {
+            const int leader = __ffsll((long long)m) - 1;
+            const int li = __shfl(i, leader);
+            unsigned r = (i == li) ? v : 0x7f800000u;
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) r = min(r, (unsigned)__shfl_xor((int)r, o));
+            if (lane == leader) atomicMin(&wmin[li], r);
+            m &= ~__ballot(i == li);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (s_cnt) atomicAdd(nflagged, s_cnt);
+        if (ilo == ihi && s_min != 0x7f800000u) atomicMin(&wmin[ilo], s_min);
+    }
 }
 
 // Pass 2: list the active tiles whose pending value is within the instance's band of its lowest,
 // with their pending value and face bits, and clear them; the others wait for a later round.
-__global__ void k_band_select(unsigned* __restrict__ flags, unsigned* __restrict__ pend, int64_t ntiles,
-                              const InstDev* __restrict__ inst, int ninst, const unsigned* __restrict__ wmin,
-                              int64_t* __restrict__ list, uint2* __restrict__ info, unsigned* __restrict__ count) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ntiles || !flags[t]) return;
-    const int i = find_inst(inst, ninst, t);
-    const float lim = __uint_as_float(wmin[i]) + (float)inst[i].band;
-    if (__uint_as_float(pend[t]) <= lim) {
-        const unsigned pos = atomicAdd(count, 1u);
+// One list reservation per block.
+__global__ void __launch_bounds__(kScanThreads) k_band_select(unsigned* __restrict__ flags, unsigned* __restrict__ pend,
+                                                            int64_t ntiles, const InstDev* __restrict__ inst,
+                                                            int ninst, const unsigned* __restrict__ wmin,
+                                                            int64_t* __restrict__ list, uint2* __restrict__ info,
+                                                            unsigned* __restrict__ count) {
+    __shared__ unsigned s_wc[kScanThreads / 64];
+    __shared__ unsigned s_base;
+    const int64_t t = (int64_t)blockIdx.x * kScanThreads + threadIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const unsigned f = t < ntiles ? flags[t] : 0u;
+    bool sel = false;
+    unsigned pv = 0x7f800000u;
+    if (f) {
+        const int i = find_inst(inst, ninst, t);
+        const float lim = __uint_as_float(wmin[i]) + (float)inst[i].band;
+        pv = pend[t];
+        sel = __uint_as_float(pv) <= lim;
+    }
+    const unsigned long long m = __ballot(sel);
+    if (lane == 0) s_wc[wave] = (unsigned)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned tot = 0u;
+        for (int w = 0; w < kScanThreads / 64; w++) {
+            const unsigned c = s_wc[w];
+            s_wc[w] = tot;
+            tot += c;
+        }
+        s_base = tot ? atomicAdd(count, tot) : 0u;
+    }
+    __syncthreads();
+    if (sel) {
+        const unsigned pos = s_base + s_wc[wave] + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
         list[pos] = t;
-        info[pos] = make_uint2(pend[t], flags[t]);
+        info[pos] = make_uint2(pv, f);
         flags[t] = 0u;
         pend[t] = 0x7f800000u;
     }
@@ -678,12 +752,15 @@ static int fim_run_t(eik_ctx* ctx, const InstDev* d_inst, int ninst, int64_t nti
     unsigned* h_cnt = (unsigned*)ctx->h_small;
     unsigned long long* d_c64 = (unsigned long long*)(d_cnt + kCnt64);
     hipStream_t st = ctx->stream;
-    static int grid = 0;  // resident blocks of this instantiation on the device
+    static int grid = 0;  // resident blocks of this instantiation on the device (EIK_GRID overrides)
     if (grid == 0) {
         int ncu = 0, occ = 0;
         CK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device));
         CK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_fim_round<ND, T>, kTileThreads, 0));
         grid = std::max(1, ncu * std::max(1, occ));
+        if (const char* e = getenv("EIK_GRID")) grid = std::max(1, atoi(e));
+        if (getenv("EIK_VERBOSE")) fprintf(stderr, "libeik: k_fim_round<%d,%s> %d CUs x %d blocks -> grid %d\n", ND,
+                                           sizeof(T) == 8 ? "double" : "float", ncu, occ, grid);
     }
     const int g = (int)std::min<int64_t>(grid, ntiles);
     FimStats stt{};
@@ -693,9 +770,9 @@ static int fim_run_t(eik_ctx* ctx, const InstDev* d_inst, int ninst, int64_t nti
         if (batch >= maxbatches) return ctx->fail(EIK_ENOCONV, "traveltime solve did not converge");
         for (int r = 0; r < kRoundBatch; r++) {
             k_round_reset<<<1, 256, 0, st>>>((unsigned*)wm.p, ninst, d_cnt);
-            k_band_min<<<nblocks(ntiles, 256), 256, 0, st>>>(flags, pend, ntiles, d_inst, ninst, (unsigned*)wm.p,
-                                                            d_cnt + kCntFlagged);
-            k_band_select<<<nblocks(ntiles, 256), 256, 0, st>>>(flags, pend, ntiles, d_inst, ninst,
+            k_band_min<<<nblocks(ntiles, kScanThreads), kScanThreads, 0, st>>>(flags, pend, ntiles, d_inst, ninst,
+                                                                              (unsigned*)wm.p, d_cnt + kCntFlagged);
+            k_band_select<<<nblocks(ntiles, kScanThreads), kScanThreads, 0, st>>>(flags, pend, ntiles, d_inst, ninst,
                                                                (const unsigned*)wm.p, (int64_t*)list.p,
                                                                (uint2*)linfo.p, d_cnt + kCntList);
             CK(hipEventRecord(ctx->kev[2 * r], st));
@@ -706,8 +783,8 @@ static int fim_run_t(eik_ctx* ctx, const InstDev* d_inst, int ninst, int64_t nti
             CK(hipEventRecord(ctx->kev[2 * r + 1], st));
         }
         k_round_reset<<<1, 256, 0, st>>>((unsigned*)wm.p, ninst, d_cnt);
-        k_band_min<<<nblocks(ntiles, 256), 256, 0, st>>>(flags, pend, ntiles, d_inst, ninst, (unsigned*)wm.p,
-                                                        d_cnt + kCntFlagged);
+        k_band_min<<<nblocks(ntiles, kScanThreads), kScanThreads, 0, st>>>(flags, pend, ntiles, d_inst, ninst,
+                                                                          (unsigned*)wm.p, d_cnt + kCntFlagged);
         CK(hipMemcpyAsync(h_cnt, d_cnt, kSmallBytes, hipMemcpyDeviceToHost, st));
         CK(hipStreamSynchronize(st));
         for (int r = 0; r < kRoundBatch; r++) {
@@ -738,21 +815,20 @@ int fim_run(eik_ctx* ctx, int nd, int prec, const InstDev* d_inst, int ninst, in
 }
 
 // ----------------------------------------------------------------- forward pass of one call
-// Band width of the tile scheduler, in tile crossing times (tile edge x spacing x mean slowness).
-// A round runs only the active tiles whose pending traveltime lies within the band of the lowest
-// pending one, so tiles far ahead of the front are not relaxed against values that will still
-// change.  EIK_BAND overrides the multiplier (0 or less: no band, every active tile runs).
+// Band width of the tile scheduler, in node crossing times (spacing x mean slowness).  A round
+// runs only the active tiles whose pending traveltime lies within the band of the lowest pending
+// one, so tiles far ahead of the front are not relaxed against values that will still change.
+// EIK_BAND overrides the multiplier (0 or less: no band, every active tile runs).
 static double band_mult() {
     static const double m = [] {
         const char* e = getenv("EIK_BAND");
-        return e ? atof(e) : 0.5;
+        return e ? atof(e) : 4.0;
     }();
     return m;
 }
 
 // EIK_NODEBAND: the same for the nodes inside a tile (default 0: a listed tile relaxes all of
-// its nodes; measured faster on the 201x201x101 workload, where cut tiles were queued again too
-// often).
+// its nodes).
 static double nodeband_mult() {
     static const double m = [] {
         const char* e = getenv("EIK_NODEBAND");
@@ -761,11 +837,10 @@ static double nodeband_mult() {
     return m;
 }
 
-static void set_band(InstDev& I, int nd, double smean) {
+static void set_band(InstDev& I, double smean) {
     const double m = band_mult(), mn = nodeband_mult();
-    const int edge = nd == 2 ? 16 : 8;
-    I.band = m > 0.0 ? m * edge * I.h[0] * smean : INFINITY;
-    I.nband = mn > 0.0 ? mn * edge * I.h[0] * smean : INFINITY;
+    I.band = m > 0.0 ? m * I.h[0] * smean : INFINITY;
+    I.nband = mn > 0.0 ? mn * I.h[0] * smean : INFINITY;
 }
 
 int forward_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, int prec, const eik_params* P) {
@@ -778,18 +853,18 @@ int forward_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, int prec, cons
 
     // coarse instances: one grid per source, sharing the model's slowness
     std::vector<InstDev> cinst(nsrc);
-    const int TX = nd == 2 ? 16 : 8, TZ = nd == 2 ? 1 : 8;
-    int tiles[3] = {(int)((M->g.n[0] + TX - 1) / TX), (int)((M->g.n[1] + TX - 1) / TX),
-                    nd == 3 ? (int)((M->g.n[2] + TZ - 1) / TZ) : 1};
-    const int64_t tiles_per = (int64_t)tiles[0] * tiles[1] * tiles[2];
+    int64_t tiles_per = 0, atiles_per = 0;
     for (int64_t s = 0; s < nsrc; s++) {
         InstDev& I = cinst[s];
         make_inst(M->g, I);
-        for (int d = 0; d < 3; d++) I.t[d] = tiles[d];
+        set_tiles(I, nd);
+        tiles_per = (int64_t)I.t[0] * I.t[1] * I.t[2];
+        atiles_per = (int64_t)I.at[0] * I.at[1] * I.at[2];
         I.tt_off = s * N;
         I.s_off = 0;
         I.tile_base = s * tiles_per;
-        set_band(I, nd, M->smean);
+        I.atile_base = s * atiles_per;
+        set_band(I, M->smean);
     }
     const int64_t ntiles_c = nsrc * tiles_per;
     const size_t esz = prec == EIK_PREC_FP64 ? sizeof(double) : sizeof(float);
@@ -806,10 +881,10 @@ int forward_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, int prec, cons
     if (dci.ensure(cinst.size() * sizeof(InstDev))) return ctx->fail(EIK_ENOMEM, "out of device memory");
     CK(hipMemcpyAsync(dci.p, cinst.data(), cinst.size() * sizeof(InstDev), hipMemcpyHostToDevice, st));
 
-    int64_t maxt = ntiles_c;
+    int64_t maxt = ntiles_c, maxat = nsrc * atiles_per;
     std::vector<FineDev> fine;
     std::vector<InstDev> finst;
-    int64_t ntiles_f = 0, nfine_pts = 0;
+    int64_t ntiles_f = 0, antiles_f = 0, nfine_pts = 0;
     if (P->refinearoundsrc) {
         fine.resize(nsrc);
         finst.resize(nsrc);
@@ -819,15 +894,18 @@ int forward_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, int prec, cons
             finst[s].tt_off = nfine_pts;
             finst[s].s_off = nfine_pts;
             finst[s].tile_base = ntiles_f;
-            set_band(finst[s], nd, M->smean);
+            finst[s].atile_base = antiles_f;
+            set_band(finst[s], M->smean);
             nfine_pts += (int64_t)finst[s].n[0] * finst[s].n[1] * finst[s].n[2];
             ntiles_f += (int64_t)finst[s].t[0] * finst[s].t[1] * finst[s].t[2];
+            antiles_f += (int64_t)finst[s].at[0] * finst[s].at[1] * finst[s].at[2];
         }
         maxt = std::max(maxt, ntiles_f);
+        maxat = std::max(maxat, antiles_f);
     }
-    DevBuf& aflags = ctx->buf("fim_flags");  // adjoint rounds
-    if (aflags.ensure((size_t)maxt)) return ctx->fail(EIK_ENOMEM, "out of device memory (tile flags)");
-    CK(hipMemsetAsync(aflags.p, 0, (size_t)maxt, st));
+    DevBuf& aflags = ctx->buf("fim_flags");  // adjoint rounds, one byte per adjoint tile
+    if (aflags.ensure((size_t)maxat)) return ctx->fail(EIK_ENOMEM, "out of device memory (tile flags)");
+    CK(hipMemsetAsync(aflags.p, 0, (size_t)maxat, st));
     DevBuf& flags = ctx->buf("fim_tflags");
     if (flags.ensure((size_t)maxt * sizeof(unsigned))) return ctx->fail(EIK_ENOMEM, "out of device memory (tile flags)");
     CK(hipMemsetAsync(flags.p, 0, (size_t)maxt * sizeof(unsigned), st));

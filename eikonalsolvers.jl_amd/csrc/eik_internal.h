@@ -21,10 +21,13 @@ namespace eik {
 // One grid solved by the tiled solver: a source's coarse grid or its refined near-source box.
 struct InstDev {
     int32_t n[3];        // grid nodes per axis (n[2] = 1 in 2D)
-    int32_t t[3];        // tiles per axis
+    int32_t t[3];        // forward-solver tiles per axis
     int64_t tt_off;      // element offset of this grid in its traveltime pool
     int64_t s_off;       // element offset of its slowness in the slowness pool
-    int64_t tile_base;   // first global tile id (tile flags / lists)
+    int64_t tile_base;   // first global forward tile id (tile flags / lists)
+    int32_t at[3];       // adjoint tiles per axis
+    int32_t pad3;
+    int64_t atile_base;  // first global adjoint tile id
     double h[3];         // node spacing per axis (Cartesian hgrid; spherical dr, dtheta[deg], dphi[deg])
     double o[3];         // origin (cooinit)
     double dthrad;       // spherical: dtheta in radians
@@ -109,23 +112,28 @@ __host__ __device__ __forceinline__ double distance2points(int kind, int nd, con
     return sqrt(s);
 }
 
-// mark the tile of node g and the face-neighbour tiles whose stencil reaches it (adjoint rounds:
-// one byte per tile)
+// Tile edges: the forward solver relaxes one-wave tiles (8x8 in 2D, 4x4x4 in 3D); the adjoint
+// solve uses 16x16 / 8x8x8 tiles.
+__host__ __device__ constexpr int fwd_tile_edge(int nd) { return nd == 2 ? 8 : 4; }
+__host__ __device__ constexpr int adj_tile_edge(int nd) { return nd == 2 ? 16 : 8; }
+
+// mark the adjoint tile of node g and the face-neighbour tiles whose stencil reaches it (one
+// byte per tile)
 __device__ __forceinline__ void seed_tiles(const InstDev& I, int nd, const int* g, uint8_t* flags) {
-    const int TX = nd == 2 ? 16 : 8, TY = TX, TZ = nd == 2 ? 1 : 8;
+    const int TX = adj_tile_edge(nd), TY = TX, TZ = nd == 2 ? 1 : TX;
     const int tx = g[0] / TX, ty = g[1] / TY, tz = nd == 3 ? g[2] / TZ : 0;
-    const int64_t t0 = I.t[0], t01 = (int64_t)I.t[0] * I.t[1];
+    const int64_t t0 = I.at[0], t01 = (int64_t)I.at[0] * I.at[1];
     const int64_t tl = tx + t0 * ty + t01 * tz;
-    uint8_t* fl = flags + I.tile_base;
+    uint8_t* fl = flags + I.atile_base;
     fl[tl] = 1;
     const int lx = g[0] - tx * TX, ly = g[1] - ty * TY, lz = nd == 3 ? g[2] - tz * TZ : 0;
     if (lx < 2 && tx > 0) fl[tl - 1] = 1;
-    if (lx >= TX - 2 && tx < I.t[0] - 1) fl[tl + 1] = 1;
+    if (lx >= TX - 2 && tx < I.at[0] - 1) fl[tl + 1] = 1;
     if (ly < 2 && ty > 0) fl[tl - t0] = 1;
-    if (ly >= TY - 2 && ty < I.t[1] - 1) fl[tl + t0] = 1;
+    if (ly >= TY - 2 && ty < I.at[1] - 1) fl[tl + t0] = 1;
     if (nd == 3) {
         if (lz < 2 && tz > 0) fl[tl - t01] = 1;
-        if (lz >= TZ - 2 && tz < I.t[2] - 1) fl[tl + t01] = 1;
+        if (lz >= TZ - 2 && tz < I.at[2] - 1) fl[tl + t01] = 1;
     }
 }
 
@@ -137,11 +145,11 @@ constexpr unsigned kFullBit = 0x40u;
 // pending value and that has a reached neighbour.
 constexpr unsigned kKeyBit = 0x80u;
 
-// Seed node g with value val: its tile and the face neighbours whose stencil reaches it relax
-// every node in their next round.
+// Seed node g with value val: its forward tile and the face neighbours whose stencil reaches it
+// relax every node in their next round.
 __device__ __forceinline__ void seed_tiles_fwd(const InstDev& I, int nd, const int* g, unsigned* flags,
                                                unsigned* pend, double val) {
-    const int TX = nd == 2 ? 16 : 8, TY = TX, TZ = nd == 2 ? 1 : 8;
+    const int TX = fwd_tile_edge(nd), TY = TX, TZ = nd == 2 ? 1 : TX;
     const int tx = g[0] / TX, ty = g[1] / TY, tz = nd == 3 ? g[2] / TZ : 0;
     const int64_t t0 = I.t[0], t01 = (int64_t)I.t[0] * I.t[1];
     const int64_t tl = tx + t0 * ty + t01 * tz;
@@ -363,6 +371,7 @@ struct Timer {
 
 // host geometry (eik_geom.cpp)
 void make_inst(const eik_grid& g, InstDev& I);
+void set_tiles(InstDev& I, int nd);  // forward and adjoint tile counts from I.n
 int find_enclosing_box(const eik_grid& g, const double* src, int corn[8][3]);
 void build_fine(const eik_grid& g, const double* src, int df, int noderadius, FineDev& F, InstDev& I);
 

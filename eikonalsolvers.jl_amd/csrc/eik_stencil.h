@@ -101,50 +101,60 @@ __device__ __forceinline__ AxisPick<T> pick_axis(const T* v1, const T* v2, unsig
 // ---------------------------------------------------------------- FP64 (reference arithmetic)
 // calcttpt_2ndord! with the reference's operation order: the mixed-order quadratic, its
 // all-1st-order fallback when the discriminant is negative, then the 1-D fallback t1 + h s.
-// Both quadratics are accumulated in one pass and one square root and one division are taken
-// on the one selected; max(t1, t2) is always t1 here (alpha > 0, rounding is monotone).
+// The fallbacks are rare, so they are computed only when a lane of the wave needs them; one
+// square root and one division are taken on the selected quadratic; max(t1, t2) is always t1
+// here (alpha > 0, rounding is monotone).
 template <int ND>
 __device__ __forceinline__ double eval64(const double* v1, const double* v2, unsigned acc1, unsigned acc2,
                                          double s, const Metric<double>& m, int* code) {
 #pragma clang fp contract(off)
     double alpha = 0.0, beta = 0.0, gamma = -(s * s);
-    double alpha1 = 0.0, beta1 = 0.0, gamma1 = -(s * s);
-    double cmin = EIK_INF, hmin = m.dh[0];
-    int idc = 0;
-    int cA[3] = {0, 0, 0}, cB[3] = {0, 0, 0};
+    AxisPick<double> p[3];
 #pragma unroll
     for (int ax = 0; ax < ND; ax++) {
-        const AxisPick<double> p = pick_axis<double>(v1, v2, acc1, acc2, ax);
-        cA[ax] = p.codeA;
-        cB[ax] = p.codeB;
-        const bool sec = p.use1 && p.use2;
-        const double tmpa2 = 1.0 / 3.0 * (4.0 * p.cv1 - p.cv2);
+        p[ax] = pick_axis<double>(v1, v2, acc1, acc2, ax);
+        const bool sec = p[ax].use1 && p[ax].use2;
+        const double tmpa2 = 1.0 / 3.0 * (4.0 * p[ax].cv1 - p[ax].cv2);
         const double ca = sec ? m.c2[ax] : m.c1[ax];
-        const double tv = sec ? tmpa2 : p.cv1;
-        alpha = p.use1 ? alpha + ca : alpha;
-        beta = p.use1 ? beta + (-2.0 * ca * tv) : beta;
-        gamma = p.use1 ? gamma + ca * (tv * tv) : gamma;
-        const double cb = m.c1[ax];
-        alpha1 = p.use1 ? alpha1 + cb : alpha1;
-        beta1 = p.use1 ? beta1 + (-2.0 * cb * p.cv1) : beta1;
-        gamma1 = p.use1 ? gamma1 + cb * (p.cv1 * p.cv1) : gamma1;
-        // 1-D fallback: first axis with the smallest adjacent value
-        const bool better = (ax == 0) || p.cv1 < cmin;
-        idc = better ? ax : idc;
-        cmin = better ? p.cv1 : cmin;
-        hmin = better ? m.dh[ax] : hmin;
+        const double tv = sec ? tmpa2 : p[ax].cv1;
+        alpha = p[ax].use1 ? alpha + ca : alpha;
+        beta = p[ax].use1 ? beta + (-2.0 * ca * tv) : beta;
+        gamma = p[ax].use1 ? gamma + ca * (tv * tv) : gamma;
+        code[ax] = p[ax].codeA;
     }
     const double sqA = beta * beta - 4.0 * alpha * gamma;
-    const double sqB = beta1 * beta1 - 4.0 * alpha1 * gamma1;
-    const bool useA = !(sqA < 0.0);
-    const bool useB = !useA && !(sqB < 0.0);
-    const double a = useA ? alpha : alpha1;
-    const double b = useA ? beta : beta1;
-    const double q = useA ? sqA : sqB;
+    double a = alpha, b = beta, q = sqA, oned = 0.0;
+    bool useoned = false;
+    if (__ballot(sqA < 0.0) != 0ull) {
+        double alpha1 = 0.0, beta1 = 0.0, gamma1 = -(s * s);
+        double cmin = EIK_INF, hmin = m.dh[0];
+        int idc = 0;
 #pragma unroll
-    for (int d = 0; d < ND; d++) code[d] = useA ? cA[d] : (useB ? cB[d] : (d == idc ? cB[d] : 0));
-    if (useA || useB) return (-b + sqrt(q)) / (2.0 * a);
-    return (hmin * s) + cmin;
+        for (int ax = 0; ax < ND; ax++) {
+            const double cb = m.c1[ax];
+            alpha1 = p[ax].use1 ? alpha1 + cb : alpha1;
+            beta1 = p[ax].use1 ? beta1 + (-2.0 * cb * p[ax].cv1) : beta1;
+            gamma1 = p[ax].use1 ? gamma1 + cb * (p[ax].cv1 * p[ax].cv1) : gamma1;
+            // 1-D fallback: first axis with the smallest adjacent value
+            const bool better = (ax == 0) || p[ax].cv1 < cmin;
+            idc = better ? ax : idc;
+            cmin = better ? p[ax].cv1 : cmin;
+            hmin = better ? m.dh[ax] : hmin;
+        }
+        const double sqB = beta1 * beta1 - 4.0 * alpha1 * gamma1;
+        const bool useA = !(sqA < 0.0);
+        const bool useB = !useA && !(sqB < 0.0);
+        a = useA ? alpha : alpha1;
+        b = useA ? beta : beta1;
+        q = useA ? sqA : sqB;
+        useoned = !useA && !useB;
+        oned = (hmin * s) + cmin;
+#pragma unroll
+        for (int d = 0; d < ND; d++)
+            code[d] = useA ? p[d].codeA : (useB ? p[d].codeB : (d == idc ? p[d].codeB : 0));
+    }
+    const double t = (-b + sqrt(q)) / (2.0 * a);
+    return useoned ? oned : t;
 }
 
 // ---------------------------------------------------------------- FP32 (shifted quadratic)
@@ -163,9 +173,6 @@ __device__ __forceinline__ float eval32(const float* v1, const float* v2, unsign
         r = p[ax].use1 ? fminf(r, p[ax].cv1) : r;
     }
     float A = 0.f, B = 0.f, C = -(s * s);
-    float A1 = 0.f, B1 = 0.f, C1 = -(s * s);
-    float cmin = EIK_INFF, hmin = m.dh[0];
-    int idc = 0;
 #pragma unroll
     for (int ax = 0; ax < ND; ax++) {
         const bool sec = p[ax].use1 && p[ax].use2;
@@ -175,27 +182,41 @@ __device__ __forceinline__ float eval32(const float* v1, const float* v2, unsign
         A = p[ax].use1 ? A + al : A;
         B = p[ax].use1 ? B + al * d : B;
         C = p[ax].use1 ? C + al * d * d : C;
-        const float ih2 = m.c1[ax];
-        A1 = p[ax].use1 ? A1 + ih2 : A1;
-        B1 = p[ax].use1 ? B1 + ih2 * d1 : B1;
-        C1 = p[ax].use1 ? C1 + ih2 * d1 * d1 : C1;
-        const bool better = (ax == 0) || p[ax].cv1 < cmin;
-        idc = better ? ax : idc;
-        cmin = better ? p[ax].cv1 : cmin;
-        hmin = better ? m.dh[ax] : hmin;
+        code[ax] = p[ax].codeA;
     }
     const float discA = B * B - A * C;
-    const float discB = B1 * B1 - A1 * C1;
-    const bool useA = !(discA < 0.f);
-    const bool useB = !useA && !(discB < 0.f);
+    float a = A, b = B, q = discA, oned = 0.f;
+    bool useoned = false;
+    if (__ballot(discA < 0.f) != 0ull) {
+        float A1 = 0.f, B1 = 0.f, C1 = -(s * s);
+        float cmin = EIK_INFF, hmin = m.dh[0];
+        int idc = 0;
 #pragma unroll
-    for (int d = 0; d < ND; d++)
-        code[d] = useA ? p[d].codeA : (useB ? p[d].codeB : (d == idc ? p[d].codeB : 0));
-    const float a = useA ? A : A1;
-    const float b = useA ? B : B1;
-    const float q = useA ? discA : discB;
-    if (useA || useB) return r + (b + sqrtf(q)) / a;
-    return cmin + hmin * s;
+        for (int ax = 0; ax < ND; ax++) {
+            const float ih2 = m.c1[ax];
+            const float d1 = p[ax].cv1 - r;
+            A1 = p[ax].use1 ? A1 + ih2 : A1;
+            B1 = p[ax].use1 ? B1 + ih2 * d1 : B1;
+            C1 = p[ax].use1 ? C1 + ih2 * d1 * d1 : C1;
+            const bool better = (ax == 0) || p[ax].cv1 < cmin;
+            idc = better ? ax : idc;
+            cmin = better ? p[ax].cv1 : cmin;
+            hmin = better ? m.dh[ax] : hmin;
+        }
+        const float discB = B1 * B1 - A1 * C1;
+        const bool useA = !(discA < 0.f);
+        const bool useB = !useA && !(discB < 0.f);
+        a = useA ? A : A1;
+        b = useA ? B : B1;
+        q = useA ? discA : discB;
+        useoned = !useA && !useB;
+        oned = cmin + hmin * s;
+#pragma unroll
+        for (int d = 0; d < ND; d++)
+            code[d] = useA ? p[d].codeA : (useB ? p[d].codeB : (d == idc ? p[d].codeB : 0));
+    }
+    const float t = r + (b + sqrtf(q)) / a;
+    return useoned ? oned : t;
 }
 
 // ---------------------------------------------------------------- heap pop keys
@@ -238,19 +259,74 @@ __device__ __forceinline__ float eval_any(const float* v1, const float* v2, unsi
 // Local replay of the heap (see the file comment).  v1/v2: adjacent / 2-away values; k1/k2 their
 // keys; f1/f2 fixed (init) bits; ok1/ok2 grid-boundary bits.  Returns the value and sets *kout to
 // the node's pop key and code to the derivative codes of the last stencil evaluation.
+//
+// Outside cascades every key is (value, -inf, 0) and the order is the value order, so when no
+// neighbour of any lane in the wave carries a cascade key the replay compares values only
+// (replay_plain); otherwise the full key comparison runs (replay_keys).  Both give the same
+// result on such neighbourhoods.
 template <int ND, typename T>
-__device__ __forceinline__ T replay(const T* v1, const T* v2, const Key<T>* k1, const Key<T>* k2, unsigned f1,
-                                    unsigned f2, unsigned ok1, unsigned ok2, T s, const Metric<T>& m, int* code,
-                                    Key<T>* kout) {
+__device__ __forceinline__ T replay_plain(const T* v1, const T* v2, unsigned cand, bool haveinit, unsigned f1,
+                                          unsigned f2, unsigned ok1, unsigned ok2, T s, const Metric<T>& m,
+                                          int* code, Key<T>* kout) {
+    T cur = tinf<T>();
+    T kzK = tinf<T>(), kzS = -tinf<T>();
+    int kzD = 0;
+#pragma unroll
+    for (int d = 0; d < ND; d++) code[d] = 0;
+    if (haveinit) {
+        cur = eval_any<ND>(v1, v2, f1 & ok1, f2 & ok2, s, m, code);
+        kzK = cur;
+    }
+    // distinct candidate values in increasing order; after a step that ends below its trigger
+    // (cascade key), the next candidate never pops first
+    T prev = -tinf<T>();
+    unsigned acc1 = f1 & ok1;
+#pragma unroll 1
+    for (int step = 0; step < 2 * ND && cand; step++) {
+        T c = tinf<T>();
+#pragma unroll
+        for (int k = 0; k < 2 * ND; k++) c = (((cand >> k) & 1u) && v1[k] > prev && v1[k] < c) ? v1[k] : c;
+        if (!(c < tinf<T>()) || kzD != 0 || !(c < kzK)) break;
+        prev = c;
+        unsigned acc2 = 0u;
+#pragma unroll
+        for (int k = 0; k < 2 * ND; k++) {
+            acc1 |= (unsigned)(((cand >> k) & 1u) && v1[k] == c) << k;
+            const bool a2 = (bool)((ok2 >> k) & 1u) & ((bool)((f2 >> k) & 1u) | (v2[k] <= c));
+            acc2 |= (unsigned)a2 << k;
+        }
+        cur = eval_any<ND>(v1, v2, acc1, acc2, s, m, code);
+        const bool ord = cur >= c;
+        kzK = ord ? cur : c;
+        kzS = ord ? -tinf<T>() : cur;
+        kzD = ord ? 0 : 1;
+    }
+    *kout = Key<T>{kzK, kzS, kzD};
+    return cur;
+}
+
+// keys(k, two) gives the pop key of slot k's adjacent (two = false) or 2-away (two = true)
+// node; it is called only on the rare cascade path, so callers can read keys from memory there
+// instead of holding them in registers.  d1/d2: slots whose adjacent / 2-away node has a cascade
+// key (D != 0); dcap: adjacent nodes at the cascade depth cap.
+template <int ND, typename T, class KeyAt>
+__device__ __forceinline__ T replay(const T* v1, const T* v2, KeyAt keys, unsigned d1, unsigned d2, unsigned dcap,
+                                    unsigned f1, unsigned f2, unsigned ok1, unsigned ok2, T s, const Metric<T>& m,
+                                    int* code, Key<T>* kout) {
     // candidates: adjacent nodes inside the grid, not init, reached, cascade depth below the cap
     unsigned cand = 0u;
     bool haveinit = false;
+    bool keyed = false;  // a neighbour in use carries a cascade key
 #pragma unroll
     for (int k = 0; k < 2 * ND; k++) {
-        const bool in = (ok1 >> k) & 1u, fx = (f1 >> k) & 1u;
+        const bool in = (ok1 >> k) & 1u, fx = (f1 >> k) & 1u, cp = (dcap >> k) & 1u;
         haveinit |= in & fx;
-        cand |= (unsigned)(in & !fx & (v1[k] < tinf<T>()) & (k1[k].D < kMaxCascade)) << k;
+        const bool reached1 = in & !fx & (v1[k] < tinf<T>());
+        cand |= (unsigned)(reached1 & !cp) << k;
+        const bool reached2 = (bool)((ok2 >> k) & 1u) & !((f2 >> k) & 1u) & (v2[k] < tinf<T>());
+        keyed |= (reached1 & (bool)((d1 >> k) & 1u)) | (reached2 & (bool)((d2 >> k) & 1u));
     }
+    if (__ballot(keyed) == 0ull) return replay_plain<ND, T>(v1, v2, cand, haveinit, f1, f2, ok1, ok2, s, m, code, kout);
     T cur = tinf<T>();
     Key<T> kz{tinf<T>(), -tinf<T>(), 0};
 #pragma unroll
@@ -260,17 +336,19 @@ __device__ __forceinline__ T replay(const T* v1, const T* v2, const Key<T>* k1, 
         kz = Key<T>{cur, -tinf<T>(), 0};
     }
     // distinct candidate keys in increasing order (equal keys pop as one batch); each step scans
-    // the 2*ND slots with static indices so everything stays in registers
+    // the 2*ND slots with static indices
     Key<T> prev{-tinf<T>(), -tinf<T>(), -1};
+#pragma unroll 1
     for (int step = 0; step < 2 * ND && cand; step++) {
         Key<T> c{tinf<T>(), tinf<T>(), 1 << 30};
         bool found = false;
 #pragma unroll
         for (int k = 0; k < 2 * ND; k++) {
-            const bool take = (bool)((cand >> k) & 1u) & klt(prev, k1[k]) & (!found | klt(k1[k], c));
-            c.K = take ? k1[k].K : c.K;
-            c.S = take ? k1[k].S : c.S;
-            c.D = take ? k1[k].D : c.D;
+            const Key<T> kk = keys(k, false);
+            const bool take = (bool)((cand >> k) & 1u) & klt(prev, kk) & (!found | klt(kk, c));
+            c.K = take ? kk.K : c.K;
+            c.S = take ? kk.S : c.S;
+            c.D = take ? kk.D : c.D;
             found |= take;
         }
         if (!found || !klt(c, kz)) break;
@@ -278,8 +356,10 @@ __device__ __forceinline__ T replay(const T* v1, const T* v2, const Key<T>* k1, 
         unsigned acc1 = 0u, acc2 = 0u;
 #pragma unroll
         for (int k = 0; k < 2 * ND; k++) {
-            const bool a1 = (bool)((ok1 >> k) & 1u) & ((bool)((f1 >> k) & 1u) | ((v1[k] < tinf<T>()) & !klt(c, k1[k])));
-            const bool a2 = (bool)((ok2 >> k) & 1u) & ((bool)((f2 >> k) & 1u) | ((v2[k] < tinf<T>()) & !klt(c, k2[k])));
+            const bool a1 = (bool)((ok1 >> k) & 1u) &
+                            ((bool)((f1 >> k) & 1u) | ((v1[k] < tinf<T>()) & !klt(c, keys(k, false))));
+            const bool a2 = (bool)((ok2 >> k) & 1u) &
+                            ((bool)((f2 >> k) & 1u) | ((v2[k] < tinf<T>()) & !klt(c, keys(k, true))));
             acc1 |= (unsigned)a1 << k;
             acc2 |= (unsigned)a2 << k;
         }
@@ -291,6 +371,22 @@ __device__ __forceinline__ T replay(const T* v1, const T* v2, const Key<T>* k1, 
     }
     *kout = kz;
     return cur;
+}
+
+// Same with the keys held in arrays (k1: adjacent, k2: 2-away).
+template <int ND, typename T>
+__device__ __forceinline__ T replay(const T* v1, const T* v2, const Key<T>* k1, const Key<T>* k2, unsigned f1,
+                                    unsigned f2, unsigned ok1, unsigned ok2, T s, const Metric<T>& m, int* code,
+                                    Key<T>* kout) {
+    unsigned d1 = 0u, d2 = 0u, dcap = 0u;
+#pragma unroll
+    for (int k = 0; k < 2 * ND; k++) {
+        d1 |= (unsigned)(k1[k].D != 0) << k;
+        d2 |= (unsigned)(k2[k].D != 0) << k;
+        dcap |= (unsigned)(k1[k].D >= kMaxCascade) << k;
+    }
+    auto keys = [&](int k, bool two) { return two ? k2[k] : k1[k]; };
+    return replay<ND, T>(v1, v2, keys, d1, d2, dcap, f1, f2, ok1, ok2, s, m, code, kout);
 }
 
 }  // namespace eik
