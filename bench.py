@@ -26,7 +26,7 @@ import numpy as np  # noqa: E402
 
 METRIC = "source-solves/s + gradient evals/s, 3D 201²×101 grid; serving p50/p99 latency"
 HBM_PEAK_GBS = 8000.0
-TILE_NODES_3D = 512
+TILE_NODES_3D = 64  # forward tiles are 4x4x4 (one wavefront each)
 BYTES_PER_VISIT = {"fp64": 24, "mixed": 12}  # slowness + tt read + tt write per node-visit
 
 
@@ -43,6 +43,55 @@ def run_steps(model, src, recs, ep, steps, warmup):
             tot[k] += st[k]
     dt = time.perf_counter() - t0
     return dt, tot
+
+
+def serving_leg(n_req: int, nslots: int) -> dict:
+    """Serving mix (70 % forward / 30 % gradient, 2D 100^2-400^2 and 3D 64^3-160^3, 1-64 sources,
+    16-128 receivers, FP32 stencil / FP64 adjoint) through the continuous-batching server: first
+    all requests at once (capacity), then Poisson arrivals at 50 % and 80 % of that capacity.
+    Latency = completion - arrival; models are resident before the clock starts."""
+    import eikonalsolvers as eik
+    from eikonalsolvers.grids import ExtraParams
+    from eikonalsolvers.synthetic import c5_arrivals, c5_request
+
+    ep = ExtraParams(precision="mixed")
+    reqs = [c5_request(i) for i in range(n_req)]
+    models = [eik.EikModel(r["vel"], r["grd"]) for r in reqs]
+    nsolves = sum(len(r["src"]) for r in reqs)
+
+    def run(arrivals):
+        with eik.Server(nslots=nslots) as srv:
+            t0 = time.perf_counter()
+            handles = []
+            for r, m, ta in zip(reqs, models, arrivals):
+                dt = t0 + ta - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+                if r["kind"] == "traveltime":
+                    handles.append(srv.submit_traveltime(m, r["src"], r["recs"], extraparams=ep))
+                else:
+                    handles.append(srv.submit_gradient(m, r["src"], r["recs"], r["pickobs"], r["stdobs"],
+                                                       "gradvelandsrcloc", extraparams=ep))
+            for h in handles:
+                h.result()
+            makespan = time.perf_counter() - t0
+        lat = np.array([h.latency_ms for h in handles])
+        return {"p50_ms": round(float(np.percentile(lat, 50)), 2), "p99_ms": round(float(np.percentile(lat, 99)), 2),
+                "makespan_s": round(makespan, 3), "source_solves_per_s": round(nsolves / makespan, 2),
+                "requests_per_s": round(n_req / makespan, 3)}
+
+    run(np.zeros(n_req))  # warm-up: pools of every slot sized
+    sat = run(np.zeros(n_req))
+    cap = sat["requests_per_s"]
+    out = {"mode": "f32 stencil / f64 near-source, adjoint and reductions", "slots": nslots, "requests": n_req,
+           "gradient_requests": sum(r["kind"] == "gradient" for r in reqs), "source_solves": nsolves,
+           "saturated": {"offered_rate": "all at t=0", **sat}, "poisson": []}
+    for frac in (0.5, 0.8):
+        res = run(c5_arrivals(n_req, frac * cap))
+        out["poisson"].append({"offered_req_per_s": round(frac * cap, 3), "load": frac, **res})
+    for m in models:
+        m.close()
+    return out
 
 
 def _traffic_from_profile():
@@ -64,6 +113,9 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=0, help="sources timed on the CPU (0 = threads used)")
     ap.add_argument("--legs", default="all", choices=["all", "fp64"],
                     help="fp64: only the FP64 forward leg (for counter-collection runs)")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving (request mix) leg")
+    ap.add_argument("--serving-requests", type=int, default=32)
+    ap.add_argument("--serving-slots", type=int, default=4)
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain device copy of 512 MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -176,6 +228,9 @@ def main() -> int:
     tr = _traffic_from_profile()
     if tr is not None:
         out["roofline"]["traffic"] = tr
+
+    if not args.no_serving:
+        out["serving"] = serving_leg(args.serving_requests, args.serving_slots)
 
     if not args.no_cpu_baseline:
         from oracle import oracle as O

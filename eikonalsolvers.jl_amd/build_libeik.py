@@ -8,7 +8,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "eikonalsolvers", "libeik.so")
-SOURCES = ["eik_capi.hip", "eik_fim.hip", "eik_adj.hip"]
+SOURCES = ["eik_capi.hip", "eik_fim.hip", "eik_adj.hip", "eik_serve.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [
@@ -18,6 +18,8 @@ FLAGS = [
     # FP64 stencils follow the reference's operation order; no implicit FMA contraction
     "-ffp-contract=off",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
+    # the stencil combines comparisons with & and | on purpose (no short-circuit branches)
+    "-Wno-bitwise-instead-of-logical",
 ]
 
 

@@ -297,6 +297,8 @@ struct DevBuf {
 
 // solver rounds enqueued between two host checks for convergence
 constexpr int kRoundBatch = 8;
+// rounds after which a traveltime solve starts settling (see fim_run_t)
+constexpr int kSettleRound = 2000;
 // device counters (ctx->d_small, 32-bit words): list length, list cursor, flagged tiles, sweeps,
 // non-empty rounds; from word kCnt64 on, 64-bit node updates, changed updates, tile activations
 constexpr size_t kSmallBytes = 256;
@@ -392,6 +394,7 @@ struct FimStats {
     double updates = 0;        // node updates (stencil replays) summed over activations
     double changed = 0;        // node updates that changed the node
     double keychanged = 0;     // of those, the ones that changed only the pop key
+    int settled = 0;           // settling stage reached (0: exact; k: tolerance 10^(k-1) x base)
     double kernel_ms = 0;      // summed duration of the round launches
 };
 // traveltime pool plus its pop-key pools (K, S in the pool's precision, D as bytes)

@@ -5,8 +5,10 @@ gradient path; the computation runs in libeik.so (hand-written HIP, include/eik.
 """
 from .grids import (ExtraParams, Grid2DCart, Grid2DSphere, Grid3DCart, Grid3DSphere, GridRefinementPars)
 from .api import (Context, EikModel, default_context, eikgradient, eikttimemisfit, eiktraveltime)
+from .serving import Request, Server
 
 __all__ = [
     "ExtraParams", "GridRefinementPars", "Grid2DCart", "Grid3DCart", "Grid2DSphere", "Grid3DSphere",
     "eiktraveltime", "eikgradient", "eikttimemisfit", "EikModel", "Context", "default_context",
+    "Server", "Request",
 ]

@@ -26,6 +26,7 @@ EXPORTS = (
     "eik_ctx_create", "eik_ctx_destroy", "eik_last_error", "eik_version",
     "eik_model_create", "eik_model_update_vel", "eik_model_destroy",
     "eik_traveltime", "eik_gradient", "eik_misfit", "eik_last_stats",
+    "eik_server_create", "eik_server_destroy", "eik_submit", "eik_wait", "eik_server_stats",
 )
 
 
@@ -39,6 +40,16 @@ class EikParams(ctypes.Structure):
                 ("noderadius", ctypes.c_int32), ("radiussmoothgradsrc", ctypes.c_int32),
                 ("smoothgradkern", ctypes.c_int32), ("precision", ctypes.c_int32),
                 ("reserved", ctypes.c_int32 * 2)]
+
+
+class EikRequest(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int32), ("whichgrad", ctypes.c_int32), ("model", ctypes.c_void_p),
+                ("nsrc", ctypes.c_int64), ("coordsrc", ctypes.POINTER(ctypes.c_double)),
+                ("rec_off", ctypes.POINTER(ctypes.c_int64)), ("coordrec", ctypes.POINTER(ctypes.c_double)),
+                ("pickobs", ctypes.POINTER(ctypes.c_double)), ("stdobs", ctypes.POINTER(ctypes.c_double)),
+                ("params", EikParams), ("ttpicks", ctypes.POINTER(ctypes.c_double)),
+                ("gradvel", ctypes.POINTER(ctypes.c_double)), ("gradsrc", ctypes.POINTER(ctypes.c_double)),
+                ("misf", ctypes.POINTER(ctypes.c_double))]
 
 
 class LibEikMissing(RuntimeError):
@@ -85,15 +96,26 @@ def load():
         L.eik_misfit.restype = ctypes.c_int
         L.eik_last_stats.argtypes = [vp, dp, ctypes.c_int32]
         L.eik_last_stats.restype = ctypes.c_int
+        L.eik_server_create.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, P(vp)]
+        L.eik_server_create.restype = ctypes.c_int
+        L.eik_server_destroy.argtypes = [vp]
+        L.eik_server_destroy.restype = None
+        L.eik_submit.argtypes = [vp, P(EikRequest), ip]
+        L.eik_submit.restype = ctypes.c_int
+        L.eik_wait.argtypes = [vp, ctypes.c_int64, dp, ctypes.c_char_p, ctypes.c_int64]
+        L.eik_wait.restype = ctypes.c_int
+        L.eik_server_stats.argtypes = [vp, ip, ctypes.c_int32]
+        L.eik_server_stats.restype = ctypes.c_int
         _lib = L
         return L
 
 
-def check(rc: int, ctx) -> None:
+def check(rc: int, ctx, msg: str | None = None) -> None:
     """Raise the exception kind the reference raises for the same failure."""
     if rc == EIK_OK:
         return
-    msg = load().eik_last_error(ctx).decode(errors="replace") if ctx else "libeik error"
+    if msg is None:
+        msg = load().eik_last_error(ctx).decode(errors="replace") if ctx else "libeik error"
     if rc == EIK_EASSERT:
         raise AssertionError(msg)
     if rc == EIK_EINVAL:

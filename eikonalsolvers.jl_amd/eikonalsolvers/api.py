@@ -52,7 +52,8 @@ class Context:
         _lib.load().eik_last_stats(self.handle, _lib.ptr(out), 12)
         return {"total_ms": out[0], "fine_ms": out[1], "coarse_ms": out[2], "adjoint_ms": out[3],
                 "tile_activations": out[4], "rounds": out[5], "_kernel_ms": out[6], "tile_sweeps": out[7],
-                "node_updates": out[8], "changed_updates": out[9], "key_only_updates": out[10]}
+                "node_updates": out[8], "changed_updates": out[9], "key_only_updates": out[10],
+                                "settle_stage": int(out[11])}
 
 
 _default_ctx = None

@@ -1,0 +1,160 @@
+"""Request loop on one GPU: continuous batching of traveltime and gradient requests.
+
+No counterpart in the reference (it runs one call at a time).  ``Server`` wraps ``eik_server``
+(include/eik.h): ``nslots`` execution slots, each a context with its own stream, take queued
+requests as soon as a slot is free and the request's device working set fits in the memory
+budget; admitted requests overlap on the device and each batches all of its sources into shared
+launches.  Models are ``EikModel`` handles kept resident by the caller.
+
+    srv = Server(nslots=4)
+    r1 = srv.submit_traveltime(model_a, coordsrc, coordrec)
+    r2 = srv.submit_gradient(model_b, coordsrc, coordrec, pickobs, stdobs, "gradvel")
+    picks = r1.result(); gradvel, misf = r2.result()
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .api import _WHICH, _pack, _params_struct
+
+REQ_TRAVELTIME = 1
+REQ_GRADIENT = 2
+
+
+class Request:
+    """A submitted request.  Holds its input and output arrays until ``result()``."""
+
+    def __init__(self, server, rid, kind, model, nsrc, off, arrays, outputs, whichgrad):
+        self._server = server
+        self.id = rid
+        self.kind = kind
+        self.model = model
+        self.nsrc = nsrc
+        self._off = off
+        self._arrays = arrays  # keeps the buffers the device side reads alive
+        self._out = outputs
+        self.whichgrad = whichgrad
+        self.latency_ms = self.queue_ms = self.run_ms = None
+        self._done = False
+        self._result = None
+
+    def result(self):
+        """Wait for completion and return what ``eiktraveltime`` / ``eikgradient`` return."""
+        if self._done:
+            return self._result
+        t = np.zeros(3)
+        err = ctypes.create_string_buffer(512)
+        rc = _lib.load().eik_wait(self._server.handle, self.id, _lib.ptr(t), err, 512)
+        self.latency_ms, self.queue_ms, self.run_ms = (float(x) for x in t)
+        self._done = True
+        _lib.check(rc, None, err.value.decode(errors="replace"))
+        off = self._off
+        if self.kind == REQ_TRAVELTIME:
+            picks = self._out["picks"]
+            self._result = [picks[off[s]:off[s + 1]].copy() for s in range(self.nsrc)]
+        else:
+            grd = self.model.grd
+            gvel = self._out["gradvel"].reshape(grd.grsize, order="F")
+            gsrc = self._out["gradsrc"][:self.nsrc].copy()
+            misf = float(self._out["misf"][0])
+            self._result = {1: (gvel, misf), 2: (gsrc, misf), 3: (gvel, gsrc, misf)}[self.whichgrad]
+        self._arrays = None
+        self._server._pending.pop(self.id, None)
+        return self._result
+
+
+class Server:
+    """Continuous-batching request loop (``eik_server``) on one device."""
+
+    def __init__(self, nslots: int = 4, mem_budget: int | None = None, device: int = 0):
+        if mem_budget is None:
+            mem_budget = 64 << 30
+        h = ctypes.c_void_p()
+        rc = _lib.load().eik_server_create(int(device), int(nslots), int(mem_budget), ctypes.byref(h))
+        _lib.check(rc, None, "eik_server_create failed")
+        self.handle = h
+        self._pending = {}
+
+    def _submit(self, req: _lib.EikRequest) -> int:
+        rid = ctypes.c_int64()
+        rc = _lib.load().eik_submit(self.handle, ctypes.byref(req), ctypes.byref(rid))
+        _lib.check(rc, None, "eik_submit rejected the request")
+        return int(rid.value)
+
+    def submit_traveltime(self, model, coordsrc, coordrec, *, extraparams=None) -> Request:
+        grd = model.grd
+        cs, off, packed = _pack(coordsrc, coordrec, grd.ndim)
+        nsrc = cs.shape[0]
+        picks = np.zeros(max(int(off[-1]), 1))
+        req = _lib.EikRequest()
+        req.type = REQ_TRAVELTIME
+        req.model = model.handle
+        req.nsrc = nsrc
+        req.coordsrc = _lib.ptr(cs)
+        req.rec_off = _lib.ptr(off, ctypes.c_int64)
+        req.coordrec = _lib.ptr(packed)
+        req.params = _params_struct(extraparams)
+        req.ttpicks = _lib.ptr(picks)
+        rid = self._submit(req)
+        r = Request(self, rid, REQ_TRAVELTIME, model, nsrc, off, (cs, off, packed), {"picks": picks}, 0)
+        self._pending[rid] = r  # buffers stay alive until the request is waited for
+        return r
+
+    def submit_gradient(self, model, coordsrc, coordrec, pickobs, stdobs, whichgrad, *, extraparams=None) -> Request:
+        grd = model.grd
+        wg = _WHICH.get(str(whichgrad).lstrip(":"))
+        assert wg is not None, "Symbol 'whichgrad' not in (:gradvel, :gradsrcloc, :gradvelandsrcloc) "
+        cs, off, packed = _pack(coordsrc, coordrec, grd.ndim)
+        nsrc = cs.shape[0]
+        po = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.float64).ravel() for x in pickobs]))
+        so = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.float64).ravel() for x in stdobs]))
+        assert po.size == off[-1] and so.size == off[-1], "one observation and one std per receiver"
+        gv = np.zeros(grd.npts)
+        gs = np.zeros((max(nsrc, 1), grd.ndim))
+        misf = np.zeros(1)
+        req = _lib.EikRequest()
+        req.type = REQ_GRADIENT
+        req.whichgrad = wg
+        req.model = model.handle
+        req.nsrc = nsrc
+        req.coordsrc = _lib.ptr(cs)
+        req.rec_off = _lib.ptr(off, ctypes.c_int64)
+        req.coordrec = _lib.ptr(packed)
+        req.pickobs = _lib.ptr(po)
+        req.stdobs = _lib.ptr(so)
+        req.params = _params_struct(extraparams)
+        req.gradvel = _lib.ptr(gv)
+        req.gradsrc = _lib.ptr(gs)
+        req.misf = _lib.ptr(misf)
+        rid = self._submit(req)
+        r = Request(self, rid, REQ_GRADIENT, model, nsrc, off, (cs, off, packed, po, so),
+                    {"gradvel": gv, "gradsrc": gs, "misf": misf}, wg)
+        self._pending[rid] = r
+        return r
+
+    def stats(self) -> dict:
+        out = np.zeros(4, dtype=np.int64)
+        _lib.load().eik_server_stats(self.handle, _lib.ptr(out, ctypes.c_int64), 4)
+        return {"queued": int(out[0]), "running": int(out[1]), "admitted_bytes": int(out[2]), "budget": int(out[3])}
+
+    def close(self):
+        """Wait for every submitted request, then release the slots."""
+        if getattr(self, "handle", None):
+            _lib.load().eik_server_destroy(self.handle)  # returns once queued and running work is done
+            self.handle = None
+            self._pending.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

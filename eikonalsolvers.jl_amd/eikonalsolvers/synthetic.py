@@ -9,6 +9,10 @@ normals) so any machine regenerates bit-identical inputs without shipping the fi
                          observed picks from a second perturbed model plus N(0, 0.01^2) noise
 * ``c3_2d_batched``   -- 2D 600x400, h=0.25, 8 random layers + smooth field, 256 sources, 128
                          surface receivers each
+* ``c5_request``      -- one request of the serving mix (seeded by its id): 70 % forward / 30 %
+                         gradient, 2D 100^2-400^2 or 3D 64^3-160^3 (h=1), 1-64 sources, 16-128
+                         receivers, smooth random velocity in [1.5, 6]
+* ``c5_arrivals``     -- Poisson arrival times (seed 50501)
 """
 from __future__ import annotations
 
@@ -108,3 +112,57 @@ def c3_2d_batched(nsrc: int = 256, nrec: int = 128):
                            grd.hgrid + u[:, 1] * (grd.y[-1] - 2 * grd.hgrid)])
     rec = np.column_stack([np.linspace(0.5, 149.25, nrec), np.zeros(nrec)])
     return grd, vel, src, [rec.copy() for _ in range(nsrc)]
+
+
+def smooth_velocity(shape, seed: int, lo: float = 1.5, hi: float = 6.0, nmodes: int = 6) -> np.ndarray:
+    """Sum of a few random low-wavenumber cosine modes, mapped to [lo, hi]; cheap on large grids."""
+    u = uniform(seed, nmodes * (2 * len(shape) + 1)).reshape(nmodes, -1)
+    f = np.zeros(shape)
+    axes = [np.arange(n, dtype=np.float64) / n for n in shape]
+    for m in range(nmodes):
+        arg = None
+        for d, a in enumerate(axes):
+            term = (1.0 + 3.0 * u[m, d]) * 2.0 * np.pi * a
+            sh = [1] * len(shape)
+            sh[d] = shape[d]
+            term = term.reshape(sh)
+            arg = term if arg is None else arg + term
+        f = f + np.cos(arg + 2.0 * np.pi * u[m, -1])
+    f = (f - f.min()) / max(f.max() - f.min(), 1e-300)
+    return lo + (hi - lo) * (0.15 + 0.7 * f)
+
+
+def c5_request(rid: int, small: bool = False) -> dict:
+    """Request `rid` of the serving mix.  small=True shrinks grids and source counts (tests)."""
+    u = uniform(50500 + rid, 8)
+    gradient = u[0] < 0.3
+    three_d = u[1] < 0.5
+    if three_d:
+        n = int(24 + 16 * u[2]) if small else int(64 + 96 * u[2])
+        grd = Grid3DCart(hgrid=1.0, cooinit=(0.0, 0.0, 0.0), grsize=(n, n, n))
+    else:
+        n = int(100 + 60 * u[2]) if small else int(100 + 300 * u[2])
+        grd = Grid2DCart(hgrid=1.0, cooinit=(0.0, 0.0), grsize=(n, n))
+    nsrc = 1 + int((7 if small else 63) * u[3])
+    nrec = 16 + int(112 * u[4])
+    vel = smooth_velocity(grd.grsize, 50600 + rid)
+    L = np.array([(k - 1) * grd.hgrid for k in grd.grsize])
+    ps = uniform(50700 + rid, nsrc * grd.ndim).reshape(nsrc, grd.ndim)
+    src = (0.05 + 0.9 * ps) * L
+    pr = uniform(50800 + rid, nrec * grd.ndim).reshape(nrec, grd.ndim)
+    rec = pr * L
+    rec[:, -1] = 0.0  # top face / surface
+    out = {"id": rid, "kind": "gradient" if gradient else "traveltime", "grd": grd, "vel": vel, "src": src,
+           "recs": [rec.copy() for _ in range(nsrc)]}
+    if gradient:
+        # observations: straight-ray times in a uniform 3.5 km/s medium (a plausible misfit)
+        obs = [np.sqrt(((rec - s) ** 2).sum(1)) / 3.5 for s in src]
+        out["pickobs"] = obs
+        out["stdobs"] = [np.full(nrec, 0.01) for _ in range(nsrc)]
+    return out
+
+
+def c5_arrivals(n: int, rate: float, seed: int = 50501) -> np.ndarray:
+    """Arrival times (s) of n requests of a Poisson stream with the given rate (1/s)."""
+    gaps = -np.log(np.maximum(uniform(seed, n), 1e-300)) / rate
+    return np.cumsum(gaps) - gaps[0]

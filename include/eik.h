@@ -107,8 +107,51 @@ int eik_misfit(eik_ctx* ctx, eik_model* model, int64_t nsrc, const double* coord
  * [4] = tile activations of the coarse solve (node-visits = activations x tile nodes),
  * [5] = coarse solve rounds (launches), [6] = summed duration of those launches (ms, HIP events
  * on the library's stream), [7] = local relaxation sweeps summed over the activations,
- * [8] = node updates (stencil replays) of the coarse solve, [9] = those that changed the node. */
+ * [8] = node updates (stencil replays) of the coarse solve, [9] = those that changed the node,
+ * [10] = of those, key-only changes, [11] = settling stage of the coarse solve: 0 = exact; k > 0
+ * = after 2000 rounds, changes below 10^(k-1) x 1e-12 relative (FP64; 4e-6 in FP32) stopped
+ * propagating (the tolerance grows tenfold every 500 rounds).  Converging solves stay at 0. */
 int eik_last_stats(const eik_ctx* ctx, double* out, int32_t nout);
+
+/* ---------------------------------------------------------------- request loop (serving)
+ * No reference counterpart: the reference evaluates one call at a time.  A server runs
+ * traveltime and gradient requests with continuous batching: nslots slots (each a context with
+ * its own stream and reusable device pools) take queued requests as soon as a slot is free and
+ * the request's device working set fits in what mem_budget (bytes) has left; admitted requests
+ * run concurrently and each batches all of its sources into shared launches.  The arrays a
+ * request points to (inputs and outputs) must stay valid until eik_wait returns for it, and its
+ * model must not be updated or destroyed meanwhile. */
+#define EIK_REQ_TRAVELTIME 1 /* eiktraveltime: fills ttpicks */
+#define EIK_REQ_GRADIENT 2   /* eikgradient: fills gradvel / gradsrc per whichgrad, and misf */
+
+typedef struct {
+    int32_t type;      /* EIK_REQ_* */
+    int32_t whichgrad; /* EIK_GRAD* (gradient requests) */
+    eik_model* model;
+    int64_t nsrc;
+    const double* coordsrc;
+    const int64_t* rec_off;
+    const double* coordrec;
+    const double* pickobs; /* gradient requests */
+    const double* stdobs;
+    eik_params params;
+    double* ttpicks;
+    double* gradvel;
+    double* gradsrc;
+    double* misf;
+} eik_request;
+
+typedef struct eik_server eik_server;
+int eik_server_create(int32_t device, int32_t nslots, int64_t mem_budget, eik_server** out);
+/* waits for queued and running requests, then releases the slots */
+void eik_server_destroy(eik_server* server);
+int eik_submit(eik_server* server, const eik_request* request, int64_t* id);
+/* Blocks until request id has finished; returns its status (EIK_OK or the call's error code).
+ * times_ms (optional, 3 doubles): latency from submission, queueing delay, run time; errbuf
+ * (optional) receives the error message. */
+int eik_wait(eik_server* server, int64_t id, double* times_ms, char* errbuf, int64_t errlen);
+/* out[0] queued requests, [1] running, [2] admitted working-set bytes, [3] budget */
+int eik_server_stats(eik_server* server, int64_t* out, int32_t nout);
 
 #ifdef __cplusplus
 }

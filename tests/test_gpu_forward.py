@@ -63,3 +63,23 @@ def test_forward_mixed_within_tolerance(eik, oracle, name, refine):
         assert l2 <= 1e-4
         assert q <= 1e-4
         assert relp.max() <= 1e-5
+
+
+def test_settling_stays_within_fp64_tolerance(eik, oracle, monkeypatch):
+    """The settling rounds (entered only when a solve circulates rounding-level changes instead of
+    converging) stop changes below 1e-12 relative from propagating; forced on from the first
+    round, the result must still match the heap FMM far below the stated tolerances."""
+    monkeypatch.setenv("EIK_SETTLE_ROUND", "0")
+    grd, vel, src, recs = CASES["rand3d"]()
+    ep = ExtraParams(refinearoundsrc=True)
+    m = eik.EikModel(vel, grd)
+    pg, tg = m.traveltime(src, recs, returntt=True, extraparams=ep)
+    stage = m.ctx.last_stats()["settle_stage"]
+    m.close()
+    po, to = oracle.traveltime(vel, grd, src, recs, returntt=True, extraparams=ep)
+    for s in range(len(tg)):
+        rel = np.abs(tg[s] - to[s]) / np.abs(to[s])
+        print(f"  settle stage {stage}; src {s}: grid max rel {rel.max():.3e}")
+        assert rel.max() <= 1e-9
+        np.testing.assert_allclose(pg[s], po[s], rtol=1e-9, atol=0)
+    assert stage >= 1
