@@ -205,6 +205,16 @@ int eik_model_create(eik_ctx* ctx, const eik_grid* grid, const double* vel, eik_
         eik_model_destroy(M);
         return ctx->fail(EIK_ENOMEM, "out of device memory (model)");
     }
+    if (grid->kind == EIK_GRID3D_SPHERE) {
+        // sin(theta_j) of the 3D sphere metric r sin(theta) dphi, from the host libm
+        std::vector<double> st((size_t)M->g.n[1]);
+        for (int64_t j = 0; j < M->g.n[1]; j++) st[j] = sin(deg2rad(gcoord(M->g, 1, j)));
+        if (hipMalloc(&M->d_sinth, st.size() * sizeof(double)) != hipSuccess ||
+            hipMemcpy(M->d_sinth, st.data(), st.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+            eik_model_destroy(M);
+            return ctx->fail(EIK_ENOMEM, "out of device memory (model)");
+        }
+    }
     rc = update_vel_locked(M, vel);
     if (rc) {
         eik_model_destroy(M);
@@ -226,6 +236,7 @@ void eik_model_destroy(eik_model* M) {
     if (M->d_vel) (void)hipFree(M->d_vel);
     if (M->d_slow64) (void)hipFree(M->d_slow64);
     if (M->d_slow32) (void)hipFree(M->d_slow32);
+    if (M->d_sinth) (void)hipFree(M->d_sinth);
     delete M;
 }
 

@@ -583,9 +583,7 @@ __global__ void k_corners(const CornerDev* __restrict__ cs, int ncs, const InstD
     } else {
         v = velc[l];
     }
-    double xyz[3];
-    for (int d = 0; d < nd; d++) xyz[d] = fma((double)g[d], I.h[d], I.o[d]);
-    const double dist = distance2points(kind, nd, xyz, C.src);
+    const double dist = C.dist;
     ttpool[I.tt_off + l] = -(T)(dist / v);
     seed_tiles_fwd(I, nd, g, flags, pend, dist / v);
 }
@@ -873,6 +871,14 @@ static void set_band(InstDev& I, double smean) {
     I.nband = mn > 0.0 ? mn * I.h[0] * smean : INFINITY;
 }
 
+// distance from a source-box corner to the source, on the host (sourceboxloctt!,
+// eikmainfwd.jl:1195-1322): the same libm as the oracle, so the init values agree bit for bit
+static double corner_dist(const InstDev& I, int nd, int kind, const int* ijk, const double* src) {
+    double xyz[3] = {0.0, 0.0, 0.0};
+    for (int d = 0; d < nd; d++) xyz[d] = fma((double)ijk[d], I.h[d], I.o[d]);
+    return distance2points(kind, nd, xyz, src);
+}
+
 int forward_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, int prec, const eik_params* P) {
     const int nd = M->g.ndim;
     const int kind = M->g.kind;
@@ -887,6 +893,8 @@ int forward_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, int prec, cons
     for (int64_t s = 0; s < nsrc; s++) {
         InstDev& I = cinst[s];
         make_inst(M->g, I);
+        I.sinth = M->d_sinth;
+        if (kind == EIK_GRID3D_SPHERE && !I.sinth) return ctx->fail(EIK_EINVAL, "3D sphere model without its metric table");
         set_tiles(I, nd);
         tiles_per = (int64_t)I.t[0] * I.t[1] * I.t[2];
         atiles_per = (int64_t)I.at[0] * I.at[1] * I.at[2];
@@ -932,6 +940,21 @@ int forward_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, int prec, cons
         }
         maxt = std::max(maxt, ntiles_f);
         maxat = std::max(maxat, antiles_f);
+        if (kind == EIK_GRID3D_SPHERE) {
+            // the boxes' own sin(theta) tables (a 3D sphere is refined only by the mixed-precision
+            // FP64 box), from the host libm like the model's
+            std::vector<double> tab;
+            std::vector<size_t> off(nsrc);
+            for (int64_t s = 0; s < nsrc; s++) {
+                off[s] = tab.size();
+                for (int j = 0; j < finst[s].n[1]; j++)
+                    tab.push_back(sin(deg2rad(fma((double)j, finst[s].h[1], finst[s].o[1]))));
+            }
+            DevBuf& ft = ctx->buf("fine_sinth");
+            if (ft.ensure(tab.size() * sizeof(double))) return ctx->fail(EIK_ENOMEM, "out of device memory");
+            CK(hipMemcpy(ft.p, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+            for (int64_t s = 0; s < nsrc; s++) finst[s].sinth = (const double*)ft.p + off[s];
+        }
     }
     DevBuf& aflags = ctx->buf("fim_flags");  // adjoint rounds, one byte per adjoint tile
     if (aflags.ensure((size_t)maxat)) return ctx->fail(EIK_ENOMEM, "out of device memory (tile flags)");
@@ -976,6 +999,7 @@ int forward_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, int prec, cons
                 C.inst = (int)s;
                 for (int d = 0; d < 3; d++) C.ijk[d] = fine[s].corn[c][d];
                 for (int d = 0; d < nd; d++) C.src[d] = S.src[s * nd + d];
+                C.dist = corner_dist(finst[s], nd, kind, C.ijk, C.src);
                 corners.push_back(C);
             }
         }
@@ -1020,6 +1044,7 @@ int forward_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, int prec, cons
                 C.inst = (int)s;
                 for (int d = 0; d < 3; d++) C.ijk[d] = corn[c][d];
                 for (int d = 0; d < nd; d++) C.src[d] = S.src[s * nd + d];
+                C.dist = corner_dist(cinst[s], nd, kind, C.ijk, C.src);
                 corners.push_back(C);
             }
         }

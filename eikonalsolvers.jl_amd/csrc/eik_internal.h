@@ -40,6 +40,7 @@ struct InstDev {
     double tstopS;
     int32_t tstopD;
     int32_t pad2;
+    const double* sinth; // 3D sphere: sin(theta_j) per theta index, computed on the host (else null)
     double band;         // band scheduler: tiles within this traveltime of the lowest pending one run
     double nband;        // the same for the nodes updated inside a tile
 };
@@ -78,6 +79,7 @@ struct CornerDev {
     int32_t inst;
     int32_t ijk[3];
     double src[3];
+    double dist;  // distance corner-source, computed on the host (the same libm as the oracle)
 };
 
 // fine index -> index inside the coarse window, nearest-neighbour rule of createfinegrid
@@ -182,8 +184,7 @@ __device__ __forceinline__ void node_spacing(const InstDev& I, const int* g, T* 
     dh[0] = (T)I.h[0];
     dh[1] = (T)(r * I.dthrad);
     if (I.kind == EIK_GRID3D_SPHERE) {
-        const double th = fma((double)g[1], I.h[1], I.o[1]);
-        dh[2] = (T)(r * sin(deg2rad(th)) * I.dphrad);
+        dh[2] = (T)(r * I.sinth[g[1]] * I.dphrad);
     } else {
         dh[2] = (T)0;
     }
@@ -336,6 +337,7 @@ struct eik_model {
     double* d_slow64 = nullptr;
     float* d_slow32 = nullptr;
     double smean = 1.0;  // mean slowness (band scheduler scale)
+    double* d_sinth = nullptr;  // 3D sphere: sin(theta) per theta index
 };
 
 #define CK(x)                                                                                    \

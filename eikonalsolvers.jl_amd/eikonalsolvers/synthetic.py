@@ -9,6 +9,9 @@ normals) so any machine regenerates bit-identical inputs without shipping the fi
                          observed picks from a second perturbed model plus N(0, 0.01^2) noise
 * ``c3_2d_batched``   -- 2D 600x400, h=0.25, 8 random layers + smooth field, 256 sources, 128
                          surface receivers each
+* ``c4_3d_sphere``    -- 3D spherical 120x150x150 (dr = 2 km, dtheta = dphi = 0.1 deg, outer radius
+                         6371 km), v(r) = 5.8 -> 8.0 km/s with depth times (1 + 0.03 GRF), 32
+                         sources at 10-150 km depth, 200 surface receivers
 * ``c5_request``      -- one request of the serving mix (seeded by its id): 70 % forward / 30 %
                          gradient, 2D 100^2-400^2 or 3D 64^3-160^3 (h=1), 1-64 sources, 16-128
                          receivers, smooth random velocity in [1.5, 6]
@@ -18,7 +21,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from .grids import Grid2DCart, Grid3DCart
+from .grids import Grid2DCart, Grid3DCart, Grid3DSphere
 
 _M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
 
@@ -50,14 +53,14 @@ def normal(seed: int, n: int) -> np.ndarray:
 def gaussian_random_field(shape, seed: int, corr_cells: float, std: float) -> np.ndarray:
     """White N(0,1) filtered by a separable periodic Gaussian of sigma=corr_cells (FFT), rescaled."""
     w = normal(seed, int(np.prod(shape))).reshape(shape, order="F")
-    spec = np.fft.rfftn(w)
+    spec = np.fft.rfftn(w, axes=list(range(len(shape))))
     for ax, n in enumerate(shape):
         k = np.fft.fftfreq(n) if ax < len(shape) - 1 else np.fft.rfftfreq(n)
         g = np.exp(-2.0 * (np.pi * k * corr_cells) ** 2)
         sh = [1] * len(shape)
         sh[ax] = g.size
         spec = spec * g.reshape(sh)
-    f = np.fft.irfftn(spec, s=shape)
+    f = np.fft.irfftn(spec, s=shape, axes=list(range(len(shape))))
     f = (f - f.mean()) / (f.std() + 1e-300)
     return std * f
 
@@ -166,3 +169,20 @@ def c5_arrivals(n: int, rate: float, seed: int = 50501) -> np.ndarray:
     """Arrival times (s) of n requests of a Poisson stream with the given rate (1/s)."""
     gaps = -np.log(np.maximum(uniform(seed, n), 1e-300)) / rate
     return np.cumsum(gaps) - gaps[0]
+
+
+def c4_3d_sphere(nsrc: int = 32, nrec: int = 200, grsize=(120, 150, 150)):
+    nr, nt, npp = grsize
+    grd = Grid3DSphere(Δr=2.0, Δθ=0.1, Δφ=0.1, grsize=tuple(grsize), cooinit=(6371.0 - 2.0 * (nr - 1), 82.55, 10.0))
+    r = grd.r[:, None, None]
+    v1 = 5.8 + 2.2 * (6371.0 - r) / 238.0
+    vel = v1 * (1.0 + 0.03 * gaussian_random_field(grd.grsize, 40401, 10.0, 1.0))
+    u = uniform(40402, 3 * nsrc).reshape(nsrc, 3)
+    th0, th1 = grd.θ[0], grd.θ[-1]
+    ph0, ph1 = grd.φ[0], grd.φ[-1]
+    depth = np.minimum(10.0 + 140.0 * u[:, 0], 6371.0 - grd.r[0])
+    src = np.column_stack([6371.0 - depth, th0 + (0.05 + 0.9 * u[:, 1]) * (th1 - th0),
+                           ph0 + (0.05 + 0.9 * u[:, 2]) * (ph1 - ph0)])
+    w = uniform(40403, 2 * nrec).reshape(nrec, 2)
+    rec = np.column_stack([np.full(nrec, 6371.0), th0 + w[:, 0] * (th1 - th0), ph0 + w[:, 1] * (ph1 - ph0)])
+    return grd, vel, np.ascontiguousarray(src), [rec.copy() for _ in range(nsrc)]

@@ -1,0 +1,63 @@
+"""3D spherical forward at the C4 configuration: (r, theta, phi) grid 120x150x150, dr = 2 km,
+dtheta = dphi = 0.1 deg, outer radius 6371 km, 32 sources at 10-150 km depth, 200 surface
+receivers each, FP64 and mixed precision.
+
+Grid3DSphere does not run in the reference (its fine grid and derivative matrices raise), so
+this build gives it its own forward semantics (r sin(theta) dphi metric, chord distances,
+no refinement) and the oracle restates those: parity unpinned against the reference, pinned
+against the oracle here.  Refinement and gradients raise, as in the reference.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from eikonalsolvers.grids import ExtraParams
+from eikonalsolvers.synthetic import c4_3d_sphere
+
+pytestmark = pytest.mark.gpu
+
+NTHREADS = max(1, min(16, os.cpu_count() or 1))
+
+
+@pytest.fixture(scope="module")
+def c4(oracle):
+    grd, vel, src, recs = c4_3d_sphere()
+    ep = ExtraParams(refinearoundsrc=False)
+    po, to = oracle.traveltime(vel, grd, src, recs, returntt=True, extraparams=ep, nthreads=NTHREADS)
+    return grd, vel, src, recs, po, to
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_c4_sphere_forward(eik, c4, precision):
+    grd, vel, src, recs, po, to = c4
+    m = eik.EikModel(vel, grd)
+    pg, tg = m.traveltime(src, recs, returntt=True, extraparams=ExtraParams(refinearoundsrc=False,
+                                                                            precision=precision))
+    m.close()
+    worst = 0.0
+    for s in range(len(src)):
+        rel = np.abs(tg[s] - to[s]) / np.abs(to[s])
+        relp = np.abs(pg[s] - po[s]) / np.abs(po[s])
+        worst = max(worst, rel.max())
+        if precision == "fp64":
+            assert rel.max() <= 1e-10
+            np.testing.assert_allclose(pg[s], po[s], rtol=1e-12, atol=0)
+        else:
+            # grid tolerance on 99.5 % of the nodes and in relative L2 (heap near-ties, see
+            # test_gpu_forward.py); receivers strictly
+            assert np.quantile(rel, 0.995) <= 1e-4
+            assert np.linalg.norm(tg[s] - to[s]) <= 1e-4 * np.linalg.norm(to[s])
+            assert relp.max() <= 1e-5
+    print(f"C4 {precision}: worst grid max rel over 32 sources {worst:.3e}", flush=True)
+
+
+def test_c4_refinement_and_gradient_raise(eik, c4):
+    grd, vel, src, recs, po, to = c4
+    m = eik.EikModel(vel, grd)
+    with pytest.raises(ValueError):
+        m.traveltime(src[:1], recs[:1], extraparams=ExtraParams(refinearoundsrc=True))
+    with pytest.raises(ValueError):
+        m.gradient(src[:1], recs[:1], po[:1], [np.full(len(po[0]), 0.01)], "gradvel",
+                   extraparams=ExtraParams(refinearoundsrc=False))
+    m.close()
