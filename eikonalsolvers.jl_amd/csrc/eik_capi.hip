@@ -251,6 +251,7 @@ static int load_sources(eik_ctx* ctx, eik_model* M, int64_t nsrc, const double* 
     if (P->smoothgradkern > 0 && P->smoothgradkern % 2 == 0)
         return ctx->fail(EIK_EINVAL, "ExtraParams: smoothgradkern must be an odd number");
     if (P->precision != EIK_PREC_FP64 && P->precision != EIK_PREC_MIXED) return ctx->fail(EIK_EINVAL, "bad precision");
+    if (P->fast != 0 && P->fast != 1) return ctx->fail(EIK_EINVAL, "fast must be 0 or 1");
     if (P->refinearoundsrc && (P->downscalefactor < 1 || P->noderadius < 0))
         return ctx->fail(EIK_EINVAL, "bad grid refinement parameters");
     if (P->refinearoundsrc && M->g.kind == EIK_GRID3D_SPHERE)

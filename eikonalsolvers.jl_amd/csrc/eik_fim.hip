@@ -60,9 +60,11 @@ __device__ __forceinline__ bool is_fixed(T raw) {
 }
 
 // Settling: a change whose value and key move by at most `settle` relative to the old ones is
-// written but wakes no other node.  settle = 0 (exact) unless the solve has run far more rounds
-// than any converging one: on rare models the replayed update then circulates changes of ~1e-13
-// relative through a few nodes instead of reaching the heap's fixed point (see fim_run_t).
+// written but wakes no other node.  In FP64 with `fast` settle = 1e-12 from the first round
+// (most of those changes are rounding ripples that would re-wake the whole cone downstream);
+// otherwise 0 until the solve has run far more rounds than any converging one: on rare models the
+// replayed update then circulates changes of ~1e-13 relative through a few nodes instead of
+// reaching the heap's fixed point (see fim_run_t).
 template <typename T>
 __device__ __forceinline__ bool tiny_change(T v0, const Key<T>& k0, T v1, const Key<T>& k1, T settle) {
     if (!(settle > T(0)) || !(fabs(v0) < tinf<T>())) return false;
@@ -750,7 +752,7 @@ __global__ void k_round_reset(unsigned* __restrict__ wmin, int ninst, unsigned* 
 // host reads how many tiles are still flagged; none means converged.
 template <int ND, typename T>
 static int fim_run_t(eik_ctx* ctx, const InstDev* d_inst, int ninst, int64_t ntiles, const TTPools& P,
-                     const T* slowpool, FimStats* fs) {
+                     const T* slowpool, bool exact, FimStats* fs) {
     DevBuf& list = ctx->buf("fim_list");
     DevBuf& linfo = ctx->buf("fim_list_info");
     DevBuf& wm = ctx->buf("fim_wmin");
@@ -787,10 +789,12 @@ static int fim_run_t(eik_ctx* ctx, const InstDev* d_inst, int ninst, int64_t nti
     const char* es = getenv("EIK_SETTLE_ROUND");  // test hook: settle from an earlier round
     const int64_t settle_batch = (es ? std::max(0, atoi(es)) : kSettleRound) / kRoundBatch;
     const T settle_tol = sizeof(T) == 8 ? T(1e-12) : T(4e-6);
+    // FP64 with `fast`: changes below settle_tol stop propagating from the first round
+    const T base_settle = (sizeof(T) == 8 && !exact) ? settle_tol : T(0);
     for (int64_t batch = 0;; batch++) {
         if (batch >= maxbatches) return ctx->fail(EIK_ENOCONV, "traveltime solve did not converge");
         // the tolerance grows tenfold every 500 rounds while the cycle persists
-        T settle = T(0);
+        T settle = base_settle;
         if (batch >= settle_batch) {
             settle = settle_tol;
             for (int64_t b = settle_batch + 500 / kRoundBatch; b <= batch; b += 500 / kRoundBatch) settle *= T(10);
@@ -834,12 +838,14 @@ static int fim_run_t(eik_ctx* ctx, const InstDev* d_inst, int ninst, int64_t nti
 }
 
 int fim_run(eik_ctx* ctx, int nd, int prec, const InstDev* d_inst, int ninst, int64_t ntiles, const TTPools& pools,
-            const void* slowpool, FimStats* fs) {
+            const void* slowpool, bool exact, FimStats* fs) {
     if (nd == 2)
-        return prec == EIK_PREC_FP64 ? fim_run_t<2, double>(ctx, d_inst, ninst, ntiles, pools, (const double*)slowpool, fs)
-                                     : fim_run_t<2, float>(ctx, d_inst, ninst, ntiles, pools, (const float*)slowpool, fs);
-    return prec == EIK_PREC_FP64 ? fim_run_t<3, double>(ctx, d_inst, ninst, ntiles, pools, (const double*)slowpool, fs)
-                                 : fim_run_t<3, float>(ctx, d_inst, ninst, ntiles, pools, (const float*)slowpool, fs);
+        return prec == EIK_PREC_FP64
+                   ? fim_run_t<2, double>(ctx, d_inst, ninst, ntiles, pools, (const double*)slowpool, exact, fs)
+                   : fim_run_t<2, float>(ctx, d_inst, ninst, ntiles, pools, (const float*)slowpool, exact, fs);
+    return prec == EIK_PREC_FP64
+               ? fim_run_t<3, double>(ctx, d_inst, ninst, ntiles, pools, (const double*)slowpool, exact, fs)
+               : fim_run_t<3, float>(ctx, d_inst, ninst, ntiles, pools, (const float*)slowpool, exact, fs);
 }
 
 // ----------------------------------------------------------------- forward pass of one call
@@ -1012,7 +1018,8 @@ int forward_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, int prec, cons
         CK(hipGetLastError());
         tm.mark();
         FimStats fst{};
-        int rc = fim_run(ctx, nd, EIK_PREC_FP64, (const InstDev*)dfi.p, (int)nsrc, ntiles_f, fpools, slf.p, &fst);
+        int rc = fim_run(ctx, nd, EIK_PREC_FP64, (const InstDev*)dfi.p, (int)nsrc, ntiles_f, fpools, slf.p,
+                         P->fast == 0, &fst);
         if (rc) return rc;
         ctx->stats[1] = tm.mark();
         DevBuf& ntr = ctx->buf("ntransfer");
@@ -1064,7 +1071,7 @@ int forward_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, int prec, cons
     }
     const void* slow = prec == EIK_PREC_FP64 ? (const void*)M->d_slow64 : (const void*)M->d_slow32;
     FimStats cst{};
-    int rc = fim_run(ctx, nd, prec, (const InstDev*)dci.p, (int)nsrc, ntiles_c, cpools, slow, &cst);
+    int rc = fim_run(ctx, nd, prec, (const InstDev*)dci.p, (int)nsrc, ntiles_c, cpools, slow, P->fast == 0, &cst);
     if (rc) return rc;
     ctx->stats[2] = tm.mark();
     ctx->stats[4] = cst.activations;

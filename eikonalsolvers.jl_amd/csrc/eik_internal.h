@@ -407,6 +407,6 @@ struct TTPools {
     uint8_t* d;
 };
 int fim_run(eik_ctx* ctx, int nd, int prec, const InstDev* d_inst, int ninst, int64_t ntiles, const TTPools& pools,
-            const void* slowpool, FimStats* fs);
+            const void* slowpool, bool exact, FimStats* fs);
 
 }  // namespace eik

@@ -39,7 +39,7 @@ class EikParams(ctypes.Structure):
     _fields_ = [("refinearoundsrc", ctypes.c_int32), ("downscalefactor", ctypes.c_int32),
                 ("noderadius", ctypes.c_int32), ("radiussmoothgradsrc", ctypes.c_int32),
                 ("smoothgradkern", ctypes.c_int32), ("precision", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 2)]
+                ("fast", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 class EikRequest(ctypes.Structure):

@@ -88,6 +88,7 @@ def _params_struct(extraparams) -> _lib.EikParams:
     p.radiussmoothgradsrc = int(ep.radiussmoothgradsrc)
     p.smoothgradkern = int(ep.smoothgradkern)
     p.precision = PRECISION_MIXED if ep.precision == "mixed" else PRECISION_FP64
+    p.fast = int(bool(ep.fast))
     return p
 
 

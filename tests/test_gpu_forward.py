@@ -28,9 +28,9 @@ CASES = {
 }
 
 
-def _run(eik, oracle, name, refine, precision):
+def _run(eik, oracle, name, refine, precision, fast=False):
     grd, vel, src, recs = CASES[name]()
-    ep = ExtraParams(refinearoundsrc=refine, precision=precision)
+    ep = ExtraParams(refinearoundsrc=refine, precision=precision, fast=fast)
     print(f"case={name} refine={refine} precision={precision} grid={grd.grsize} nsrc={len(src)}", flush=True)
     pg, tg = eik.eiktraveltime(vel, grd, src, recs, returntt=True, extraparams=ep)
     po, to = oracle.traveltime(vel, grd, src, recs, returntt=True, extraparams=ep)
@@ -39,14 +39,24 @@ def _run(eik, oracle, name, refine, precision):
 
 @pytest.mark.parametrize("name", list(CASES))
 @pytest.mark.parametrize("refine", [False, True])
-def test_forward_fp64_matches_oracle(eik, oracle, name, refine):
-    pg, tg, po, to = _run(eik, oracle, name, refine, "fp64")
+@pytest.mark.parametrize("fast", [False, True])
+def test_forward_fp64_matches_oracle(eik, oracle, name, refine, fast):
+    """Default FP64 replays the heap bit for bit (measured difference 0; checked at 1e-12).  With
+    fast=True changes below 1e-12 relative stop propagating; a suppressed change can flip a
+    decision at an exact heap tie, so that mode is held to the stated grid/receiver tolerances."""
+    pg, tg, po, to = _run(eik, oracle, name, refine, "fp64", fast)
     for s in range(len(tg)):
         rel = np.abs(tg[s] - to[s]) / np.maximum(np.abs(to[s]), 1e-300)
-        nbad = int(np.sum(rel > 1e-12))
-        print(f"  src {s}: max rel grid diff {rel.max():.3e}, nodes off by >1e-12: {nbad}/{rel.size}")
-        assert rel.max() <= 1e-10
-        np.testing.assert_allclose(pg[s], po[s], rtol=1e-12, atol=0)
+        relp = np.abs(pg[s] - po[s]) / np.abs(po[s])
+        nbad = int(np.sum(rel > 0))
+        print(f"  fast={fast} src {s}: max rel grid diff {rel.max():.3e}, nodes differing: {nbad}/{rel.size}, "
+              f"picks {relp.max():.3e}")
+        if not fast:
+            assert rel.max() <= 1e-12
+            np.testing.assert_allclose(pg[s], po[s], rtol=1e-12, atol=0)
+        else:
+            assert rel.max() <= 1e-4
+            assert relp.max() <= 1e-5
 
 
 @pytest.mark.parametrize("name", list(CASES))
