@@ -76,8 +76,56 @@ __device__ __forceinline__ bool tiny_change(T v0, const Key<T>& k0, T v1, const 
 __device__ __forceinline__ float key_down(double k) { return __double2float_rd(k); }
 __device__ __forceinline__ float key_down(float k) { return k; }
 
-// One round: every listed tile is relaxed in LDS until nothing within the band is left.  Each
-// node carries a pending trigger key (+inf: nothing pending):
+// Compact LDS image of a tile: the NODES core nodes (x fastest), then for each face k = 2*axis + l
+// (l = 0: + side, l = 1: - side; the neighbour-slot numbering of eik_stencil.h) its two halo
+// layers of FA nodes each, nearer layer first.  The edges and corners of the halo are never read
+// by the stencil and get no slot, which halves the image of a 3D tile (256 slots instead of 8^3).
+template <int ND>
+struct TileLds {
+    static constexpr int E = TileShape<ND>::TX;  // tiles are squares / cubes
+    static constexpr int NN = TileShape<ND>::NODES;
+    static constexpr int FA = NN / E;
+    static constexpr int NL = NN + 4 * ND * FA;
+    static constexpr int stride(int ax) { return ax == 0 ? 1 : (ax == 1 ? E : E * E); }
+    __device__ static int coord(int n, int ax) { return ax == 0 ? n % E : (ax == 1 ? (n / E) % E : n / (E * E)); }
+    // the coordinates of core node n other than `ax`, as an index into a face layer
+    __device__ static int face_index(int n, int ax) {
+        if (ax == 0) return n / E;
+        if (ax == 1) return n % E + E * (n / (E * E));
+        return n % (E * E);
+    }
+    // slot of the node d steps (d = 1, 2) from core node n across face k
+    __device__ static int nbr(int n, int k, int d) {
+        const int ax = k >> 1, l = k & 1;
+        const int c = coord(n, ax);
+        const bool in = l ? c - d >= 0 : c + d < E;
+        const int q = l ? d - 1 - c : c + d - E;
+        return in ? n + (l ? -d : d) * stride(ax) : NN + (2 * ax + l) * 2 * FA + q * FA + face_index(n, ax);
+    }
+    // the same, or NN when that node lies outside the core (halo nodes never change in a round)
+    __device__ static int core_nbr(int n, int k, int d) {
+        const int ax = k >> 1, l = k & 1;
+        const int c = coord(n, ax);
+        const bool in = l ? c - d >= 0 : c + d < E;
+        return in ? n + (l ? -d : d) * stride(ax) : NN;
+    }
+    // tile-local position of halo slot c (NN <= c < NL); coordinates may be -2, -1, E, E + 1
+    __device__ static void halo_pos(int c, int* p) {
+        const int r = c - NN;
+        const int f = r / (2 * FA), q = (r / FA) & 1, u = r % FA;
+        const int ax = f >> 1, l = f & 1;
+        const int a1 = ax == 0 ? 1 : 0, a2 = ax == 2 ? 1 : 2;  // the other axes, in order
+        p[0] = p[1] = p[2] = 0;
+        p[ax] = l ? -1 - q : E + q;
+        p[a1] = u % E;
+        if (ND == 3) p[a2] = u / E;
+    }
+};
+
+// One round: every listed tile is relaxed in LDS until nothing within the band is left.  One
+// wavefront per tile, one node per lane (lane n holds the same tile-local node in every tile, so
+// its LDS slots and stencil neighbours are fixed for the launch).  Each node carries a pending
+// trigger key (+inf: nothing pending):
 //   at entry: 0 for every node if the tile was seeded or did not converge last time; else the
 //            tile's pending value, for nodes whose key reaches it and that lie within two of a
 //            face whose halo changed (or, after a band cut, that have a reached neighbour);
@@ -86,355 +134,290 @@ __device__ __forceinline__ float key_down(float k) { return k; }
 // A change whose old and new keys both exceed a node's key leaves that node's heap replay as it
 // is (the replay stops before reaching such a neighbour), which is why the key filter is exact.
 // A sweep updates the nodes whose trigger lies within the instance's band (lowest pending tile
-// value + band width), compacted into a work list so the waves stay full; nodes beyond it stay
-// pending and the tile is queued again with their smallest trigger.  Faces whose 2-deep layer
-// changed activate the neighbour tile with that face's bit and the smallest changed key as its
-// pending value.  The list length is read on the device; blocks stride over it.
-template <int ND, typename T>
+// value + band width); nodes beyond it stay pending and the tile is queued again with their
+// smallest trigger.  Faces whose 2-deep layer changed activate the neighbour tile with that
+// face's bit and the smallest changed key as its pending value.  Blocks take list entries
+// (local tile, instance, pending value, face bits) from a cursor, claiming the next entry while
+// the current tile runs.  SPH: spherical grids (per-node metric); Cartesian grids share one
+// metric, held in scalar registers.
+template <int ND, typename T, bool SPH>
 __global__ void __attribute__((amdgpu_flat_work_group_size(kTileThreads, kTileThreads),
                                amdgpu_waves_per_eu(RoundOcc<ND, T>::value))) k_fim_round(
-    const int64_t* __restrict__ list, const uint2* __restrict__ info, unsigned* __restrict__ count,
-    const InstDev* __restrict__ inst, int ninst, const unsigned* __restrict__ wmin, T* __restrict__ ttpool,
-    T* __restrict__ kpool, T* __restrict__ spool, uint8_t* __restrict__ dpool, const T* __restrict__ slowpool,
-    unsigned* __restrict__ flags, unsigned* __restrict__ pend, unsigned long long* __restrict__ counters, T settle) {
+    const uint4* __restrict__ list, unsigned* __restrict__ count, const InstDev* __restrict__ inst,
+    const unsigned* __restrict__ wmin, T* __restrict__ ttpool, T* __restrict__ kpool, T* __restrict__ spool,
+    uint8_t* __restrict__ dpool, const T* __restrict__ slowpool, unsigned* __restrict__ flags,
+    unsigned* __restrict__ pend, unsigned long long* __restrict__ counters, T settle) {
     using S = TileShape<ND>;
-    constexpr int NL = S::LX * S::LY * S::LZ;
-    constexpr int NB = kTileThreads;
-    constexpr int NN = S::NODES;
-    constexpr int NPT = NN / NB;
-    constexpr int NW = NB / 64;
-    __shared__ T s_t[NL];        // value, sign bit = init node
-    __shared__ T s_k[NL];        // pop key K (the value itself outside cascades)
-    __shared__ T s_s[NL];        // pop key S (cascade nodes)
-    __shared__ uint8_t s_d[NL];  // pop key D
-    __shared__ float s_chg[2][NL];  // smaller of old/new key of a node changed in a sweep, +inf if not
-    __shared__ T s_sl[NN];
-    __shared__ uint16_t s_work[NN];
-    __shared__ unsigned s_wc[NW];
-    __shared__ unsigned s_face[8];  // per face (and whole tile) smallest changed key; [7] deferred
-    __shared__ unsigned s_nupd, s_nchg;  // s_nchg: value changes << 16 | key-only changes
-    const int tid = threadIdx.x;
-    const int lane = tid & 63, wave = tid >> 6;
-    __shared__ unsigned s_bi;
+    using L = TileLds<ND>;
+    constexpr int NN = S::NODES, NL = L::NL, E = L::E;
+    static_assert(NN == kTileThreads, "one node per lane");
+    static_assert(NL % NN == 0, "staging loop");
+    __shared__ T s_t[NL];               // value, sign bit = init node
+    __shared__ T s_k[NL];               // pop key K (the value itself outside cascades)
+    __shared__ T s_s[NL];               // pop key S (cascade nodes)
+    __shared__ uint8_t s_d[NL];         // pop key D
+    __shared__ float s_chg[2][NN + 1];  // smaller of old/new key of a node changed in a sweep; [NN]: +inf
+    __shared__ unsigned s_face[8];      // per face (and whole tile) smallest changed key; [7] deferred
+    const int n = threadIdx.x;
+    const int cx = L::coord(n, 0), cy = L::coord(n, 1), cz = ND == 3 ? L::coord(n, 2) : 0;
+    // faces this node lies within two of: bit 2a = - side of axis a, bit 2a + 1 = + side
+    unsigned fmask = (unsigned)(cx < 2) | (unsigned)(cx >= E - 2) << 1 | (unsigned)(cy < 2) << 2 |
+                     (unsigned)(cy >= E - 2) << 3;
+    if (ND == 3) fmask |= (unsigned)(cz < 2) << 4 | (unsigned)(cz >= E - 2) << 5;
     const unsigned nlist = count[kCntList];
-    unsigned long long tot_upd = 0ull, tot_chg = 0ull, tot_act = 0ull, tot_kchg = 0ull;
-    if (blockIdx.x == 0 && tid == 0 && nlist) atomicAdd(&count[kCntRounds], 1u);
-    for (;;) {
-        if (tid == 0) s_bi = atomicAdd(&count[kCntCursor], 1u);
-        __syncthreads();
-        const unsigned bi = s_bi;
-        if (bi >= nlist) break;
-        const int64_t gid = list[bi];
-        const uint2 inf0 = info[bi];
-        const float pend0 = __uint_as_float(inf0.x);
-        const unsigned fbits = inf0.y;
-        const int ii = find_inst(inst, ninst, gid);
-        const InstDev I = inst[ii];
-        const float lim = __uint_as_float(wmin[ii]) + (float)I.nband;
-        const int64_t tl = gid - I.tile_base;
-        const int tix = (int)(tl % I.t[0]);
-        const int tiy = (int)((tl / I.t[0]) % I.t[1]);
-        const int tiz = (ND == 3) ? (int)(tl / ((int64_t)I.t[0] * I.t[1])) : 0;
-        const int ox = tix * S::TX, oy = tiy * S::TY, oz = tiz * S::TZ;
+    unsigned tot_upd = 0u, tot_vchg = 0u, tot_kchg = 0u, tot_act = 0u, tot_sweeps = 0u;
+    if (blockIdx.x == 0 && n == 0 && nlist) atomicAdd(&count[kCntRounds], 1u);
+    if (n == 0) {
+        s_chg[0][NN] = EIK_INFF;
+        s_chg[1][NN] = EIK_INFF;
+    }
+    unsigned claim = 0u;
+    if (n == 0) claim = atomicAdd(&count[kCntCursor], 1u);
+    unsigned bi = (unsigned)__builtin_amdgcn_readfirstlane((int)claim);
+    while (bi < nlist) {
+        const uint4 e = list[bi];
+        if (n == 0) claim = atomicAdd(&count[kCntCursor], 1u);  // the next entry, consumed after this tile
+        const InstDev& I = inst[e.y];
+        const float pend0 = __uint_as_float(e.z);
+        const unsigned fbits = e.w;
+        const float lim = __uint_as_float(wmin[e.y]) + (float)I.nband;
+        const unsigned tl = e.x;
+        const unsigned tix = tl % (unsigned)I.t[0];
+        const unsigned tiy = (tl / (unsigned)I.t[0]) % (unsigned)I.t[1];
+        const unsigned tiz = ND == 3 ? tl / ((unsigned)I.t[0] * (unsigned)I.t[1]) : 0u;
+        const int ox = (int)tix * E, oy = (int)tiy * E, oz = (int)tiz * E;
         T* __restrict__ tt = ttpool + I.tt_off;
         T* __restrict__ kk = kpool + I.tt_off;
         T* __restrict__ ss = spool + I.tt_off;
         uint8_t* __restrict__ dd = dpool + I.tt_off;
         const T* __restrict__ slow = slowpool + I.s_off;
-        const int64_t nx = I.n[0], nxy = (int64_t)I.n[0] * I.n[1];
+        const int nx = I.n[0], ny = I.n[1], nz = ND == 3 ? I.n[2] : 1;
+        const int64_t nxy = (int64_t)nx * ny;
+        if (n < 8) s_face[n] = 0x7f800000u;
 
-        if (tid < 8) s_face[tid] = 0x7f800000u;
-        if (tid == 0) {
-            s_nupd = 0u;
-            s_nchg = 0u;
+        // stage the core and the face halos; this lane's own node comes first
+        const int g[3] = {ox + cx, oy + cy, oz + cz};
+        const bool inside = g[0] < nx && g[1] < ny && g[2] < nz;
+        const int64_t gl = g[0] + (int64_t)nx * g[1] + nxy * g[2];
+        T v0 = tinf<T>(), sl = T(0);
+        Key<T> k0{tinf<T>(), -tinf<T>(), 0};
+        if (inside) {
+            v0 = tt[gl];
+            k0.D = dd[gl];
+            sl = slow[gl];
+            if (k0.D) {
+                k0.K = kk[gl];
+                k0.S = ss[gl];
+            } else {
+                k0.K = fabs(v0);
+            }
         }
 #pragma unroll
-        for (int c = tid; c < NL; c += NB) {
-            const int lx = c % S::LX, ly = (c / S::LX) % S::LY, lz = (ND == 3) ? c / (S::LX * S::LY) : 0;
-            int nout = (lx < S::H || lx >= S::H + S::TX) + (ly < S::H || ly >= S::H + S::TY);
-            if (ND == 3) nout += (lz < S::H || lz >= S::H + S::TZ);
+        for (int c0 = NN; c0 < NL; c0 += NN) {
+            const int c = c0 + n;
+            int p[3];
+            L::halo_pos(c, p);
+            const int hx = ox + p[0], hy = oy + p[1], hz = oz + p[2];
             T v = tinf<T>(), kv = tinf<T>(), sv = -tinf<T>();
             uint8_t dv = 0;
-            if (nout <= 1) {
-                const int gx = ox + lx - S::H, gy = oy + ly - S::H, gz = (ND == 3) ? oz + lz - S::H : 0;
-                if (gx >= 0 && gx < I.n[0] && gy >= 0 && gy < I.n[1] && (ND == 2 || (gz >= 0 && gz < I.n[2]))) {
-                    const int64_t l = gx + nx * gy + nxy * gz;
-                    v = tt[l];
-                    dv = dd[l];
-                    if (dv) {
-                        kv = kk[l];
-                        sv = ss[l];
-                    } else {
-                        kv = fabs(v);
-                    }
+            if (hx >= 0 && hx < nx && hy >= 0 && hy < ny && hz >= 0 && hz < nz) {
+                const int64_t l = hx + (int64_t)nx * hy + nxy * hz;
+                v = tt[l];
+                dv = dd[l];
+                if (dv) {
+                    kv = kk[l];
+                    sv = ss[l];
+                } else {
+                    kv = fabs(v);
                 }
             }
             s_t[c] = v;
             s_k[c] = kv;
             s_s[c] = sv;
             s_d[c] = dv;
-            s_chg[0][c] = EIK_INFF;
-            s_chg[1][c] = EIK_INFF;
         }
-        auto node_pos = [&](int n, int& lx, int& ly, int& lz) {
-            lx = n % S::TX;
-            ly = (n / S::TX) % S::TY;
-            lz = (ND == 3) ? n / (S::TX * S::TY) : 0;
-        };
-        auto lds_index = [&](int lx, int ly, int lz) {
-            return (lx + S::H) + S::LX * ((ly + S::H) + S::LY * (ND == 3 ? lz + S::H : 0));
-        };
-        // this thread's nodes: LDS index, whether it is solved for, whether it is near a changed face
-        int li[NPT];
-        bool act[NPT];
-        bool near[NPT];
-        float ptr[NPT];  // pending trigger key
-        T v0[NPT];       // staged value and key of the node (the global state at entry)
-        Key<T> k0[NPT];
-    #pragma unroll
-        for (int j = 0; j < NPT; j++) {
-            const int n = tid + j * NB;
-            int lx, ly, lz;
-            node_pos(n, lx, ly, lz);
-            li[j] = lds_index(lx, ly, lz);
-            const int g[3] = {ox + lx, oy + ly, oz + lz};
-            const bool inside = g[0] < I.n[0] && g[1] < I.n[1] && (ND == 2 || g[2] < I.n[2]);
-            s_sl[n] = inside ? slow[g[0] + nx * g[1] + nxy * g[2]] : T(0);
-            unsigned m = (unsigned)(lx < 2) | (unsigned)(lx >= S::TX - 2) << 1 | (unsigned)(ly < 2) << 2 |
-                         (unsigned)(ly >= S::TY - 2) << 3;
-            if (ND == 3) m |= (unsigned)(lz < 2) << 4 | (unsigned)(lz >= S::TZ - 2) << 5;
-            near[j] = (m & fbits) != 0u;
-            act[j] = inside;
+        s_t[n] = v0;
+        s_k[n] = k0.K;
+        s_s[n] = k0.S;
+        s_d[n] = (uint8_t)k0.D;
+        s_chg[0][n] = EIK_INFF;
+        s_chg[1][n] = EIK_INFF;
+        // stencil neighbours inside the grid (ok1) and usable 2-away nodes (ok2), per slot k
+        unsigned ok1 = 0u, ok2 = 0u;
+#pragma unroll
+        for (int k = 0; k < 2 * ND; k++) {
+            const int ax = k >> 1;
+            int nb[3] = {g[0], g[1], g[2]};
+            nb[ax] += (k & 1) ? -1 : 1;
+            const int lim_ax = ax == 0 ? nx : (ax == 1 ? ny : nz);
+            const bool in1 = nb[ax] >= 0 && nb[ax] <= lim_ax - 1;
+            bool in2 = in1 && nb[0] >= 1 && nb[0] <= nx - 2 && nb[1] >= 1 && nb[1] <= ny - 2;
+            if (ND == 3) in2 = in2 && nb[2] >= 1 && nb[2] <= nz - 2;
+            ok1 |= (unsigned)in1 << k;
+            ok2 |= (unsigned)in2 << k;
         }
         __syncthreads();
-        constexpr int strides[3] = {1, S::LX, S::LX * S::LY};
-    #pragma unroll
-        for (int j = 0; j < NPT; j++) {
-            act[j] = act[j] && !is_fixed(s_t[li[j]]);
-            v0[j] = s_t[li[j]];
-            k0[j] = Key<T>{s_k[li[j]], s_s[li[j]], (int)s_d[li[j]]};
-            ptr[j] = EIK_INFF;
-            if (!act[j]) continue;
+        bool act = inside && !is_fixed(v0);
+        float ptr = EIK_INFF;  // pending trigger key
+        if (act) {
             if (fbits & kFullBit) {
-                ptr[j] = 0.f;
-                continue;
+                ptr = 0.f;
+            } else {
+                bool reached = false;
+#pragma unroll
+                for (int k = 0; k < 2 * ND; k++) reached = reached | (fabs(s_t[L::nbr(n, k, 1)]) < tinf<T>());
+                const bool cand = (fmask & fbits) != 0u || ((fbits & kKeyBit) != 0u && reached);
+                if (cand && (T)pend0 <= k0.K) ptr = pend0;
             }
-            bool reached = false;
-    #pragma unroll
-            for (int ax = 0; ax < ND; ax++)
-                reached = reached | (fabs(s_t[li[j] + strides[ax]]) < tinf<T>()) | (fabs(s_t[li[j] - strides[ax]]) < tinf<T>());
-            const bool cand = near[j] | ((fbits & kKeyBit) != 0u && reached);
-            if (cand && (T)pend0 <= s_k[li[j]]) ptr[j] = pend0;
         }
 
-        // Cartesian grids share one metric; spherical ones compute it per node
-        const bool cart = I.kind == EIK_GRID2D_CART || I.kind == EIK_GRID3D_CART;
         Metric<T> mcart;
-    #pragma unroll
+#pragma unroll
         for (int ax = 0; ax < 3; ax++) mcart.dh[ax] = (T)I.h[0];
         metric_coefs(mcart, ND);
-        unsigned nupd = 0u, nchg = 0u;
-        int it = 0;
+int it = 0;
         for (; it < S::MAXIT; it++) {
             const int cb = it & 1, pb = cb ^ 1;
-            // 1. collect this thread's triggers; the nodes within the band need an update
-            bool need[NPT];
-    #pragma unroll
-            for (int j = 0; j < NPT; j++) {
-                if (act[j] && it > 0) {
-                    const T kx = s_k[li[j]];
-                    float tr = ptr[j];
-    #pragma unroll
-                    for (int ax = 0; ax < ND; ax++) {
-                        const int st = strides[ax];
-                        const float c[4] = {s_chg[pb][li[j] + st], s_chg[pb][li[j] - st], s_chg[pb][li[j] + 2 * st],
-                                            s_chg[pb][li[j] - 2 * st]};
-    #pragma unroll
-                        for (int q = 0; q < 4; q++) tr = ((T)c[q] <= kx) ? fminf(tr, c[q]) : tr;
-                    }
-                    ptr[j] = tr;
-                }
-                need[j] = act[j] & (ptr[j] < EIK_INFF) & (ptr[j] <= lim);
-                ptr[j] = need[j] ? EIK_INFF : ptr[j];
-            }
-            // 2. compact the work list (wave ballots, then a prefix over the waves)
-            unsigned long long msk[NPT];
-            unsigned wtot = 0u;
-    #pragma unroll
-            for (int j = 0; j < NPT; j++) {
-                msk[j] = __ballot(need[j]);
-                wtot += (unsigned)__popcll(msk[j]);
-            }
-            if (lane == 0) s_wc[wave] = wtot;
-            __syncthreads();
-    #pragma unroll
-            for (int j = 0; j < NPT; j++) s_chg[pb][li[j]] = EIK_INFF;
-            unsigned base = 0u, total = 0u;
-    #pragma unroll
-            for (int w = 0; w < NW; w++) {
-                const unsigned c = s_wc[w];
-                base += (w < wave) ? c : 0u;
-                total += c;
-            }
-            if (total == 0u) break;
-            const unsigned long long below = (1ull << lane) - 1ull;
-    #pragma unroll
-            for (int j = 0; j < NPT; j++) {
-                if (need[j]) s_work[base + (unsigned)__popcll(msk[j] & below)] = (uint16_t)(tid + j * NB);
-                base += (unsigned)__popcll(msk[j]);
-            }
-            __syncthreads();
-            // 3. update the listed nodes
-            for (unsigned w = tid; w < total; w += NB) {
-                const int n = s_work[w];
-                int lx, ly, lz;
-                node_pos(n, lx, ly, lz);
-                const int lc = lds_index(lx, ly, lz);
-                const int g[3] = {ox + lx, oy + ly, oz + lz};
-                unsigned ok1 = 0u, ok2 = 0u;
-    #pragma unroll
-                for (int ax = 0; ax < ND; ax++) {
-    #pragma unroll
-                    for (int l = 0; l < 2; l++) {
-                        const int k = 2 * ax + l;
-                        int nb[3] = {g[0], g[1], g[2]};
-                        nb[ax] += (l == 0) ? 1 : -1;
-                        const bool in1 = nb[ax] >= 0 && nb[ax] <= I.n[ax] - 1;
-                        bool in2 = in1;
-    #pragma unroll
-                        for (int d = 0; d < ND; d++) in2 = in2 && nb[d] >= 1 && nb[d] <= I.n[d] - 2;
-                        ok1 |= (unsigned)in1 << k;
-                        ok2 |= (unsigned)in2 << k;
+            // 1. collect the triggers of the previous sweep; a node within the band is updated
+            if (act && it > 0) {
+                const T kx = s_k[n];
+                float tr = ptr;
+#pragma unroll
+                for (int k = 0; k < 2 * ND; k++) {
+#pragma unroll
+                    for (int d = 1; d <= 2; d++) {
+                        const float c = s_chg[pb][L::core_nbr(n, k, d)];
+                        tr = ((T)c <= kx) ? fminf(tr, c) : tr;
                     }
                 }
-                Metric<T> mt = mcart;
-                if (!cart) {
-                    node_spacing<T>(I, g, mt.dh);
-                    metric_coefs(mt, ND);
-                }
-                T v1[2 * ND], v2[2 * ND];
+                ptr = tr;
+            }
+            const bool need = act & (ptr < EIK_INFF) & (ptr <= lim);
+            ptr = need ? EIK_INFF : ptr;
+            __syncthreads();  // every lane has read the previous sweep's changes
+            s_chg[pb][n] = EIK_INFF;
+            if (__ballot(need) == 0ull) break;
+            // 2. update
+            if (need) {
                 unsigned f1 = 0u, f2 = 0u, d1 = 0u, d2 = 0u, dcap = 0u;
-    #pragma unroll
-                for (int ax = 0; ax < ND; ax++) {
-    #pragma unroll
-                    for (int l = 0; l < 2; l++) {
-                        const int k = 2 * ax + l;
-                        const int o1 = lc + ((l == 0) ? 1 : -1) * strides[ax];
-                        const int o2 = lc + ((l == 0) ? 2 : -2) * strides[ax];
-                        const bool in1 = (ok1 >> k) & 1u, in2 = (ok2 >> k) & 1u;
-                        const T a = in1 ? s_t[o1] : tinf<T>();
-                        const T b = in2 ? s_t[o2] : tinf<T>();
-                        const int da = in1 ? (int)s_d[o1] : 0, db = in2 ? (int)s_d[o2] : 0;
-                        f1 |= (unsigned)is_fixed(a) << k;
-                        f2 |= (unsigned)is_fixed(b) << k;
-                        d1 |= (unsigned)(da != 0) << k;
-                        d2 |= (unsigned)(db != 0) << k;
-                        dcap |= (unsigned)(da >= kMaxCascade) << k;
-                        v1[k] = fabs(a);
-                        v2[k] = fabs(b);
-                    }
+                T v1[2 * ND], v2[2 * ND];
+#pragma unroll
+                for (int k = 0; k < 2 * ND; k++) {
+                    const bool in1 = (ok1 >> k) & 1u, in2 = (ok2 >> k) & 1u;
+                    const int o1 = L::nbr(n, k, 1), o2 = L::nbr(n, k, 2);
+                    const T a = in1 ? s_t[o1] : tinf<T>();
+                    const T b = in2 ? s_t[o2] : tinf<T>();
+                    const int da = in1 ? (int)s_d[o1] : 0, db = in2 ? (int)s_d[o2] : 0;
+                    f1 |= (unsigned)is_fixed(a) << k;
+                    f2 |= (unsigned)is_fixed(b) << k;
+                    d1 |= (unsigned)(da != 0) << k;
+                    d2 |= (unsigned)(db != 0) << k;
+                    dcap |= (unsigned)(da >= kMaxCascade) << k;
+                    v1[k] = fabs(a);
+                    v2[k] = fabs(b);
                 }
                 // pop keys straight from LDS (only the cascade path reads them)
                 auto keys = [&](int k, bool two) {
-                    const int ax = k >> 1, l = k & 1;
-                    const int o = lc + (l == 0 ? 1 : -1) * (two ? 2 : 1) * strides[ax];
+                    const int o = L::nbr(n, k, two ? 2 : 1);
                     return Key<T>{s_k[o], s_s[o], (int)s_d[o]};
                 };
                 int code[3];
                 Key<T> nk;
-                const T nv = replay<ND, T>(v1, v2, keys, d1, d2, dcap, f1, f2, ok1, ok2, s_sl[n], mt, code, &nk);
-                nupd++;
-                const T cur = s_t[lc];
-                const T kold = s_k[lc];
+                T nv;
+                if constexpr (SPH) {
+                    Metric<T> mt;
+                    node_spacing<T>(I, g, mt.dh);
+                    metric_coefs(mt, ND);
+                    nv = replay<ND, T>(v1, v2, keys, d1, d2, dcap, f1, f2, ok1, ok2, sl, mt, code, &nk);
+                } else {
+                    nv = replay<ND, T>(v1, v2, keys, d1, d2, dcap, f1, f2, ok1, ok2, sl, mcart, code, &nk);
+                }
+                tot_upd++;
+                const T cur = s_t[n];
+                const T kold = s_k[n];
+                const T sold = s_s[n];
+                const int dold = (int)s_d[n];
                 const bool vchg = nv == nv && nv != cur && !(nv >= tinf<T>() && cur >= tinf<T>());
-                const bool kchg = nv < tinf<T>() && (nk.K != kold || nk.S != s_s[lc] || nk.D != (int)s_d[lc]);
+                const bool kchg = nv < tinf<T>() && (nk.K != kold || nk.S != sold || nk.D != dold);
                 if (vchg || kchg) {
-                    nchg += vchg ? 0x10000u : 1u;
-                    const bool quiet = tiny_change<T>(cur, Key<T>{kold, s_s[lc], (int)s_d[lc]}, nv, nk, settle);
-                    s_t[lc] = nv;
-                    s_k[lc] = nk.K;
-                    s_s[lc] = nk.S;
-                    s_d[lc] = (uint8_t)nk.D;
-                    if (!quiet) s_chg[cb][lc] = key_down(kold < nk.K ? kold : nk.K);
+                    tot_vchg += vchg ? 1u : 0u;
+                    tot_kchg += vchg ? 0u : 1u;
+                    const bool quiet = tiny_change<T>(cur, Key<T>{kold, sold, dold}, nv, nk, settle);
+                    s_t[n] = nv;
+                    s_k[n] = nk.K;
+                    s_s[n] = nk.S;
+                    s_d[n] = (uint8_t)nk.D;
+                    if (!quiet) s_chg[cb][n] = key_down(kold < nk.K ? kold : nk.K);
                 }
             }
             __syncthreads();
         }
         const bool converged = it < S::MAXIT;
 
-        // write back changed nodes; smallest changed key (old or new) per face; deferred triggers
-        if (nupd) {
-            atomicAdd(&s_nupd, nupd);
-            atomicAdd(&s_nchg, nchg);
-        }
-    #pragma unroll
-        for (int j = 0; j < NPT; j++)
-            if (ptr[j] < EIK_INFF) atomicMin(&s_face[7], __float_as_uint(ptr[j]));
-    #pragma unroll
-        for (int j = 0; j < NPT; j++) {
-            if (!act[j]) continue;
-            const int n = tid + j * NB;
-            int lx, ly, lz;
-            node_pos(n, lx, ly, lz);
-            const int64_t l = (ox + lx) + nx * (oy + ly) + nxy * (oz + lz);
-            const T v = s_t[li[j]];
-            const Key<T> kn{s_k[li[j]], s_s[li[j]], (int)s_d[li[j]]};
-            const Key<T> ko = k0[j];
-            if (v == v0[j] && keq(kn, ko)) continue;
-            const bool quiet = tiny_change<T>(v0[j], ko, v, kn, settle);
-            tt[l] = v;
-            dd[l] = (uint8_t)kn.D;
-            if (kn.D) {
-                kk[l] = kn.K;
-                ss[l] = kn.S;
+        // write back a changed node; smallest changed key (old or new) per face; deferred triggers
+        if (ptr < EIK_INFF) atomicMin(&s_face[7], __float_as_uint(ptr));
+        if (act) {
+            const T v = s_t[n];
+            const Key<T> kn{s_k[n], s_s[n], (int)s_d[n]};
+            if (!(v == v0 && keq(kn, k0))) {
+                const bool quiet = tiny_change<T>(v0, k0, v, kn, settle);
+                tt[gl] = v;
+                dd[gl] = (uint8_t)kn.D;
+                if (kn.D) {
+                    kk[gl] = kn.K;
+                    ss[gl] = kn.S;
+                }
+                if (!quiet) {
+                    const unsigned pv = __float_as_uint(key_down(k0.K < kn.K ? k0.K : kn.K));
+#pragma unroll
+                    for (int f = 0; f < 2 * ND; f++)
+                        if ((fmask >> f) & 1u) atomicMin(&s_face[f], pv);
+                    atomicMin(&s_face[6], pv);
+                }
             }
-            if (quiet) continue;
-            const unsigned pv = __float_as_uint(key_down(ko.K < kn.K ? ko.K : kn.K));
-            if (lx < 2) atomicMin(&s_face[0], pv);
-            if (lx >= S::TX - 2) atomicMin(&s_face[1], pv);
-            if (ly < 2) atomicMin(&s_face[2], pv);
-            if (ly >= S::TY - 2) atomicMin(&s_face[3], pv);
-            if (ND == 3) {
-                if (lz < 2) atomicMin(&s_face[4], pv);
-                if (lz >= S::TZ - 2) atomicMin(&s_face[5], pv);
-            }
-            atomicMin(&s_face[6], pv);
         }
         __syncthreads();
-        if (tid == 0) {
+        if (n == 0) {
             unsigned* fl = flags + I.tile_base;
             unsigned* pd = pend + I.tile_base;
-            const int64_t t0 = I.t[0], t01 = (int64_t)I.t[0] * I.t[1];
-            auto act_tile = [&](int64_t t, unsigned bit, unsigned v) {
+            const unsigned t0 = (unsigned)I.t[0], t01 = (unsigned)I.t[0] * (unsigned)I.t[1];
+            auto act_tile = [&](unsigned t, unsigned bit, unsigned v) {
                 atomicOr(&fl[t], bit);
                 atomicMin(&pd[t], v);
             };
             // the x- neighbour sees this tile across its x+ face, and so on
             if (s_face[0] != 0x7f800000u && tix > 0) act_tile(tl - 1, 1u << 1, s_face[0]);
-            if (s_face[1] != 0x7f800000u && tix < I.t[0] - 1) act_tile(tl + 1, 1u << 0, s_face[1]);
+            if (s_face[1] != 0x7f800000u && tix < (unsigned)I.t[0] - 1) act_tile(tl + 1, 1u << 0, s_face[1]);
             if (s_face[2] != 0x7f800000u && tiy > 0) act_tile(tl - t0, 1u << 3, s_face[2]);
-            if (s_face[3] != 0x7f800000u && tiy < I.t[1] - 1) act_tile(tl + t0, 1u << 2, s_face[3]);
+            if (s_face[3] != 0x7f800000u && tiy < (unsigned)I.t[1] - 1) act_tile(tl + t0, 1u << 2, s_face[3]);
             if (ND == 3) {
                 if (s_face[4] != 0x7f800000u && tiz > 0) act_tile(tl - t01, 1u << 5, s_face[4]);
-                if (s_face[5] != 0x7f800000u && tiz < I.t[2] - 1) act_tile(tl + t01, 1u << 4, s_face[5]);
+                if (s_face[5] != 0x7f800000u && tiz < (unsigned)I.t[2] - 1) act_tile(tl + t01, 1u << 4, s_face[5]);
             }
             if (!converged)
                 act_tile(tl, kFullBit, s_face[6] == 0x7f800000u ? 0u : s_face[6]);
             else if (s_face[7] != 0x7f800000u)
                 act_tile(tl, kKeyBit, s_face[7]);
-            atomicAdd(&count[kCntSweeps], (unsigned)it);
-            tot_upd += s_nupd;
-            tot_chg += (s_nchg >> 16) + (s_nchg & 0xffffu);
-            tot_kchg += s_nchg & 0xffffu;
-            tot_act++;
         }
-        __syncthreads();  // LDS and s_bi are reused by the next tile
+        tot_sweeps += (unsigned)it;
+        tot_act++;
+        __syncthreads();  // LDS is reused by the next tile
+        bi = (unsigned)__builtin_amdgcn_readfirstlane((int)claim);
     }
-    if (tid == 0 && tot_act) {
-        atomicAdd(&counters[0], tot_upd);
-        atomicAdd(&counters[1], tot_chg);
-        atomicAdd(&counters[2], tot_act);
-        atomicAdd(&counters[3], tot_kchg);
+    // per-launch statistics: one atomic per counter and block
+    unsigned long long upd = tot_upd, chg = (unsigned long long)tot_vchg + tot_kchg, kchg = tot_kchg;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        upd += __shfl_xor(upd, o);
+        chg += __shfl_xor(chg, o);
+        kchg += __shfl_xor(kchg, o);
+    }
+    if (n == 0 && tot_act) {
+        atomicAdd(&count[kCntSweeps], tot_sweeps);
+        atomicAdd(&counters[0], upd);
+        atomicAdd(&counters[1], chg);
+        atomicAdd(&counters[2], (unsigned long long)tot_act);
+        atomicAdd(&counters[3], kchg);
     }
 }
 
@@ -488,13 +471,12 @@ __global__ void __launch_bounds__(kScanThreads) k_band_min(const unsigned* __res
 }
 
 // Pass 2: list the active tiles whose pending value is within the instance's band of its lowest,
-// with their pending value and face bits, and clear them; the others wait for a later round.
-// One list reservation per block.
+// as (tile within its instance, instance, pending value, face bits), and clear them; the others
+// wait for a later round.  One list reservation per block.
 __global__ void __launch_bounds__(kScanThreads) k_band_select(unsigned* __restrict__ flags, unsigned* __restrict__ pend,
                                                             int64_t ntiles, const InstDev* __restrict__ inst,
                                                             int ninst, const unsigned* __restrict__ wmin,
-                                                            int64_t* __restrict__ list, uint2* __restrict__ info,
-                                                            unsigned* __restrict__ count) {
+                                                            uint4* __restrict__ list, unsigned* __restrict__ count) {
     __shared__ unsigned s_wc[kScanThreads / 64];
     __shared__ unsigned s_base;
     const int64_t t = (int64_t)blockIdx.x * kScanThreads + threadIdx.x;
@@ -502,8 +484,9 @@ __global__ void __launch_bounds__(kScanThreads) k_band_select(unsigned* __restri
     const unsigned f = t < ntiles ? flags[t] : 0u;
     bool sel = false;
     unsigned pv = 0x7f800000u;
+    int i = 0;
     if (f) {
-        const int i = find_inst(inst, ninst, t);
+        i = find_inst(inst, ninst, t);
         const float lim = __uint_as_float(wmin[i]) + (float)inst[i].band;
         pv = pend[t];
         sel = __uint_as_float(pv) <= lim;
@@ -523,8 +506,7 @@ __global__ void __launch_bounds__(kScanThreads) k_band_select(unsigned* __restri
     __syncthreads();
     if (sel) {
         const unsigned pos = s_base + s_wc[wave] + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
-        list[pos] = t;
-        info[pos] = make_uint2(pv, f);
+        list[pos] = make_uint4((unsigned)(t - inst[i].tile_base), (unsigned)i, pv, f);
         flags[t] = 0u;
         pend[t] = 0x7f800000u;
     }
@@ -750,14 +732,12 @@ __global__ void k_round_reset(unsigned* __restrict__ wmin, int ninst, unsigned* 
 // the per-instance minimum, selects the tiles within the band, and runs the round kernel on a
 // fixed grid of resident blocks that take tiles from the device-side list.  After a batch the
 // host reads how many tiles are still flagged; none means converged.
-template <int ND, typename T>
+template <int ND, typename T, bool SPH>
 static int fim_run_t(eik_ctx* ctx, const InstDev* d_inst, int ninst, int64_t ntiles, const TTPools& P,
                      const T* slowpool, bool exact, FimStats* fs) {
-    DevBuf& list = ctx->buf("fim_list");
-    DevBuf& linfo = ctx->buf("fim_list_info");
+    DevBuf& list = ctx->buf("fim_tile_list");
     DevBuf& wm = ctx->buf("fim_wmin");
-    if (list.ensure((size_t)ntiles * sizeof(int64_t)) || linfo.ensure((size_t)ntiles * sizeof(uint2)) ||
-        wm.ensure((size_t)ninst * sizeof(unsigned)))
+    if (list.ensure((size_t)ntiles * sizeof(uint4)) || wm.ensure((size_t)ninst * sizeof(unsigned)))
         return ctx->fail(EIK_ENOMEM, "out of device memory (tile list)");
     unsigned* flags = (unsigned*)ctx->buf("fim_tflags").p;
     unsigned* pend = (unsigned*)ctx->buf("fim_pend").p;
@@ -770,13 +750,13 @@ static int fim_run_t(eik_ctx* ctx, const InstDev* d_inst, int ninst, int64_t nti
     static const int grid = [ctx] {
         int ncu = 0, occ = 0;
         if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess) ncu = 1;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_fim_round<ND, T>, kTileThreads, 0) != hipSuccess)
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_fim_round<ND, T, SPH>, kTileThreads, 0) != hipSuccess)
             occ = 1;
         int gr = std::max(1, ncu * std::max(1, occ));
         if (const char* e = getenv("EIK_GRID")) gr = std::max(1, atoi(e));
         if (getenv("EIK_VERBOSE"))
-            fprintf(stderr, "libeik: k_fim_round<%d,%s> %d CUs x %d blocks -> grid %d\n", ND,
-                    sizeof(T) == 8 ? "double" : "float", ncu, occ, gr);
+            fprintf(stderr, "libeik: k_fim_round<%d,%s,%d> %d CUs x %d blocks -> grid %d\n", ND,
+                    sizeof(T) == 8 ? "double" : "float", (int)SPH, ncu, occ, gr);
         return gr;
     }();
     const int g = (int)std::min<int64_t>(grid, ntiles);
@@ -805,12 +785,12 @@ static int fim_run_t(eik_ctx* ctx, const InstDev* d_inst, int ninst, int64_t nti
             k_band_min<<<nblocks(ntiles, kScanThreads), kScanThreads, 0, st>>>(flags, pend, ntiles, d_inst, ninst,
                                                                               (unsigned*)wm.p, d_cnt + kCntFlagged);
             k_band_select<<<nblocks(ntiles, kScanThreads), kScanThreads, 0, st>>>(flags, pend, ntiles, d_inst, ninst,
-                                                               (const unsigned*)wm.p, (int64_t*)list.p,
-                                                               (uint2*)linfo.p, d_cnt + kCntList);
+                                                               (const unsigned*)wm.p, (uint4*)list.p,
+                                                               d_cnt + kCntList);
             CK(hipEventRecord(ctx->kev[2 * r], st));
-            k_fim_round<ND, T><<<g, kTileThreads, 0, st>>>((const int64_t*)list.p, (const uint2*)linfo.p, d_cnt,
-                                                          d_inst, ninst, (const unsigned*)wm.p, (T*)P.tt, (T*)P.k,
-                                                          (T*)P.s, P.d, slowpool, flags, pend, d_c64, settle);
+            k_fim_round<ND, T, SPH><<<g, kTileThreads, 0, st>>>((const uint4*)list.p, d_cnt, d_inst, (const unsigned*)wm.p,
+                                                          (T*)P.tt, (T*)P.k, (T*)P.s, P.d, slowpool, flags, pend,
+                                                          d_c64, settle);
             CK(hipGetLastError());
             CK(hipEventRecord(ctx->kev[2 * r + 1], st));
         }
@@ -837,15 +817,21 @@ static int fim_run_t(eik_ctx* ctx, const InstDev* d_inst, int ninst, int64_t nti
     return EIK_OK;
 }
 
-int fim_run(eik_ctx* ctx, int nd, int prec, const InstDev* d_inst, int ninst, int64_t ntiles, const TTPools& pools,
-            const void* slowpool, bool exact, FimStats* fs) {
+template <int ND, typename T>
+static int fim_run_k(eik_ctx* ctx, bool sph, const InstDev* d_inst, int ninst, int64_t ntiles, const TTPools& P,
+                     const void* slowpool, bool exact, FimStats* fs) {
+    return sph ? fim_run_t<ND, T, true>(ctx, d_inst, ninst, ntiles, P, (const T*)slowpool, exact, fs)
+               : fim_run_t<ND, T, false>(ctx, d_inst, ninst, ntiles, P, (const T*)slowpool, exact, fs);
+}
+
+int fim_run(eik_ctx* ctx, int nd, int prec, int kind, const InstDev* d_inst, int ninst, int64_t ntiles,
+            const TTPools& pools, const void* slowpool, bool exact, FimStats* fs) {
+    const bool sph = kind == EIK_GRID2D_SPHERE || kind == EIK_GRID3D_SPHERE;
     if (nd == 2)
-        return prec == EIK_PREC_FP64
-                   ? fim_run_t<2, double>(ctx, d_inst, ninst, ntiles, pools, (const double*)slowpool, exact, fs)
-                   : fim_run_t<2, float>(ctx, d_inst, ninst, ntiles, pools, (const float*)slowpool, exact, fs);
-    return prec == EIK_PREC_FP64
-               ? fim_run_t<3, double>(ctx, d_inst, ninst, ntiles, pools, (const double*)slowpool, exact, fs)
-               : fim_run_t<3, float>(ctx, d_inst, ninst, ntiles, pools, (const float*)slowpool, exact, fs);
+        return prec == EIK_PREC_FP64 ? fim_run_k<2, double>(ctx, sph, d_inst, ninst, ntiles, pools, slowpool, exact, fs)
+                                     : fim_run_k<2, float>(ctx, sph, d_inst, ninst, ntiles, pools, slowpool, exact, fs);
+    return prec == EIK_PREC_FP64 ? fim_run_k<3, double>(ctx, sph, d_inst, ninst, ntiles, pools, slowpool, exact, fs)
+                                 : fim_run_k<3, float>(ctx, sph, d_inst, ninst, ntiles, pools, slowpool, exact, fs);
 }
 
 // ----------------------------------------------------------------- forward pass of one call
@@ -1018,7 +1004,7 @@ int forward_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, int prec, cons
         CK(hipGetLastError());
         tm.mark();
         FimStats fst{};
-        int rc = fim_run(ctx, nd, EIK_PREC_FP64, (const InstDev*)dfi.p, (int)nsrc, ntiles_f, fpools, slf.p,
+        int rc = fim_run(ctx, nd, EIK_PREC_FP64, kind, (const InstDev*)dfi.p, (int)nsrc, ntiles_f, fpools, slf.p,
                          P->fast == 0, &fst);
         if (rc) return rc;
         ctx->stats[1] = tm.mark();
@@ -1071,7 +1057,8 @@ int forward_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, int prec, cons
     }
     const void* slow = prec == EIK_PREC_FP64 ? (const void*)M->d_slow64 : (const void*)M->d_slow32;
     FimStats cst{};
-    int rc = fim_run(ctx, nd, prec, (const InstDev*)dci.p, (int)nsrc, ntiles_c, cpools, slow, P->fast == 0, &cst);
+    int rc = fim_run(ctx, nd, prec, kind, (const InstDev*)dci.p, (int)nsrc, ntiles_c, cpools, slow, P->fast == 0,
+                     &cst);
     if (rc) return rc;
     ctx->stats[2] = tm.mark();
     ctx->stats[4] = cst.activations;

@@ -406,7 +406,7 @@ struct TTPools {
     void* s;
     uint8_t* d;
 };
-int fim_run(eik_ctx* ctx, int nd, int prec, const InstDev* d_inst, int ninst, int64_t ntiles, const TTPools& pools,
-            const void* slowpool, bool exact, FimStats* fs);
+int fim_run(eik_ctx* ctx, int nd, int prec, int kind, const InstDev* d_inst, int ninst, int64_t ntiles,
+            const TTPools& pools, const void* slowpool, bool exact, FimStats* fs);
 
 }  // namespace eik
