@@ -6,9 +6,13 @@ gradient path; the computation runs in libeik.so (hand-written HIP, include/eik.
 from .grids import (ExtraParams, Grid2DCart, Grid2DSphere, Grid3DCart, Grid3DSphere, GridRefinementPars)
 from .api import (Context, EikModel, default_context, eikgradient, eikttimemisfit, eiktraveltime)
 from .serving import Request, Server
+from . import hmc, autodiff
+from .autodiff import misfit_torch, rrule_eikttimemisfit
+
+HMCtraveltimes = hmc  # the reference's submodule name (src/HMCtraveltimes.jl)
 
 __all__ = [
     "ExtraParams", "GridRefinementPars", "Grid2DCart", "Grid3DCart", "Grid2DSphere", "Grid3DSphere",
     "eiktraveltime", "eikgradient", "eikttimemisfit", "EikModel", "Context", "default_context",
-    "Server", "Request",
+    "Server", "Request", "HMCtraveltimes", "rrule_eikttimemisfit", "misfit_torch",
 ]

@@ -1,0 +1,147 @@
+"""Reverse-mode rule for the misfit and the HMC problem callables, through libeik.
+
+* ``rrule_eikttimemisfit``: the misfit equals ``eikttimemisfit``; the pullback is ``dmisf`` times
+  the ``eikgradient`` outputs; with the taper off, the pulled-back gradient predicts the central
+  difference of the misfit along a smooth joint direction of velocity and source coordinates.
+* ``misfit_torch``: ``backward`` gives the same gradients (both, or only the one required).
+* ``EikonalProbVel`` / ``EikonalProbSrcLoc`` / ``EikonalProbVelAndSrcLoc``: ``:nlogpdf`` is the
+  misfit and ``:gradnlogpdf`` the column-major flattened gradient (HMCtraveltimes.jl:56-112,
+  135-184, 206-283), including the reference's ``1/vel`` factor under ``logVel``; the resident
+  model follows a changed velocity between calls.
+"""
+import numpy as np
+import pytest
+
+from cases import random2d, random3d, smooth_field
+from eikonalsolvers.grids import ExtraParams
+
+pytestmark = pytest.mark.gpu
+
+
+def _problem(eik, case):
+    grd, vel, src, recs = case
+    pert = smooth_field(grd.grsize, 77, lo=-1.0, hi=1.0, base=0.0, amp=1.0)
+    obs = eik.eiktraveltime(vel * (1.0 + 0.03 * pert), grd, src, recs)
+    std = [np.full(len(o), 0.01) for o in obs]
+    return grd, vel, src, recs, obs, std
+
+
+@pytest.fixture(scope="module", params=["2d", "3d"])
+def prob(eik, request):
+    case = random2d(seed=5, n=(60, 48), nsrc=3, nrec=8) if request.param == "2d" else \
+        random3d(seed=6, n=(24, 22, 20), nsrc=2, nrec=6)
+    return _problem(eik, case)
+
+
+def test_rrule_misfit_and_pullback(eik, prob):
+    from eikonalsolvers.autodiff import rrule_eikttimemisfit
+    grd, vel, src, recs, obs, std = prob
+    misf, pb = rrule_eikttimemisfit(vel, grd, src, recs, obs, std)
+    assert misf == pytest.approx(eik.eikttimemisfit(vel, grd, src, recs, obs, std), rel=1e-12)
+    gv, gs, m2 = eik.eikgradient(vel, grd, src, recs, obs, std, "gradvelandsrcloc")
+    dv, ds = pb(2.5)
+    np.testing.assert_array_equal(dv, 2.5 * gv)
+    np.testing.assert_array_equal(ds, 2.5 * gs)
+
+
+@pytest.mark.parametrize("part", ["vel", "src"])
+def test_rrule_predicts_directional_difference(eik, prob, part):
+    """Velocity along a smooth field (1e-2 relative), source coordinates along a random direction
+    (5e-2, the bound the C3 finite-difference check uses: the misfit of a heap FMM is only
+    piecewise smooth in the source position)."""
+    from eikonalsolvers.autodiff import rrule_eikttimemisfit
+    grd, vel, src, recs, obs, std = prob
+    ep = ExtraParams(radiussmoothgradsrc=0)
+    _, pb = rrule_eikttimemisfit(vel, grd, src, recs, obs, std, extraparams=ep)
+    gv, gs = pb(1.0)
+    dvel = smooth_field(grd.grsize, 91, lo=-1.0, hi=1.0, base=0.0, amp=0.5) if part == "vel" else 0.0 * vel
+    dsrc = np.random.default_rng(92).uniform(-1.0, 1.0, size=src.shape) if part == "src" else 0.0 * src
+    pred = float(np.sum(gv * dvel) + np.sum(gs * dsrc))
+    h = 1e-4
+    fp = eik.eikttimemisfit(vel + h * dvel, grd, src + h * dsrc, recs, obs, std, extraparams=ep)
+    fm = eik.eikttimemisfit(vel - h * dvel, grd, src - h * dsrc, recs, obs, std, extraparams=ep)
+    fd = (fp - fm) / (2 * h)
+    rel = abs(pred - fd) / abs(fd)
+    print(f"directional derivative ({part}): adjoint {pred:.8e}, central difference {fd:.8e}, rel {rel:.2e}")
+    assert rel <= (1e-2 if part == "vel" else 5e-2)
+
+
+def test_misfit_torch_backward(eik, prob):
+    import torch
+    from eikonalsolvers.autodiff import misfit_torch
+    grd, vel, src, recs, obs, std = prob
+    gv, gs, misf = eik.eikgradient(vel, grd, src, recs, obs, std, "gradvelandsrcloc")
+    m = eik.EikModel(vel, grd)
+    v = torch.tensor(vel, dtype=torch.float64, requires_grad=True)
+    c = torch.tensor(src, dtype=torch.float64, requires_grad=True)
+    f = misfit_torch(v, c, m, recs, obs, std)
+    assert float(f) == pytest.approx(misf, rel=1e-12)
+    (3.0 * f).backward()
+    np.testing.assert_allclose(v.grad.numpy(), 3.0 * gv, rtol=1e-12, atol=0)
+    np.testing.assert_allclose(c.grad.numpy(), 3.0 * gs, rtol=1e-12, atol=0)
+    v2 = torch.tensor(vel, dtype=torch.float64, requires_grad=True)
+    f2 = misfit_torch(v2, torch.tensor(src, dtype=torch.float64), m, recs, obs, std)
+    f2.backward()
+    gv1, _ = eik.eikgradient(vel, grd, src, recs, obs, std, "gradvel")
+    np.testing.assert_allclose(v2.grad.numpy(), gv1, rtol=1e-12, atol=0)
+    m.close()
+
+
+def test_hmc_velocity_problem(eik, prob):
+    from eikonalsolvers.hmc import EikonalProbVel
+    grd, vel, src, recs, obs, std = prob
+    P = EikonalProbVel(grd=grd, dobs=obs, stdobs=std, coordsrc=src, coordrec=recs, whichgrad=":gradvel")
+    x = vel.ravel(order="F")
+    misf = eik.eikttimemisfit(vel, grd, src, recs, obs, std)
+    gv, _ = eik.eikgradient(vel, grd, src, recs, obs, std, "gradvel")
+    assert P(x, ":nlogpdf") == pytest.approx(misf, rel=1e-12)
+    np.testing.assert_allclose(P(x, ":gradnlogpdf"), gv.ravel(order="F"), rtol=1e-12, atol=0)
+    m2, g2 = P.nlogpdf_and_grad(x)
+    assert m2 == pytest.approx(misf, rel=1e-12)
+    np.testing.assert_allclose(g2, gv.ravel(order="F"), rtol=1e-12, atol=0)
+    # another velocity: the resident model is updated
+    vel2 = vel * 1.01
+    assert P(vel2.ravel(order="F"), "nlogpdf") == pytest.approx(eik.eikttimemisfit(vel2, grd, src, recs, obs, std),
+                                                               rel=1e-12)
+    # logVel: x = log(vel); the gradient carries the reference's 1/vel factor
+    PL = EikonalProbVel(grd=grd, dobs=obs, stdobs=std, coordsrc=src, coordrec=recs, whichgrad="gradvel", logVel=True)
+    assert PL(np.log(x), "nlogpdf") == pytest.approx(misf, rel=1e-9)
+    np.testing.assert_allclose(PL(np.log(x), "gradnlogpdf"), (gv / vel).ravel(order="F"), rtol=1e-7, atol=1e-300)
+    P.close()
+    PL.close()
+
+
+def test_hmc_source_problem(eik, prob):
+    from eikonalsolvers.hmc import EikonalProbSrcLoc
+    grd, vel, src, recs, obs, std = prob
+    P = EikonalProbSrcLoc(grd=grd, dobs=obs, stdobs=std, coordrec=recs, velmod=vel, whichgrad="gradsrcloc")
+    x = src.ravel(order="F")
+    gs, misf = eik.eikgradient(vel, grd, src, recs, obs, std, "gradsrcloc")
+    assert P(x, "nlogpdf") == pytest.approx(misf, rel=1e-12)
+    np.testing.assert_allclose(P(x, "gradnlogpdf"), gs.ravel(order="F"), rtol=1e-12, atol=0)
+    src2 = src + 0.37
+    gs2, misf2 = eik.eikgradient(vel, grd, src2, recs, obs, std, "gradsrcloc")
+    m3, g3 = P.nlogpdf_and_grad(src2.ravel(order="F"))
+    assert m3 == pytest.approx(misf2, rel=1e-12)
+    np.testing.assert_allclose(g3, gs2.ravel(order="F"), rtol=1e-12, atol=0)
+    P.close()
+
+
+def test_hmc_joint_problem(eik, prob):
+    from eikonalsolvers.hmc import EikonalProbVelAndSrcLoc
+    grd, vel, src, recs, obs, std = prob
+    P = EikonalProbVelAndSrcLoc(grd=grd, dobs=obs, stdobs=std, coordrec=recs, whichgrad="gradvelandsrcloc")
+    x = np.concatenate([vel.ravel(order="F"), src.ravel(order="F")])
+    gv, gs, misf = eik.eikgradient(vel, grd, src, recs, obs, std, "gradvelandsrcloc")
+    assert P(x, "nlogpdf") == pytest.approx(misf, rel=1e-12)
+    np.testing.assert_allclose(P(x, "gradnlogpdf"), np.concatenate([gv.ravel(order="F"), gs.ravel(order="F")]),
+                               rtol=1e-12, atol=0)
+    PL = EikonalProbVelAndSrcLoc(grd=grd, dobs=obs, stdobs=std, coordrec=recs, whichgrad="gradvelandsrcloc",
+                                 logVel=True)
+    xl = np.concatenate([np.log(vel.ravel(order="F")), src.ravel(order="F")])
+    ml, gl = PL.nlogpdf_and_grad(xl)
+    assert ml == pytest.approx(misf, rel=1e-9)
+    np.testing.assert_allclose(gl, np.concatenate([(gv / vel).ravel(order="F"), gs.ravel(order="F")]),
+                               rtol=1e-7, atol=1e-300)
+    P.close()
+    PL.close()
