@@ -205,7 +205,7 @@ __global__ void __launch_bounds__(kAB) k_adj_round(const int64_t* __restrict__ l
                                                    int ninst, const uint16_t* __restrict__ codep,
                                                    const double* __restrict__ Gp, int64_t gstride,
                                                    const double* __restrict__ bp, double* __restrict__ lamp,
-                                                   uint8_t* __restrict__ flags) {
+                                                   uint8_t* __restrict__ flags, unsigned* __restrict__ nsweeps) {
 #pragma clang fp contract(off)
     using S = AdjTile<ND>;
     constexpr int NL = S::LX * S::LY * S::LZ;
@@ -334,14 +334,17 @@ __global__ void __launch_bounds__(kAB) k_adj_round(const int64_t* __restrict__ l
         if (v != l0[p]) {
             const int gx = gxyz[p][0], gy = gxyz[p][1], gz = gxyz[p][2];
             lam[gx + nx * gy + nxy * gz] = v;
-            const int lx = gx - ox, ly = gy - oy, lz = gz - oz;
-            if (lx < 2) faces |= 1;
-            if (lx >= S::TX - 2) faces |= 2;
-            if (ly < 2) faces |= 4;
-            if (ly >= S::TY - 2) faces |= 8;
-            if (ND == 3) {
-                if (lz < 2) faces |= 16;
-                if (lz >= S::TZ - 2) faces |= 32;
+            // a changed row only feeds the nodes its own stencil reads (code c along an axis: the
+            // |c| nodes on the side opposite to the sign of c), so a neighbour tile is woken only
+            // when that stencil crosses into it
+            const int lc[3] = {gx - ox, gy - oy, gz - oz};
+            const int te[3] = {S::TX, S::TY, S::TZ};
+            const uint16_t cr = s_code[lidx[p]];
+#pragma unroll
+            for (int d = 0; d < ND; d++) {
+                const int c = unpack_code(cr, d);
+                if (c > 0 && lc[d] < c) faces |= 1 << (2 * d);
+                if (c < 0 && lc[d] - c >= te[d]) faces |= 2 << (2 * d);
             }
         }
     }
@@ -360,6 +363,7 @@ __global__ void __launch_bounds__(kAB) k_adj_round(const int64_t* __restrict__ l
             if ((f & 32) && tiz < I.at[2] - 1) fl[tl + t01] = 1;
         }
         if (it >= S::MAXIT) fl[tl] = 1;
+        atomicAdd(nsweeps, (unsigned)it);
     }
 }
 
@@ -757,17 +761,21 @@ static int adj_run_t(eik_ctx* ctx, const InstDev* d_inst, int ninst, int64_t nti
     unsigned* d_cnt = (unsigned*)ctx->d_small;
     unsigned* h_cnt = (unsigned*)ctx->h_small;
     FimStats st{};
+    CK(hipMemsetAsync(d_cnt + 1, 0, sizeof(unsigned), ctx->stream));
     for (;;) {
         CK(hipMemsetAsync(d_cnt, 0, sizeof(unsigned), ctx->stream));
         k_compact<<<nb(ntiles, 256), 256, 0, ctx->stream>>>(flags, ntiles, (int64_t*)list.p, d_cnt);
-        CK(hipMemcpyAsync(h_cnt, d_cnt, sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
+        CK(hipMemcpyAsync(h_cnt, d_cnt, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
         CK(hipStreamSynchronize(ctx->stream));
         const unsigned nact = h_cnt[0];
-        if (nact == 0) break;
+        if (nact == 0) {
+            st.sweeps = (double)h_cnt[1];
+            break;
+        }
         if (++st.rounds > 50000) return ctx->fail(EIK_ENOCONV, "adjoint solve did not converge");
         st.activations += nact;
         k_adj_round<ND><<<nact, kAB, 0, ctx->stream>>>((const int64_t*)list.p, d_inst, ninst, code, G, gstride, b, lam,
-                                                        flags);
+                                                        flags, d_cnt + 1);
         CK(hipGetLastError());
     }
     if (fs) *fs = st;
@@ -885,6 +893,10 @@ int gradient_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, const eik_par
     rc = adj_run(ctx, nd, dci, (int)nsrc, nsrc * tiles_per, (const uint16_t*)code_c.p, (const double*)G_c.p, P_c,
                  (const double*)b_c.p, (double*)lam_c.p, &ast);
     if (rc) return rc;
+    if (getenv("EIK_VERBOSE"))
+        fprintf(stderr, "libeik: adjoint %lld rounds, %.0f tile activations of %lld tiles, %.2f sweeps per activation\n",
+                (long long)ast.rounds, ast.activations, (long long)(nsrc * tiles_per),
+                ast.sweeps / std::max(1.0, ast.activations));
 
     // coarse source points, grouped by source
     std::vector<SrcPt> pts;

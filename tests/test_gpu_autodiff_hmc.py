@@ -46,24 +46,30 @@ def test_rrule_misfit_and_pullback(eik, prob):
 
 @pytest.mark.parametrize("part", ["vel", "src"])
 def test_rrule_predicts_directional_difference(eik, prob, part):
-    """Velocity along a smooth field (1e-2 relative), source coordinates along a random direction
-    (5e-2, the bound the C3 finite-difference check uses: the misfit of a heap FMM is only
-    piecewise smooth in the source position)."""
+    """The pulled-back gradient against the central difference of the misfit along a smooth
+    velocity field or a random source displacement, without refinement and taper, h = 1e-6 (the
+    misfit of a 2nd-order heap FMM jumps where an acceptance order changes; larger steps meet
+    such jumps).  In 2D the reference's adjoint is the derivative of its misfit (measured with
+    the oracle: 1e-7 / 5e-7 relative); in 3D it differs by 2.8e-3 / 2.9e-3 on this case, in the
+    oracle as well, so that is the reference's 3D adjoint and is reproduced.  With refinement the reference's velocity
+    gradient is further from the difference (0.29 in 2D, 6e-3 in 3D, oracle and GPU alike), so
+    these checks run without it; parity with the oracle is checked in test_gpu_gradient.py."""
     from eikonalsolvers.autodiff import rrule_eikttimemisfit
     grd, vel, src, recs, obs, std = prob
-    ep = ExtraParams(radiussmoothgradsrc=0)
+    ep = ExtraParams(radiussmoothgradsrc=0, refinearoundsrc=False)
     _, pb = rrule_eikttimemisfit(vel, grd, src, recs, obs, std, extraparams=ep)
     gv, gs = pb(1.0)
     dvel = smooth_field(grd.grsize, 91, lo=-1.0, hi=1.0, base=0.0, amp=0.5) if part == "vel" else 0.0 * vel
     dsrc = np.random.default_rng(92).uniform(-1.0, 1.0, size=src.shape) if part == "src" else 0.0 * src
     pred = float(np.sum(gv * dvel) + np.sum(gs * dsrc))
-    h = 1e-4
+    h = 1e-6
     fp = eik.eikttimemisfit(vel + h * dvel, grd, src + h * dsrc, recs, obs, std, extraparams=ep)
     fm = eik.eikttimemisfit(vel - h * dvel, grd, src - h * dsrc, recs, obs, std, extraparams=ep)
     fd = (fp - fm) / (2 * h)
     rel = abs(pred - fd) / abs(fd)
-    print(f"directional derivative ({part}): adjoint {pred:.8e}, central difference {fd:.8e}, rel {rel:.2e}")
-    assert rel <= (1e-2 if part == "vel" else 5e-2)
+    print(f"directional derivative ({part}, {grd.ndim}D): adjoint {pred:.8e}, central difference {fd:.8e}, "
+          f"rel {rel:.2e}")
+    assert rel <= (1e-5 if grd.ndim == 2 else 1e-2)
 
 
 def test_misfit_torch_backward(eik, prob):
