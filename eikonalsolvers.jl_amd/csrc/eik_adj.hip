@@ -199,20 +199,22 @@ struct AdjTile<3> {
 };
 static_assert(AdjTile<2>::TX == adj_tile_edge(2) && AdjTile<3>::TX == adj_tile_edge(3), "adjoint tile edge");
 
-// one adjoint round over the listed tiles
-template <int ND>
+// one adjoint round over the listed tiles; a neighbour woken by a changed row gets that row's
+// traveltime (rounded up) as its pending value
+template <int ND, typename T>
 __global__ void __launch_bounds__(kAB) k_adj_round(const int64_t* __restrict__ list, const InstDev* __restrict__ inst,
                                                    int ninst, const uint16_t* __restrict__ codep,
                                                    const double* __restrict__ Gp, int64_t gstride,
                                                    const double* __restrict__ bp, double* __restrict__ lamp,
-                                                   uint8_t* __restrict__ flags, unsigned* __restrict__ nsweeps) {
+                                                   const T* __restrict__ ttpool, uint8_t* __restrict__ flags,
+                                                   unsigned* __restrict__ apend, unsigned* __restrict__ nsweeps) {
 #pragma clang fp contract(off)
     using S = AdjTile<ND>;
     constexpr int NL = S::LX * S::LY * S::LZ;
     __shared__ double s_lam[NL];
     __shared__ double s_G[ND][NL];
     __shared__ uint16_t s_code[NL];
-    __shared__ int s_faces;
+    __shared__ unsigned s_fmax[7];  // per face (and whole tile): largest traveltime of a changed row
     const int tid = threadIdx.x;
     const int64_t gid = list[blockIdx.x];
     int lo = 0, hi = ninst - 1;
@@ -232,7 +234,7 @@ __global__ void __launch_bounds__(kAB) k_adj_round(const int64_t* __restrict__ l
     const uint16_t* code = codep + I.tt_off;
     const double* b = bp + I.tt_off;
     double* lam = lamp + I.tt_off;
-    if (tid == 0) s_faces = 0;
+    if (tid < 7) s_fmax[tid] = 0u;
     for (int c = tid; c < NL; c += kAB) {
         const int lx = c % S::LX, ly = (c / S::LX) % S::LY, lz = ND == 3 ? c / (S::LX * S::LY) : 0;
         int nout = (lx < S::H || lx >= S::H + S::TX) + (ly < S::H || ly >= S::H + S::TY);
@@ -326,14 +328,16 @@ __global__ void __launch_bounds__(kAB) k_adj_round(const int64_t* __restrict__ l
         }
         if (!__syncthreads_or(any)) break;
     }
-    int faces = 0;
 #pragma unroll
     for (int p = 0; p < S::NP; p++) {
         if (!act[p]) continue;
         const double v = s_lam[lidx[p]];
         if (v != l0[p]) {
             const int gx = gxyz[p][0], gy = gxyz[p][1], gz = gxyz[p][2];
-            lam[gx + nx * gy + nxy * gz] = v;
+            const int64_t l = gx + nx * gy + nxy * gz;
+            lam[l] = v;
+            const unsigned tv = __float_as_uint(__double2float_ru(fabs((double)ttpool[I.tt_off + l])));
+            atomicMax(&s_fmax[6], tv);
             // a changed row only feeds the nodes its own stencil reads (code c along an axis: the
             // |c| nodes on the side opposite to the sign of c), so a neighbour tile is woken only
             // when that stencil crosses into it
@@ -343,27 +347,75 @@ __global__ void __launch_bounds__(kAB) k_adj_round(const int64_t* __restrict__ l
 #pragma unroll
             for (int d = 0; d < ND; d++) {
                 const int c = unpack_code(cr, d);
-                if (c > 0 && lc[d] < c) faces |= 1 << (2 * d);
-                if (c < 0 && lc[d] - c >= te[d]) faces |= 2 << (2 * d);
+                if (c > 0 && lc[d] < c) atomicMax(&s_fmax[2 * d], tv);
+                if (c < 0 && lc[d] - c >= te[d]) atomicMax(&s_fmax[2 * d + 1], tv);
             }
         }
     }
-    if (faces) atomicOr(&s_faces, faces);
     __syncthreads();
     if (tid == 0) {
-        const int f = s_faces;
         uint8_t* fl = flags + I.atile_base;
+        unsigned* pd = apend + I.atile_base;
         const int64_t t0 = I.at[0], t01 = (int64_t)I.at[0] * I.at[1];
-        if ((f & 1) && tix > 0) fl[tl - 1] = 1;
-        if ((f & 2) && tix < I.at[0] - 1) fl[tl + 1] = 1;
-        if ((f & 4) && tiy > 0) fl[tl - t0] = 1;
-        if ((f & 8) && tiy < I.at[1] - 1) fl[tl + t0] = 1;
+        auto wake = [&](int64_t t, unsigned v) {
+            fl[t] = 1;
+            atomicMax(&pd[t], v);
+        };
+        if (s_fmax[0] && tix > 0) wake(tl - 1, s_fmax[0]);
+        if (s_fmax[1] && tix < I.at[0] - 1) wake(tl + 1, s_fmax[1]);
+        if (s_fmax[2] && tiy > 0) wake(tl - t0, s_fmax[2]);
+        if (s_fmax[3] && tiy < I.at[1] - 1) wake(tl + t0, s_fmax[3]);
         if (ND == 3) {
-            if ((f & 16) && tiz > 0) fl[tl - t01] = 1;
-            if ((f & 32) && tiz < I.at[2] - 1) fl[tl + t01] = 1;
+            if (s_fmax[4] && tiz > 0) wake(tl - t01, s_fmax[4]);
+            if (s_fmax[5] && tiz < I.at[2] - 1) wake(tl + t01, s_fmax[5]);
         }
-        if (it >= S::MAXIT) fl[tl] = 1;
+        if (it >= S::MAXIT) wake(tl, s_fmax[6]);
         atomicAdd(nsweeps, (unsigned)it);
+    }
+}
+
+// Band scheduling of the adjoint rounds (late traveltimes first).  Pass 1: highest pending value
+// of the flagged tiles per instance.
+__device__ __forceinline__ int find_ainst(const InstDev* __restrict__ inst, int ninst, int64_t gid) {
+    int lo = 0, hi = ninst - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (inst[mid].atile_base <= gid)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void k_adj_band_max(const uint8_t* __restrict__ flags, const unsigned* __restrict__ apend, int64_t ntiles,
+                               const InstDev* __restrict__ inst, int ninst, unsigned* __restrict__ wmax) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntiles || !flags[t]) return;
+    atomicMax(&wmax[find_ainst(inst, ninst, t)], apend[t]);
+}
+
+// Pass 2: list the flagged tiles within the band below their instance's highest pending value and
+// clear them; one list reservation per wave.
+__global__ void k_adj_band_select(uint8_t* __restrict__ flags, unsigned* __restrict__ apend, int64_t ntiles,
+                                  const InstDev* __restrict__ inst, int ninst, const unsigned* __restrict__ wmax,
+                                  int64_t* __restrict__ list, unsigned* __restrict__ count) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool sel = false;
+    if (t < ntiles && flags[t]) {
+        const int i = find_ainst(inst, ninst, t);
+        const float lim = __uint_as_float(wmax[i]) - (float)inst[i].aband;
+        sel = __uint_as_float(apend[t]) >= lim;
+    }
+    const unsigned long long m = __ballot(sel);
+    const int lane = threadIdx.x & 63;
+    unsigned base = 0u;
+    if (lane == 0 && m) base = atomicAdd(count, (unsigned)__popcll(m));
+    base = __shfl(base, 0);
+    if (sel) {
+        list[base + (unsigned)__popcll(m & ((1ull << lane) - 1ull))] = t;
+        flags[t] = 0;
+        apend[t] = 0u;
     }
 }
 
@@ -392,7 +444,7 @@ template <typename T>
 __global__ void k_adj_rhs(const double* __restrict__ rec, const int* __restrict__ rsrc, int64_t nrec,
                           const InstDev* __restrict__ inst, const T* __restrict__ ttpool, int nd, int kind,
                           const double* __restrict__ pobs, const double* __restrict__ sobs, double* __restrict__ bpool,
-                          double* __restrict__ mterm, uint8_t* __restrict__ flags) {
+                          double* __restrict__ mterm, uint8_t* __restrict__ flags, unsigned* __restrict__ apend) {
 #pragma clang fp contract(off)
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nrec) return;
@@ -409,7 +461,7 @@ __global__ void k_adj_rhs(const double* __restrict__ rec, const int* __restrict_
         atomicAdd(&bpool[I.tt_off + idx[c]], -cf[c] * fact);
         const int64_t l = idx[c];
         const int g[3] = {(int)(l % I.n[0]), (int)((l / I.n[0]) % I.n[1]), (int)(l / ((int64_t)I.n[0] * I.n[1]))};
-        seed_tiles(I, nd, g, flags);
+        seed_tiles(I, nd, g, flags, apend, (double)ttpool[I.tt_off + l]);
     }
 }
 
@@ -498,7 +550,7 @@ __global__ void k_src_own(const SrcPt* __restrict__ pts, int npts, const InstDev
 __global__ void k_fine_rhs(const SrcPt* __restrict__ xp, int nxp, const InstDev* __restrict__ finst,
                            const double* __restrict__ ttf, const double* __restrict__ dpsiup,
                            const double* __restrict__ dpsidus, double* __restrict__ bf, double* __restrict__ haq,
-                           uint8_t* __restrict__ flags, int nd) {
+                           uint8_t* __restrict__ flags, unsigned* __restrict__ apend, int nd) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nxp) return;
     const SrcPt P = xp[q];
@@ -510,7 +562,7 @@ __global__ void k_fine_rhs(const SrcPt* __restrict__ xp, int nxp, const InstDev*
     } else {
         bf[I.tt_off + lf] = -qv;
         const int g[3] = {(int)(lf % I.n[0]), (int)((lf / I.n[0]) % I.n[1]), (int)(lf / ((int64_t)I.n[0] * I.n[1]))};
-        seed_tiles(I, nd, g, flags);
+        seed_tiles(I, nd, g, flags, apend, ttf[I.tt_off + lf]);
     }
 }
 
@@ -752,19 +804,25 @@ __global__ void k_gradsrc(int nsrc, int nd, const SrcPt* __restrict__ pts, const
 
 static unsigned nb(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
-template <int ND>
+template <int ND, typename T>
 static int adj_run_t(eik_ctx* ctx, const InstDev* d_inst, int ninst, int64_t ntiles, const uint16_t* code,
-                     const double* G, int64_t gstride, const double* b, double* lam, FimStats* fs) {
+                     const double* G, int64_t gstride, const double* b, double* lam, const T* ttpool, FimStats* fs) {
     DevBuf& list = ctx->buf("fim_list");
-    if (list.ensure((size_t)ntiles * sizeof(int64_t))) return ctx->fail(EIK_ENOMEM, "out of device memory (tile list)");
+    DevBuf& wm = ctx->buf("adj_wmax");
+    if (list.ensure((size_t)ntiles * sizeof(int64_t)) || wm.ensure((size_t)ninst * sizeof(unsigned)))
+        return ctx->fail(EIK_ENOMEM, "out of device memory (tile list)");
     uint8_t* flags = (uint8_t*)ctx->buf("fim_flags").p;
+    unsigned* apend = (unsigned*)ctx->buf("adj_pend").p;
     unsigned* d_cnt = (unsigned*)ctx->d_small;
     unsigned* h_cnt = (unsigned*)ctx->h_small;
     FimStats st{};
     CK(hipMemsetAsync(d_cnt + 1, 0, sizeof(unsigned), ctx->stream));
     for (;;) {
         CK(hipMemsetAsync(d_cnt, 0, sizeof(unsigned), ctx->stream));
-        k_compact<<<nb(ntiles, 256), 256, 0, ctx->stream>>>(flags, ntiles, (int64_t*)list.p, d_cnt);
+        k_fill_u32<<<nb(ninst, 256), 256, 0, ctx->stream>>>((unsigned*)wm.p, ninst, 0u);
+        k_adj_band_max<<<nb(ntiles, 256), 256, 0, ctx->stream>>>(flags, apend, ntiles, d_inst, ninst, (unsigned*)wm.p);
+        k_adj_band_select<<<nb(ntiles, 256), 256, 0, ctx->stream>>>(flags, apend, ntiles, d_inst, ninst,
+                                                                   (const unsigned*)wm.p, (int64_t*)list.p, d_cnt);
         CK(hipMemcpyAsync(h_cnt, d_cnt, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
         CK(hipStreamSynchronize(ctx->stream));
         const unsigned nact = h_cnt[0];
@@ -774,18 +832,26 @@ static int adj_run_t(eik_ctx* ctx, const InstDev* d_inst, int ninst, int64_t nti
         }
         if (++st.rounds > 50000) return ctx->fail(EIK_ENOCONV, "adjoint solve did not converge");
         st.activations += nact;
-        k_adj_round<ND><<<nact, kAB, 0, ctx->stream>>>((const int64_t*)list.p, d_inst, ninst, code, G, gstride, b, lam,
-                                                        flags, d_cnt + 1);
+        k_adj_round<ND, T><<<nact, kAB, 0, ctx->stream>>>((const int64_t*)list.p, d_inst, ninst, code, G, gstride, b,
+                                                           lam, ttpool, flags, apend, d_cnt + 1);
         CK(hipGetLastError());
     }
     if (fs) *fs = st;
     return EIK_OK;
 }
 
-static int adj_run(eik_ctx* ctx, int nd, const InstDev* d_inst, int ninst, int64_t ntiles, const uint16_t* code,
-                   const double* G, int64_t gstride, const double* b, double* lam, FimStats* fs) {
-    return nd == 2 ? adj_run_t<2>(ctx, d_inst, ninst, ntiles, code, G, gstride, b, lam, fs)
-                   : adj_run_t<3>(ctx, d_inst, ninst, ntiles, code, G, gstride, b, lam, fs);
+// ttpool: the traveltime pool of the instances (FP64, or FP32 for the coarse grids in mixed
+// precision); it orders the rounds
+static int adj_run(eik_ctx* ctx, int nd, int prec, const InstDev* d_inst, int ninst, int64_t ntiles,
+                   const uint16_t* code, const double* G, int64_t gstride, const double* b, double* lam,
+                   const void* ttpool, FimStats* fs) {
+    if (prec == EIK_PREC_FP64)
+        return nd == 2 ? adj_run_t<2, double>(ctx, d_inst, ninst, ntiles, code, G, gstride, b, lam,
+                                              (const double*)ttpool, fs)
+                       : adj_run_t<3, double>(ctx, d_inst, ninst, ntiles, code, G, gstride, b, lam,
+                                              (const double*)ttpool, fs);
+    return nd == 2 ? adj_run_t<2, float>(ctx, d_inst, ninst, ntiles, code, G, gstride, b, lam, (const float*)ttpool, fs)
+                   : adj_run_t<3, float>(ctx, d_inst, ninst, ntiles, code, G, gstride, b, lam, (const float*)ttpool, fs);
 }
 
 template <typename T>
@@ -874,24 +940,25 @@ int gradient_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, const eik_par
         CK(hipMemcpyAsync(dso.p, stdobs, (size_t)nrec * sizeof(double), hipMemcpyHostToDevice, st));
     }
     CK(hipMemcpyAsync(droff.p, S.rec_off.data(), (size_t)(nsrc + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    unsigned* apend = (unsigned*)ctx->buf("adj_pend").p;
     uint8_t* flags = (uint8_t*)ctx->buf("fim_flags").p;
     if (nrec > 0) {
         if (prec == EIK_PREC_FP64)
             k_adj_rhs<double><<<nb(nrec, 128), 128, 0, st>>>((const double*)drec.p, (const int*)drs.p, nrec, dci,
                                                              (const double*)ttc, nd, kind, (const double*)dpo.p,
-                                                             (const double*)dso.p, (double*)b_c.p, (double*)dmt.p, flags);
+                                                             (const double*)dso.p, (double*)b_c.p, (double*)dmt.p, flags, apend);
         else
             k_adj_rhs<float><<<nb(nrec, 128), 128, 0, st>>>((const double*)drec.p, (const int*)drs.p, nrec, dci,
                                                             (const float*)ttc, nd, kind, (const double*)dpo.p,
-                                                            (const double*)dso.p, (double*)b_c.p, (double*)dmt.p, flags);
+                                                            (const double*)dso.p, (double*)b_c.p, (double*)dmt.p, flags, apend);
         CK(hipGetLastError());
     }
     k_misfit<<<1, 1, 0, st>>>((const double*)dmt.p, (const int64_t*)droff.p, nsrc, (double*)dmis.p);
     CK(hipMemcpyAsync(misf, dmis.p, sizeof(double), hipMemcpyDeviceToHost, st));
 
     FimStats ast{};
-    rc = adj_run(ctx, nd, dci, (int)nsrc, nsrc * tiles_per, (const uint16_t*)code_c.p, (const double*)G_c.p, P_c,
-                 (const double*)b_c.p, (double*)lam_c.p, &ast);
+    rc = adj_run(ctx, nd, prec, dci, (int)nsrc, nsrc * tiles_per, (const uint16_t*)code_c.p, (const double*)G_c.p,
+                 P_c, (const double*)b_c.p, (double*)lam_c.p, ttc, &ast);
     if (rc) return rc;
     if (getenv("EIK_VERBOSE"))
         fprintf(stderr, "libeik: adjoint %lld rounds, %.0f tile activations of %lld tiles, %.2f sweeps per activation\n",
@@ -989,10 +1056,11 @@ int gradient_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, const eik_par
         launch_prep<double>(nd, st, dfi, (int)nsrc, fpools, slf, (uint16_t*)code_f.p, (double*)G_f.p, P_f);
         if (npts > 0)
             k_fine_rhs<<<nb(npts, 64), 64, 0, st>>>((const SrcPt*)dpts.p, npts, dfi, ttf, (const double*)dup.p,
-                                                    (const double*)dus.p, (double*)b_f.p, (double*)haq.p, flags, nd);
+                                                    (const double*)dus.p, (double*)b_f.p, (double*)haq.p, flags, apend,
+                                                    nd);
         CK(hipGetLastError());
-        rc = adj_run(ctx, nd, dfi, (int)nsrc, ntiles_f, (const uint16_t*)code_f.p, (const double*)G_f.p, P_f,
-                     (const double*)b_f.p, (double*)lam_f.p, &fst);
+        rc = adj_run(ctx, nd, EIK_PREC_FP64, dfi, (int)nsrc, ntiles_f, (const uint16_t*)code_f.p, (const double*)G_f.p,
+                     P_f, (const double*)b_f.p, (double*)lam_f.p, ttf, &fst);
         if (rc) return rc;
         if (nfp > 0) {
             if (nd == 2)

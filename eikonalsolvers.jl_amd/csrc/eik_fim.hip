@@ -857,10 +857,21 @@ static double nodeband_mult() {
     return m;
 }
 
+// EIK_ADJ_BAND: the same for the adjoint rounds, which run from late to early traveltimes
+// (0 or less: every flagged tile runs).
+static double adjband_mult() {
+    static const double m = [] {
+        const char* e = getenv("EIK_ADJ_BAND");
+        return e ? atof(e) : 8.0;
+    }();
+    return m;
+}
+
 static void set_band(InstDev& I, double smean) {
-    const double m = band_mult(), mn = nodeband_mult();
+    const double m = band_mult(), mn = nodeband_mult(), ma = adjband_mult();
     I.band = m > 0.0 ? m * I.h[0] * smean : INFINITY;
     I.nband = mn > 0.0 ? mn * I.h[0] * smean : INFINITY;
+    I.aband = ma > 0.0 ? ma * I.h[0] * smean : INFINITY;
 }
 
 // distance from a source-box corner to the source, on the host (sourceboxloctt!,
@@ -949,8 +960,11 @@ int forward_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, int prec, cons
         }
     }
     DevBuf& aflags = ctx->buf("fim_flags");  // adjoint rounds, one byte per adjoint tile
-    if (aflags.ensure((size_t)maxat)) return ctx->fail(EIK_ENOMEM, "out of device memory (tile flags)");
+    DevBuf& apend = ctx->buf("adj_pend");    // and its pending traveltime
+    if (aflags.ensure((size_t)maxat) || apend.ensure((size_t)maxat * sizeof(unsigned)))
+        return ctx->fail(EIK_ENOMEM, "out of device memory (tile flags)");
     CK(hipMemsetAsync(aflags.p, 0, (size_t)maxat, st));
+    CK(hipMemsetAsync(apend.p, 0, (size_t)maxat * sizeof(unsigned), st));
     DevBuf& flags = ctx->buf("fim_tflags");
     if (flags.ensure((size_t)maxt * sizeof(unsigned))) return ctx->fail(EIK_ENOMEM, "out of device memory (tile flags)");
     CK(hipMemsetAsync(flags.p, 0, (size_t)maxt * sizeof(unsigned), st));

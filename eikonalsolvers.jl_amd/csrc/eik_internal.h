@@ -43,6 +43,7 @@ struct InstDev {
     const double* sinth; // 3D sphere: sin(theta_j) per theta index, computed on the host (else null)
     double band;         // band scheduler: tiles within this traveltime of the lowest pending one run
     double nband;        // the same for the nodes updated inside a tile
+    double aband;        // adjoint rounds: tiles within this traveltime of the highest pending one run
 };
 
 // Refined box around one source (createfinegrid, src/EikSolv/eikmainfwd.jl:950-1104)
@@ -121,21 +122,32 @@ __host__ __device__ constexpr int adj_tile_edge(int nd) { return nd == 2 ? 16 : 
 
 // mark the adjoint tile of node g and the face-neighbour tiles whose stencil reaches it (one
 // byte per tile)
-__device__ __forceinline__ void seed_tiles(const InstDev& I, int nd, const int* g, uint8_t* flags) {
+// Adjoint tile flags (one byte per tile) and pending values (float bits of the largest traveltime
+// of a node whose right-hand side or lambda changed, rounded up: the adjoint runs from late to
+// early traveltimes).  Seeding node g (traveltime t) flags its tile and the face neighbours within
+// reach of its stencil.
+__device__ __forceinline__ void seed_tiles(const InstDev& I, int nd, const int* g, uint8_t* flags, unsigned* apend,
+                                           double t) {
     const int TX = adj_tile_edge(nd), TY = TX, TZ = nd == 2 ? 1 : TX;
     const int tx = g[0] / TX, ty = g[1] / TY, tz = nd == 3 ? g[2] / TZ : 0;
     const int64_t t0 = I.at[0], t01 = (int64_t)I.at[0] * I.at[1];
     const int64_t tl = tx + t0 * ty + t01 * tz;
     uint8_t* fl = flags + I.atile_base;
-    fl[tl] = 1;
+    unsigned* pd = apend + I.atile_base;
+    const unsigned pv = __float_as_uint(__double2float_ru(fabs(t)));
+    auto mark = [&](int64_t k) {
+        fl[k] = 1;
+        atomicMax(&pd[k], pv);
+    };
+    mark(tl);
     const int lx = g[0] - tx * TX, ly = g[1] - ty * TY, lz = nd == 3 ? g[2] - tz * TZ : 0;
-    if (lx < 2 && tx > 0) fl[tl - 1] = 1;
-    if (lx >= TX - 2 && tx < I.at[0] - 1) fl[tl + 1] = 1;
-    if (ly < 2 && ty > 0) fl[tl - t0] = 1;
-    if (ly >= TY - 2 && ty < I.at[1] - 1) fl[tl + t0] = 1;
+    if (lx < 2 && tx > 0) mark(tl - 1);
+    if (lx >= TX - 2 && tx < I.at[0] - 1) mark(tl + 1);
+    if (ly < 2 && ty > 0) mark(tl - t0);
+    if (ly >= TY - 2 && ty < I.at[1] - 1) mark(tl + t0);
     if (nd == 3) {
-        if (lz < 2 && tz > 0) fl[tl - t01] = 1;
-        if (lz >= TZ - 2 && tz < I.at[2] - 1) fl[tl + t01] = 1;
+        if (lz < 2 && tz > 0) mark(tl - t01);
+        if (lz >= TZ - 2 && tz < I.at[2] - 1) mark(tl + t01);
     }
 }
 
@@ -386,6 +398,7 @@ int model_slowness(eik_model* M);
 int gradient_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, const eik_params* P, int whichgrad,
                    const double* pickobs, const double* stdobs, double* gradvel, double* gradsrc, double* misf);
 // tile-flag compaction (eik_fim.hip), also used by the adjoint rounds
+__global__ void k_fill_u32(unsigned* __restrict__ p, int64_t n, unsigned v);
 __global__ void k_compact(uint8_t* __restrict__ flags, int64_t ntiles, int64_t* __restrict__ list,
                           unsigned* __restrict__ count);
 
