@@ -7,6 +7,7 @@ from .grids import (ExtraParams, Grid2DCart, Grid2DSphere, Grid3DCart, Grid3DSph
 from .api import (Context, EikModel, default_context, eikgradient, eikttimemisfit, eiktraveltime)
 from .serving import Request, Server
 from . import hmc, autodiff
+from .vtk import savemodelvtk
 from .autodiff import misfit_torch, rrule_eikttimemisfit
 
 HMCtraveltimes = hmc  # the reference's submodule name (src/HMCtraveltimes.jl)
@@ -14,5 +15,5 @@ HMCtraveltimes = hmc  # the reference's submodule name (src/HMCtraveltimes.jl)
 __all__ = [
     "ExtraParams", "GridRefinementPars", "Grid2DCart", "Grid3DCart", "Grid2DSphere", "Grid3DSphere",
     "eiktraveltime", "eikgradient", "eikttimemisfit", "EikModel", "Context", "default_context",
-    "Server", "Request", "HMCtraveltimes", "rrule_eikttimemisfit", "misfit_torch",
+    "Server", "Request", "HMCtraveltimes", "rrule_eikttimemisfit", "misfit_torch", "savemodelvtk",
 ]
