@@ -8,6 +8,7 @@ from .api import (Context, EikModel, default_context, eikgradient, eikttimemisfi
 from .serving import Request, Server
 from . import hmc, autodiff
 from .vtk import savemodelvtk
+from .rays import tracerays
 from .autodiff import misfit_torch, rrule_eikttimemisfit
 
 HMCtraveltimes = hmc  # the reference's submodule name (src/HMCtraveltimes.jl)
@@ -15,5 +16,5 @@ HMCtraveltimes = hmc  # the reference's submodule name (src/HMCtraveltimes.jl)
 __all__ = [
     "ExtraParams", "GridRefinementPars", "Grid2DCart", "Grid3DCart", "Grid2DSphere", "Grid3DSphere",
     "eiktraveltime", "eikgradient", "eikttimemisfit", "EikModel", "Context", "default_context",
-    "Server", "Request", "HMCtraveltimes", "rrule_eikttimemisfit", "misfit_torch", "savemodelvtk",
+    "Server", "Request", "HMCtraveltimes", "rrule_eikttimemisfit", "misfit_torch", "savemodelvtk", "tracerays",
 ]

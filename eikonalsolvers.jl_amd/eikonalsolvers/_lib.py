@@ -20,6 +20,7 @@ EIK_EINVAL = -2
 EIK_ENOMEM = -3
 EIK_EDEVICE = -4
 EIK_ENOCONV = -5
+EIK_ESIZE = -6
 
 # every symbol include/eik.h declares
 EXPORTS = (
@@ -27,6 +28,7 @@ EXPORTS = (
     "eik_model_create", "eik_model_update_vel", "eik_model_destroy",
     "eik_traveltime", "eik_gradient", "eik_misfit", "eik_last_stats",
     "eik_server_create", "eik_server_destroy", "eik_submit", "eik_wait", "eik_server_stats",
+    "eik_tracerays",
 )
 
 
@@ -106,6 +108,9 @@ def load():
         L.eik_wait.restype = ctypes.c_int
         L.eik_server_stats.argtypes = [vp, ip, ctypes.c_int32]
         L.eik_server_stats.restype = ctypes.c_int
+        L.eik_tracerays.argtypes = [vp, P(EikGrid), ctypes.c_int64, dp, dp, ip, dp, ctypes.c_double, ip, dp,
+                                    ctypes.c_int64]
+        L.eik_tracerays.restype = ctypes.c_int
         _lib = L
         return L
 

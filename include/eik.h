@@ -32,6 +32,7 @@ extern "C" {
 #define EIK_ENOMEM (-3)       /* device or host allocation failed */
 #define EIK_EDEVICE (-4)      /* HIP runtime error */
 #define EIK_ENOCONV (-5)      /* iterative solve did not converge within its round limit */
+#define EIK_ESIZE (-6)        /* caller's output buffer too small; the size needed has been written */
 
 /* grid kinds (src/EikSolv/eikstructs.jl:33-219) */
 #define EIK_GRID2D_CART 0
@@ -117,6 +118,20 @@ int eik_misfit(eik_ctx* ctx, eik_model* model, int64_t nsrc, const double* coord
  * = after 2000 rounds, changes below 10^(k-1) x 1e-12 relative (FP64; 4e-6 in FP32) stopped
  * propagating (the tolerance grows tenfold every 500 rounds).  Converging solves stay at 0. */
 int eik_last_stats(const eik_ctx* ctx, double* out, int32_t nout);
+
+/* tracerays (src/EikSolv/eikutils.jl:201-328): for every source's receivers, the ray descending
+ * the gradient of the cubic B-spline interpolant of that source's traveltime grid (Interpolations
+ * BSpline(Cubic(Line(OnGrid())))) in steps of steplen*hgrid (Cartesian) or steplen (Grid2DSphere),
+ * until within hgrid*sqrt(2) (dr*sqrt(2)) of the source, whose coordinates end the ray.
+ * ttime: nsrc grids (nsrc*N doubles, source-major, index 1 fastest), e.g. eik_traveltime's ttgrid.
+ * ray_off (nrec+1) receives the prefix sums of the points per ray, rays in receiver order.  When
+ * rays != NULL the points are written as packed rows (ray_off[nrec] x ndim doubles) if they fit in
+ * cap_points, else EIK_ESIZE is returned with ray_off filled.  Grid3DSphere: EIK_EINVAL, as in the
+ * reference.  A ray that has not reached its source after 20 domain diagonals of steps fails with
+ * EIK_ENOCONV (the reference has no such limit). */
+int eik_tracerays(eik_ctx* ctx, const eik_grid* grid, int64_t nsrc, const double* ttime, const double* coordsrc,
+                  const int64_t* rec_off, const double* coordrec, double steplen, int64_t* ray_off, double* rays,
+                  int64_t cap_points);
 
 /* ---------------------------------------------------------------- request loop (serving)
  * No reference counterpart: the reference evaluates one call at a time.  A server runs
