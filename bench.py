@@ -217,7 +217,7 @@ def main() -> int:
     f = res["fp64"]
     out["roofline"] = {
         "bound": "hbm",
-        "kernel": "k_fim_round<3,double>",
+        "kernel": "k_fim_round<3,double,false>",
         "achieved": round(f["achieved_gbs"], 2) if f["achieved_gbs"] else None,
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
@@ -255,6 +255,13 @@ def main() -> int:
                                "sample": f"{ns} of the 16 sources of the same workload, heap FMM with refinement "
                                          f"(C restatement of the reference), OpenMP by source chunk, {tcpu:.2f} s"}
         out["parity_fp64_picks_max_rel_vs_oracle"] = rel
+        # the same restatement on one core, two sources (the single-thread rate of the reference path)
+        t0 = time.perf_counter()
+        O.traveltime(vel, grd, src[:2], recs[:2], extraparams=ExtraParams(), nthreads=1)
+        t1 = time.perf_counter() - t0
+        out["cpu_baseline_1core"] = {"value": round(2 / t1, 4), "unit": "source-solves/s", "cores": 1,
+                                     "kind": "port", "sample": f"2 of the 16 sources, one thread, {t1:.2f} s",
+                                     "gpu_fp64_over_1core": round(out["value"] / (2 / t1), 1)}
     print(json.dumps(out), flush=True)
     return 0
 
