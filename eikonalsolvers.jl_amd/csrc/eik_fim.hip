@@ -728,6 +728,15 @@ __global__ void k_round_reset(unsigned* __restrict__ wmin, int ninst, unsigned* 
     }
 }
 
+// Relative size below which an FP32 change wakes no other node (EIK_F32_SETTLE overrides).
+static double f32_settle() {
+    static const double v = [] {
+        const char* e = getenv("EIK_F32_SETTLE");
+        return e ? atof(e) : 0.0;
+    }();
+    return v;
+}
+
 // Rounds are enqueued in batches of kRoundBatch without host synchronisation: each round resets
 // the per-instance minimum, selects the tiles within the band, and runs the round kernel on a
 // fixed grid of resident blocks that take tiles from the device-side list.  After a batch the
@@ -769,8 +778,11 @@ static int fim_run_t(eik_ctx* ctx, const InstDev* d_inst, int ninst, int64_t nti
     const char* es = getenv("EIK_SETTLE_ROUND");  // test hook: settle from an earlier round
     const int64_t settle_batch = (es ? std::max(0, atoi(es)) : kSettleRound) / kRoundBatch;
     const T settle_tol = sizeof(T) == 8 ? T(1e-12) : T(4e-6);
-    // FP64 with `fast`: changes below settle_tol stop propagating from the first round
-    const T base_settle = (sizeof(T) == 8 && !exact) ? settle_tol : T(0);
+    // FP64 with `fast`: changes below settle_tol stop propagating from the first round.  FP32
+    // (mixed precision): changes below f32_settle() relative (a few ulps) never propagate; they
+    // are rounding ripples, and at heap near-ties within FP32 resolution they can otherwise
+    // circulate for thousands of rounds before the settling stage ends them.
+    const T base_settle = sizeof(T) == 8 ? (!exact ? settle_tol : T(0)) : T(f32_settle());
     for (int64_t batch = 0;; batch++) {
         if (batch >= maxbatches) return ctx->fail(EIK_ENOCONV, "traveltime solve did not converge");
         // the tolerance grows tenfold every 500 rounds while the cycle persists

@@ -245,6 +245,17 @@ __device__ __forceinline__ bool keq(const Key<T>& a, const Key<T>& b) {
 // each other stale keys during the iteration, and ignoring such a candidate breaks the cycle.
 constexpr int kMaxCascade = 16;
 
+// Pop-order comparisons in FP32: a traveltime within kTieUlps ulps below another is treated as
+// equal to it (one heap batch: neither node pops before the other, and an update that lands just
+// below its trigger is not a cascade).  Near such ties FP32 cannot decide the heap's order anyway,
+// and exact comparisons there let two nodes flip each other's choice round after round.  FP64
+// compares exactly (the heap's own order).
+constexpr float kTieUlps = 4.0f;
+__device__ __forceinline__ double tie_below(double x) { return x; }
+__device__ __forceinline__ float tie_below(float x) {
+    return fabsf(x) < EIK_INFF ? x - kTieUlps * 1.1920929e-7f * fabsf(x) : x;
+}
+
 template <int ND>
 __device__ __forceinline__ double eval_any(const double* v1, const double* v2, unsigned acc1, unsigned acc2, double s,
                                            const Metric<double>& m, int* code) {
@@ -286,7 +297,7 @@ __device__ __forceinline__ T replay_plain(const T* v1, const T* v2, unsigned can
         T c = tinf<T>();
 #pragma unroll
         for (int k = 0; k < 2 * ND; k++) c = (((cand >> k) & 1u) && v1[k] > prev && v1[k] < c) ? v1[k] : c;
-        if (!(c < tinf<T>()) || kzD != 0 || !(c < kzK)) break;
+        if (!(c < tinf<T>()) || kzD != 0 || !(c < tie_below(kzK))) break;
         prev = c;
         unsigned acc2 = 0u;
 #pragma unroll
@@ -296,7 +307,7 @@ __device__ __forceinline__ T replay_plain(const T* v1, const T* v2, unsigned can
             acc2 |= (unsigned)a2 << k;
         }
         cur = eval_any<ND>(v1, v2, acc1, acc2, s, m, code);
-        const bool ord = cur >= c;
+        const bool ord = cur >= tie_below(c);
         kzK = ord ? cur : c;
         kzS = ord ? -tinf<T>() : cur;
         kzD = ord ? 0 : 1;
@@ -351,7 +362,7 @@ __device__ __forceinline__ T replay(const T* v1, const T* v2, KeyAt keys, unsign
             c.D = take ? kk.D : c.D;
             found |= take;
         }
-        if (!found || !klt(c, kz)) break;
+        if (!found || !((c.K < tie_below(kz.K)) | ((c.K == kz.K) & klt(c, kz)))) break;
         prev = c;
         unsigned acc1 = 0u, acc2 = 0u;
 #pragma unroll
@@ -364,7 +375,7 @@ __device__ __forceinline__ T replay(const T* v1, const T* v2, KeyAt keys, unsign
             acc2 |= (unsigned)a2 << k;
         }
         cur = eval_any<ND>(v1, v2, acc1, acc2, s, m, code);
-        const bool ord = cur >= c.K;
+        const bool ord = cur >= tie_below(c.K);
         kz.K = ord ? cur : c.K;
         kz.S = ord ? -tinf<T>() : (cur > c.S ? cur : c.S);
         kz.D = ord ? 0 : c.D + 1;
