@@ -317,6 +317,46 @@ int eik_traveltime(eik_ctx* ctx, eik_model* model, int64_t nsrc, const double* c
     return rc;
 }
 
+int eik_traveltime_batch(eik_ctx* ctx, int32_t nreq, const eik_request* reqs) {
+    if (!ctx || nreq < 0 || (nreq > 0 && !reqs)) return EIK_EINVAL;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    if (nreq == 0) return EIK_OK;
+    std::vector<SourceSet> SS((size_t)nreq);
+    std::vector<FwdGroup> G;
+    int64_t nrec = 0;
+    for (int32_t i = 0; i < nreq; i++) {
+        const eik_request& q = reqs[i];
+        if (q.type != EIK_REQ_TRAVELTIME || !q.model) return ctx->fail(EIK_EINVAL, "request %d: not a traveltime request", i);
+        if (std::memcmp(&q.params, &reqs[0].params, sizeof(eik_params)) != 0)
+            return ctx->fail(EIK_EINVAL, "request %d: params differ from request 0", i);
+        if (q.model->g.kind != reqs[0].model->g.kind || q.model->g.ndim != reqs[0].model->g.ndim)
+            return ctx->fail(EIK_EINVAL, "request %d: grid kind differs from request 0", i);
+        const int rc = load_sources(ctx, q.model, q.nsrc, q.coordsrc, q.rec_off, q.coordrec, &q.params, SS[i]);
+        if (rc) return rc;
+        if (SS[i].nrec() > 0 && !q.ttpicks) return ctx->fail(EIK_EINVAL, "request %d: null picks buffer", i);
+        if (SS[i].nsrc > 0) G.push_back(FwdGroup{q.model, &SS[i]});
+        nrec += SS[i].nrec();
+    }
+    if (G.empty()) return EIK_OK;
+    const eik_params pe = effective_params(&reqs[0].params);
+    for (double& x : ctx->stats) x = 0.0;
+    Timer total(ctx);
+    int rc = forward_solve(ctx, G, pe.precision, &pe);
+    if (rc) return rc;
+    std::vector<double> picks((size_t)std::max<int64_t>(nrec, 1));
+    rc = forward_picks(ctx, G, pe.precision, picks.data(), nullptr);
+    ctx->stats[0] = total.mark();
+    if (rc) return rc;
+    int64_t off = 0;
+    for (int32_t i = 0; i < nreq; i++) {
+        const int64_t n = SS[i].nsrc > 0 ? SS[i].nrec() : 0;
+        if (n > 0) std::memcpy(reqs[i].ttpicks, picks.data() + off, (size_t)n * sizeof(double));
+        off += n;
+    }
+    return EIK_OK;
+}
+
 int eik_misfit(eik_ctx* ctx, eik_model* model, int64_t nsrc, const double* coordsrc, const int64_t* rec_off,
                const double* coordrec, const double* pickobs, const double* stdobs, const eik_params* params,
                double* misf) {

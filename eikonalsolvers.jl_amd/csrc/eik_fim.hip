@@ -550,8 +550,8 @@ __global__ void k_slowness(const double* __restrict__ vel, double* __restrict__ 
 // boxes (vel = the nearest-neighbour fine velocity, read through the box's map).
 template <typename T>
 __global__ void k_corners(const CornerDev* __restrict__ cs, int ncs, const InstDev* __restrict__ inst,
-                          T* __restrict__ ttpool, const double* __restrict__ velc, const FineDev* __restrict__ fine,
-                          unsigned* __restrict__ flags, unsigned* __restrict__ pend, int nd, int kind) {
+                          T* __restrict__ ttpool, const FineDev* __restrict__ fine, unsigned* __restrict__ flags,
+                          unsigned* __restrict__ pend, int nd, int kind) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= ncs) return;
     const CornerDev C = cs[q];
@@ -563,9 +563,9 @@ __global__ void k_corners(const CornerDev* __restrict__ cs, int ncs, const InstD
         const FineDev F = fine[C.inst];
         int cg[3] = {0, 0, 0};
         for (int d = 0; d < nd; d++) cg[d] = F.i1c[d] + nn_coarse(g[d], F.df);
-        v = velc[cg[0] + (int64_t)F.nc[0] * (cg[1] + (int64_t)F.nc[1] * cg[2])];
+        v = C.vel[cg[0] + (int64_t)F.nc[0] * (cg[1] + (int64_t)F.nc[1] * cg[2])];
     } else {
-        v = velc[l];
+        v = C.vel[l];
     }
     const double dist = C.dist;
     ttpool[I.tt_off + l] = -(T)(dist / v);
@@ -574,8 +574,7 @@ __global__ void k_corners(const CornerDev* __restrict__ cs, int ncs, const InstD
 
 // nearest-neighbour fine slowness of every refined box (createfinegrid, eikmainfwd.jl:1035-1056)
 __global__ void k_fine_setup(const FineDev* __restrict__ fine, int nfine, const InstDev* __restrict__ inst,
-                             const double* __restrict__ velc, double* __restrict__ slowf, double* __restrict__ ttf,
-                             int nd) {
+                             double* __restrict__ slowf, double* __restrict__ ttf, int nd) {
     const int f = blockIdx.y;
     if (f >= nfine) return;
     const FineDev F = fine[f];
@@ -588,7 +587,7 @@ __global__ void k_fine_setup(const FineDev* __restrict__ fine, int nfine, const 
         g[2] = (int)(l / ((int64_t)I.n[0] * I.n[1]));
         int cg[3] = {0, 0, 0};
         for (int d = 0; d < nd; d++) cg[d] = F.i1c[d] + nn_coarse(g[d], F.df);
-        const double v = velc[cg[0] + (int64_t)F.nc[0] * (cg[1] + (int64_t)F.nc[1] * cg[2])];
+        const double v = F.vel[cg[0] + (int64_t)F.nc[0] * (cg[1] + (int64_t)F.nc[1] * cg[2])];
         slowf[I.s_off + l] = 1.0 / v;
         ttf[I.tt_off + l] = EIK_INF;
     }
@@ -894,47 +893,81 @@ static double corner_dist(const InstDev& I, int nd, int kind, const int* ijk, co
     return distance2points(kind, nd, xyz, src);
 }
 
-int forward_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, int prec, const eik_params* P) {
-    const int nd = M->g.ndim;
-    const int kind = M->g.kind;
-    const int64_t N = M->N;
-    const int64_t nsrc = S.nsrc;
+int forward_solve(eik_ctx* ctx, const std::vector<FwdGroup>& G, int prec, const eik_params* P) {
+    const eik_model* M0 = G[0].M;
+    const int nd = M0->g.ndim;
+    const int kind = M0->g.kind;
+    for (const FwdGroup& g : G)
+        if (g.M->g.ndim != nd || g.M->g.kind != kind) return ctx->fail(EIK_EINVAL, "batched models of different grid kinds");
+    // per source: its model and its coordinates
+    std::vector<const eik_model*> smod;
+    std::vector<double> sxyz;
+    for (const FwdGroup& g : G)
+        for (int64_t s = 0; s < g.S->nsrc; s++) {
+            smod.push_back(g.M);
+            for (int d = 0; d < nd; d++) sxyz.push_back(g.S->src[s * nd + d]);
+        }
+    const int64_t nsrc = (int64_t)smod.size();
     hipStream_t st = ctx->stream;
     Timer tm(ctx);
 
-    // coarse instances: one grid per source, sharing the model's slowness
-    std::vector<InstDev> cinst(nsrc);
-    int64_t tiles_per = 0, atiles_per = 0;
-    for (int64_t s = 0; s < nsrc; s++) {
-        InstDev& I = cinst[s];
-        make_inst(M->g, I);
-        I.sinth = M->d_sinth;
-        if (kind == EIK_GRID3D_SPHERE && !I.sinth) return ctx->fail(EIK_EINVAL, "3D sphere model without its metric table");
-        set_tiles(I, nd);
-        tiles_per = (int64_t)I.t[0] * I.t[1] * I.t[2];
-        atiles_per = (int64_t)I.at[0] * I.at[1] * I.at[2];
-        I.tt_off = s * N;
-        I.s_off = 0;
-        I.tile_base = s * tiles_per;
-        I.atile_base = s * atiles_per;
-        set_band(I, M->smean);
-    }
-    const int64_t ntiles_c = nsrc * tiles_per;
+    // slowness of the models: a single model's own arrays, or the models' arrays side by side
     const size_t esz = prec == EIK_PREC_FP64 ? sizeof(double) : sizeof(float);
+    const void* slow = prec == EIK_PREC_FP64 ? (const void*)M0->d_slow64 : (const void*)M0->d_slow32;
+    std::vector<int64_t> soff(G.size(), 0);
+    if (G.size() > 1) {
+        int64_t tot = 0;
+        for (size_t k = 0; k < G.size(); k++) {
+            soff[k] = tot;
+            tot += G[k].M->N;
+        }
+        DevBuf& sb = ctx->buf("slow_batch");
+        if (sb.ensure((size_t)tot * esz)) return ctx->fail(EIK_ENOMEM, "out of device memory (batched models)");
+        for (size_t k = 0; k < G.size(); k++) {
+            const void* src = prec == EIK_PREC_FP64 ? (const void*)G[k].M->d_slow64 : (const void*)G[k].M->d_slow32;
+            CK(hipMemcpyAsync((char*)sb.p + soff[k] * esz, src, (size_t)G[k].M->N * esz, hipMemcpyDeviceToDevice, st));
+        }
+        slow = sb.p;
+    }
+
+    // coarse instances: one grid per source, sharing its model's slowness
+    std::vector<InstDev> cinst(nsrc);
+    int64_t ntiles_c = 0, antiles_c = 0, totN = 0;
+    {
+        int64_t s = 0;
+        for (size_t k = 0; k < G.size(); k++)
+            for (int64_t j = 0; j < G[k].S->nsrc; j++, s++) {
+                const eik_model* M = G[k].M;
+                InstDev& I = cinst[s];
+                make_inst(M->g, I);
+                I.sinth = M->d_sinth;
+                if (kind == EIK_GRID3D_SPHERE && !I.sinth)
+                    return ctx->fail(EIK_EINVAL, "3D sphere model without its metric table");
+                set_tiles(I, nd);
+                I.tt_off = totN;
+                I.s_off = soff[k];
+                I.tile_base = ntiles_c;
+                I.atile_base = antiles_c;
+                set_band(I, M->smean);
+                totN += M->N;
+                ntiles_c += (int64_t)I.t[0] * I.t[1] * I.t[2];
+                antiles_c += (int64_t)I.at[0] * I.at[1] * I.at[2];
+            }
+    }
     DevBuf& ttc = ctx->buf("tt_coarse");
     DevBuf& kc = ctx->buf("key_K");
     DevBuf& sc = ctx->buf("key_S");
     DevBuf& dc = ctx->buf("key_D");
-    if (ttc.ensure((size_t)nsrc * N * esz) || kc.ensure((size_t)nsrc * N * esz) || sc.ensure((size_t)nsrc * N * esz) ||
-        dc.ensure((size_t)nsrc * N))
+    if (ttc.ensure((size_t)totN * esz) || kc.ensure((size_t)totN * esz) || sc.ensure((size_t)totN * esz) ||
+        dc.ensure((size_t)totN))
         return ctx->fail(EIK_ENOMEM, "out of device memory (traveltime grids)");
-    CK(hipMemsetAsync(dc.p, 0, (size_t)nsrc * N, st));
+    CK(hipMemsetAsync(dc.p, 0, (size_t)totN, st));
     const TTPools cpools{ttc.p, kc.p, sc.p, (uint8_t*)dc.p};
     DevBuf& dci = ctx->buf("inst_coarse");
     if (dci.ensure(cinst.size() * sizeof(InstDev))) return ctx->fail(EIK_ENOMEM, "out of device memory");
     CK(hipMemcpyAsync(dci.p, cinst.data(), cinst.size() * sizeof(InstDev), hipMemcpyHostToDevice, st));
 
-    int64_t maxt = ntiles_c, maxat = nsrc * atiles_per;
+    int64_t maxt = ntiles_c, maxat = antiles_c;
     std::vector<FineDev> fine;
     std::vector<InstDev> finst;
     int64_t ntiles_f = 0, antiles_f = 0, nfine_pts = 0;
@@ -942,13 +975,14 @@ int forward_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, int prec, cons
         fine.resize(nsrc);
         finst.resize(nsrc);
         for (int64_t s = 0; s < nsrc; s++) {
-            build_fine(M->g, S.src.data() + s * nd, P->downscalefactor, P->noderadius, fine[s], finst[s]);
+            build_fine(smod[s]->g, sxyz.data() + s * nd, P->downscalefactor, P->noderadius, fine[s], finst[s]);
             fine[s].cinst = (int)s;
+            fine[s].vel = smod[s]->d_vel;
             finst[s].tt_off = nfine_pts;
             finst[s].s_off = nfine_pts;
             finst[s].tile_base = ntiles_f;
             finst[s].atile_base = antiles_f;
-            set_band(finst[s], M->smean);
+            set_band(finst[s], smod[s]->smean);
             nfine_pts += (int64_t)finst[s].n[0] * finst[s].n[1] * finst[s].n[2];
             ntiles_f += (int64_t)finst[s].t[0] * finst[s].t[1] * finst[s].t[2];
             antiles_f += (int64_t)finst[s].at[0] * finst[s].at[1] * finst[s].at[2];
@@ -986,9 +1020,9 @@ int forward_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, int prec, cons
 
     // coarse grids start at +inf
     if (prec == EIK_PREC_FP64)
-        k_fill<double><<<nblocks(nsrc * N, 256), 256, 0, st>>>((double*)ttc.p, nsrc * N, EIK_INF);
+        k_fill<double><<<nblocks(totN, 256), 256, 0, st>>>((double*)ttc.p, totN, EIK_INF);
     else
-        k_fill<float><<<nblocks(nsrc * N, 256), 256, 0, st>>>((float*)ttc.p, nsrc * N, EIK_INFF);
+        k_fill<float><<<nblocks(totN, 256), 256, 0, st>>>((float*)ttc.p, totN, EIK_INFF);
 
     std::vector<CornerDev> corners;
     if (P->refinearoundsrc) {
@@ -1009,15 +1043,16 @@ int forward_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, int prec, cons
         CK(hipMemcpyAsync(dfi.p, finst.data(), finst.size() * sizeof(InstDev), hipMemcpyHostToDevice, st));
         CK(hipMemcpyAsync(dfd.p, fine.data(), fine.size() * sizeof(FineDev), hipMemcpyHostToDevice, st));
         dim3 gs(64, (unsigned)nsrc);
-        k_fine_setup<<<gs, 256, 0, st>>>((const FineDev*)dfd.p, (int)nsrc, (const InstDev*)dfi.p, M->d_vel,
-                                         (double*)slf.p, (double*)ttf.p, nd);
+        k_fine_setup<<<gs, 256, 0, st>>>((const FineDev*)dfd.p, (int)nsrc, (const InstDev*)dfi.p, (double*)slf.p,
+                                         (double*)ttf.p, nd);
         for (int64_t s = 0; s < nsrc; s++) {
             for (int c = 0; c < fine[s].ncorn; c++) {
                 CornerDev C{};
                 C.inst = (int)s;
                 for (int d = 0; d < 3; d++) C.ijk[d] = fine[s].corn[c][d];
-                for (int d = 0; d < nd; d++) C.src[d] = S.src[s * nd + d];
+                for (int d = 0; d < nd; d++) C.src[d] = sxyz[s * nd + d];
                 C.dist = corner_dist(finst[s], nd, kind, C.ijk, C.src);
+                C.vel = smod[s]->d_vel;
                 corners.push_back(C);
             }
         }
@@ -1025,7 +1060,7 @@ int forward_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, int prec, cons
         if (dcs.ensure(corners.size() * sizeof(CornerDev))) return ctx->fail(EIK_ENOMEM, "out of device memory");
         CK(hipMemcpyAsync(dcs.p, corners.data(), corners.size() * sizeof(CornerDev), hipMemcpyHostToDevice, st));
         k_corners<double><<<nblocks((int64_t)corners.size(), 64), 64, 0, st>>>(
-            (const CornerDev*)dcs.p, (int)corners.size(), (const InstDev*)dfi.p, (double*)ttf.p, M->d_vel,
+            (const CornerDev*)dcs.p, (int)corners.size(), (const InstDev*)dfi.p, (double*)ttf.p,
             (const FineDev*)dfd.p, (unsigned*)flags.p, (unsigned*)pend.p, nd, kind);
         CK(hipGetLastError());
         tm.mark();
@@ -1057,13 +1092,14 @@ int forward_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, int prec, cons
     } else {
         for (int64_t s = 0; s < nsrc; s++) {
             int corn[8][3];
-            const int nc = find_enclosing_box(M->g, S.src.data() + s * nd, corn);
+            const int nc = find_enclosing_box(smod[s]->g, sxyz.data() + s * nd, corn);
             for (int c = 0; c < nc; c++) {
                 CornerDev C{};
                 C.inst = (int)s;
                 for (int d = 0; d < 3; d++) C.ijk[d] = corn[c][d];
-                for (int d = 0; d < nd; d++) C.src[d] = S.src[s * nd + d];
+                for (int d = 0; d < nd; d++) C.src[d] = sxyz[s * nd + d];
                 C.dist = corner_dist(cinst[s], nd, kind, C.ijk, C.src);
+                C.vel = smod[s]->d_vel;
                 corners.push_back(C);
             }
         }
@@ -1072,16 +1108,15 @@ int forward_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, int prec, cons
         CK(hipMemcpyAsync(dcs.p, corners.data(), corners.size() * sizeof(CornerDev), hipMemcpyHostToDevice, st));
         if (prec == EIK_PREC_FP64)
             k_corners<double><<<nblocks((int64_t)corners.size(), 64), 64, 0, st>>>(
-                (const CornerDev*)dcs.p, (int)corners.size(), (const InstDev*)dci.p, (double*)ttc.p, M->d_vel, nullptr,
+                (const CornerDev*)dcs.p, (int)corners.size(), (const InstDev*)dci.p, (double*)ttc.p, nullptr,
                 (unsigned*)flags.p, (unsigned*)pend.p, nd, kind);
         else
             k_corners<float><<<nblocks((int64_t)corners.size(), 64), 64, 0, st>>>(
-                (const CornerDev*)dcs.p, (int)corners.size(), (const InstDev*)dci.p, (float*)ttc.p, M->d_vel, nullptr,
+                (const CornerDev*)dcs.p, (int)corners.size(), (const InstDev*)dci.p, (float*)ttc.p, nullptr,
                 (unsigned*)flags.p, (unsigned*)pend.p, nd, kind);
         CK(hipGetLastError());
         tm.mark();
     }
-    const void* slow = prec == EIK_PREC_FP64 ? (const void*)M->d_slow64 : (const void*)M->d_slow32;
     FimStats cst{};
     int rc = fim_run(ctx, nd, prec, kind, (const InstDev*)dci.p, (int)nsrc, ntiles_c, cpools, slow, P->fast == 0,
                      &cst);
@@ -1098,11 +1133,24 @@ int forward_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, int prec, cons
     return EIK_OK;
 }
 
-int forward_picks(eik_ctx* ctx, eik_model* M, const SourceSet& S, int prec, double* picks, double* ttgrid) {
+int forward_picks(eik_ctx* ctx, const std::vector<FwdGroup>& G, int prec, double* picks, double* ttgrid) {
     hipStream_t st = ctx->stream;
-    const int nd = M->g.ndim;
-    const int64_t nrec = S.nrec();
-    const int64_t N = M->N;
+    const int nd = G[0].M->g.ndim;
+    const int kind = G[0].M->g.kind;
+    // receivers of every source, in group and source order
+    std::vector<double> rec;
+    std::vector<int> rs;
+    std::vector<int64_t> N;
+    int s0 = 0;
+    for (const FwdGroup& g : G) {
+        rec.insert(rec.end(), g.S->rec.begin(), g.S->rec.end());
+        for (int64_t s = 0; s < g.S->nsrc; s++) {
+            for (int64_t r = g.S->rec_off[s]; r < g.S->rec_off[s + 1]; r++) rs.push_back(s0 + (int)s);
+            N.push_back(g.M->N);
+        }
+        s0 += (int)g.S->nsrc;
+    }
+    const int64_t nrec = (int64_t)rs.size();
     if (nrec > 0) {
         DevBuf& drec = ctx->buf("rec");
         DevBuf& drs = ctx->buf("recsrc");
@@ -1110,32 +1158,33 @@ int forward_picks(eik_ctx* ctx, eik_model* M, const SourceSet& S, int prec, doub
         if (drec.ensure((size_t)nrec * nd * sizeof(double)) || drs.ensure((size_t)nrec * sizeof(int)) ||
             dpk.ensure((size_t)nrec * sizeof(double)))
             return ctx->fail(EIK_ENOMEM, "out of device memory (receivers)");
-        std::vector<int> rs(nrec);
-        for (int64_t s = 0; s < S.nsrc; s++)
-            for (int64_t r = S.rec_off[s]; r < S.rec_off[s + 1]; r++) rs[r] = (int)s;
-        CK(hipMemcpyAsync(drec.p, S.rec.data(), (size_t)nrec * nd * sizeof(double), hipMemcpyHostToDevice, st));
+        CK(hipMemcpyAsync(drec.p, rec.data(), (size_t)nrec * nd * sizeof(double), hipMemcpyHostToDevice, st));
         CK(hipMemcpyAsync(drs.p, rs.data(), (size_t)nrec * sizeof(int), hipMemcpyHostToDevice, st));
         const InstDev* dci = (const InstDev*)ctx->buf("inst_coarse").p;
         void* ttc = ctx->buf("tt_coarse").p;
         if (prec == EIK_PREC_FP64)
             k_receivers<double><<<nblocks(nrec, 128), 128, 0, st>>>((const double*)drec.p, (const int*)drs.p, nrec, dci,
-                                                                    (const double*)ttc, nd, M->g.kind, (double*)dpk.p);
+                                                                    (const double*)ttc, nd, kind, (double*)dpk.p);
         else
             k_receivers<float><<<nblocks(nrec, 128), 128, 0, st>>>((const double*)drec.p, (const int*)drs.p, nrec, dci,
-                                                                   (const float*)ttc, nd, M->g.kind, (double*)dpk.p);
+                                                                   (const float*)ttc, nd, kind, (double*)dpk.p);
         CK(hipGetLastError());
         CK(hipMemcpyAsync(picks, dpk.p, (size_t)nrec * sizeof(double), hipMemcpyDeviceToHost, st));
     }
     if (ttgrid) {
+        int64_t maxN = 0;
+        for (int64_t n : N) maxN = std::max(maxN, n);
         DevBuf& ex = ctx->buf("export");
-        if (ex.ensure((size_t)N * sizeof(double))) return ctx->fail(EIK_ENOMEM, "out of device memory");
+        if (ex.ensure((size_t)maxN * sizeof(double))) return ctx->fail(EIK_ENOMEM, "out of device memory");
         void* ttc = ctx->buf("tt_coarse").p;
-        for (int64_t s = 0; s < S.nsrc; s++) {
+        int64_t off = 0;
+        for (size_t s = 0; s < N.size(); s++) {
             if (prec == EIK_PREC_FP64)
-                k_export<double><<<nblocks(N, 256), 256, 0, st>>>((const double*)ttc + s * N, (double*)ex.p, N);
+                k_export<double><<<nblocks(N[s], 256), 256, 0, st>>>((const double*)ttc + off, (double*)ex.p, N[s]);
             else
-                k_export<float><<<nblocks(N, 256), 256, 0, st>>>((const float*)ttc + s * N, (double*)ex.p, N);
-            CK(hipMemcpyAsync(ttgrid + s * N, ex.p, (size_t)N * sizeof(double), hipMemcpyDeviceToHost, st));
+                k_export<float><<<nblocks(N[s], 256), 256, 0, st>>>((const float*)ttc + off, (double*)ex.p, N[s]);
+            CK(hipMemcpyAsync(ttgrid + off, ex.p, (size_t)N[s] * sizeof(double), hipMemcpyDeviceToHost, st));
+            off += N[s];
         }
     }
     CK(hipStreamSynchronize(st));

@@ -56,6 +56,7 @@ struct FineDev {
     int32_t ncorn;       // source-box corners on the fine grid
     int32_t corn[8][3];
     int32_t cinst;       // coarse instance of the same source
+    const double* vel;   // velocity of the source's model (coarse grid)
 };
 
 // A source (init) point of one instance: coarse source-box corner, transferred coarse node, or
@@ -81,6 +82,7 @@ struct CornerDev {
     int32_t ijk[3];
     double src[3];
     double dist;  // distance corner-source, computed on the host (the same libm as the oracle)
+    const double* vel;  // velocity of the instance's model (coarse grid)
 };
 
 // fine index -> index inside the coarse window, nearest-neighbour rule of createfinegrid
@@ -392,8 +394,20 @@ int find_enclosing_box(const eik_grid& g, const double* src, int corn[8][3]);
 void build_fine(const eik_grid& g, const double* src, int df, int noderadius, FineDev& F, InstDev& I);
 
 // forward (eik_fim.hip)
-int forward_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, int prec, const eik_params* P);
-int forward_picks(eik_ctx* ctx, eik_model* M, const SourceSet& S, int prec, double* picks, double* ttgrid);
+// The sources of several models solved in shared launches (models of one grid kind): instances
+// are numbered group by group, source by source.
+struct FwdGroup {
+    eik_model* M;
+    const SourceSet* S;
+};
+int forward_solve(eik_ctx* ctx, const std::vector<FwdGroup>& G, int prec, const eik_params* P);
+int forward_picks(eik_ctx* ctx, const std::vector<FwdGroup>& G, int prec, double* picks, double* ttgrid);
+inline int forward_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, int prec, const eik_params* P) {
+    return forward_solve(ctx, std::vector<FwdGroup>{{M, &S}}, prec, P);
+}
+inline int forward_picks(eik_ctx* ctx, eik_model* M, const SourceSet& S, int prec, double* picks, double* ttgrid) {
+    return forward_picks(ctx, std::vector<FwdGroup>{{M, &S}}, prec, picks, ttgrid);
+}
 int model_slowness(eik_model* M);
 int gradient_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, const eik_params* P, int whichgrad,
                    const double* pickobs, const double* stdobs, double* gradvel, double* gradsrc, double* misf);

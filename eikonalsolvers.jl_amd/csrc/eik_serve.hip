@@ -5,10 +5,13 @@
 // own device pools, reused from request to request) driven by one host thread.  Submitted
 // requests wait in a queue; a free slot admits the oldest request whose device working set fits
 // in what the memory budget has left (smaller requests may pass a large one that does not fit,
-// unless it has waited longer than kMaxBypassMs).  Admitted requests run concurrently on their
-// streams, so their rounds overlap on the device, and every request still batches all of its
-// sources into shared launches.  Models are resident handles created by the caller; requests on
-// one model only read it.
+// unless it has waited longer than kMaxBypassMs).  A traveltime request takes along the other
+// queued traveltime requests of the same grid kind and parameters and of a similar size (working
+// set within a factor kBatchSpread, so a small request does not wait for a large one) that fit,
+// up to kMaxBatch requests and kMaxBatchSources sources: they run as one call whose launches
+// hold all their sources (eik_traveltime_batch).  Admitted work runs concurrently on the slots' streams, so
+// rounds of different slots overlap on the device.  Models are resident handles created by the
+// caller; requests on one model only read it.
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -31,6 +34,15 @@ struct Req {
 };
 
 constexpr double kMaxBypassMs = 2000.0;
+constexpr int kMaxBatch = 16;
+constexpr double kBatchSpread = 2.0;
+constexpr int64_t kMaxBatchSources = 512;
+
+bool batchable(const eik_request& a, const eik_request& b) {
+    return a.type == EIK_REQ_TRAVELTIME && b.type == EIK_REQ_TRAVELTIME && a.model && b.model &&
+           a.model->g.kind == b.model->g.kind && a.model->g.ndim == b.model->g.ndim &&
+           std::memcmp(&a.params, &b.params, sizeof(eik_params)) == 0;
+}
 
 }  // namespace
 
@@ -45,6 +57,7 @@ struct eik_server {
     std::map<int64_t, Req> reqs;
     int64_t inflight_bytes = 0;
     int64_t next_id = 1;
+    int64_t nbatches = 0, nbatched = 0;
     bool stop = false;
 };
 
@@ -105,16 +118,54 @@ static void worker(eik_server* S, int slot) {
             return false;
         });
         if (id < 0) return;  // stopping and nothing queued
-        Req& q = S->reqs[id];
-        S->inflight_bytes += q.bytes;
-        q.started = true;
-        q.t_start = Clock::now();
+        // the batch: this request and compatible queued traveltime requests that fit
+        std::vector<int64_t> ids{id};
+        int64_t bytes = S->reqs[id].bytes, nsrc = S->reqs[id].r.nsrc;
+        for (auto it = S->queue.begin(); it != S->queue.end() && (int)ids.size() < kMaxBatch;) {
+            const Req& q = S->reqs[*it];
+            const double ratio = (double)q.bytes / (double)S->reqs[id].bytes;
+            if (batchable(S->reqs[id].r, q.r) && ratio <= kBatchSpread && ratio >= 1.0 / kBatchSpread &&
+                nsrc + q.r.nsrc <= kMaxBatchSources && S->inflight_bytes + bytes + q.bytes <= S->budget) {
+                ids.push_back(*it);
+                bytes += q.bytes;
+                nsrc += q.r.nsrc;
+                it = S->queue.erase(it);
+            } else {
+                ++it;
+            }
+        }
+        S->inflight_bytes += bytes;
+        const Clock::time_point t0 = Clock::now();
+        for (int64_t k : ids) {
+            S->reqs[k].started = true;
+            S->reqs[k].t_start = t0;
+        }
+        if (ids.size() > 1) {
+            S->nbatches++;
+            S->nbatched += (int64_t)ids.size();
+        }
+        std::vector<Req*> qs;
+        for (int64_t k : ids) qs.push_back(&S->reqs[k]);  // std::map nodes stay put while unlocked
         lk.unlock();
-        run_request(ctx, q);
+        if (qs.size() == 1) {
+            run_request(ctx, *qs[0]);
+        } else {
+            std::vector<eik_request> rr;
+            for (Req* q : qs) rr.push_back(q->r);
+            const int rc = eik_traveltime_batch(ctx, (int32_t)rr.size(), rr.data());
+            if (rc == EIK_OK) {
+                for (Req* q : qs) q->status = EIK_OK;
+            } else {
+                for (Req* q : qs) run_request(ctx, *q);  // each request gets its own outcome
+            }
+        }
         lk.lock();
-        q.t_end = Clock::now();
-        q.done = true;
-        S->inflight_bytes -= q.bytes;
+        const Clock::time_point t1 = Clock::now();
+        for (Req* q : qs) {
+            q->t_end = t1;
+            q->done = true;
+        }
+        S->inflight_bytes -= bytes;
         S->cv_done.notify_all();
         S->cv_work.notify_all();
     }
@@ -203,8 +254,8 @@ int eik_server_stats(eik_server* S, int64_t* out, int32_t nout) {
     std::lock_guard<std::mutex> lk(S->mu);
     int64_t running = 0;
     for (auto& kv : S->reqs) running += kv.second.started && !kv.second.done;
-    const int64_t v[4] = {(int64_t)S->queue.size(), running, S->inflight_bytes, S->budget};
-    for (int i = 0; i < nout && i < 4; i++) out[i] = v[i];
+    const int64_t v[6] = {(int64_t)S->queue.size(), running, S->inflight_bytes, S->budget, S->nbatches, S->nbatched};
+    for (int i = 0; i < nout && i < 6; i++) out[i] = v[i];
     return EIK_OK;
 }
 

@@ -5,7 +5,7 @@ gradient path; the computation runs in libeik.so (hand-written HIP, include/eik.
 """
 from .grids import (ExtraParams, Grid2DCart, Grid2DSphere, Grid3DCart, Grid3DSphere, GridRefinementPars)
 from .api import (Context, EikModel, default_context, eikgradient, eikttimemisfit, eiktraveltime)
-from .serving import Request, Server
+from .serving import Request, Server, traveltime_batch
 from . import hmc, autodiff
 from .vtk import savemodelvtk
 from .rays import tracerays
@@ -16,5 +16,5 @@ HMCtraveltimes = hmc  # the reference's submodule name (src/HMCtraveltimes.jl)
 __all__ = [
     "ExtraParams", "GridRefinementPars", "Grid2DCart", "Grid3DCart", "Grid2DSphere", "Grid3DSphere",
     "eiktraveltime", "eikgradient", "eikttimemisfit", "EikModel", "Context", "default_context",
-    "Server", "Request", "HMCtraveltimes", "rrule_eikttimemisfit", "misfit_torch", "savemodelvtk", "tracerays",
+    "Server", "Request", "traveltime_batch", "HMCtraveltimes", "rrule_eikttimemisfit", "misfit_torch", "savemodelvtk", "tracerays",
 ]

@@ -66,6 +66,36 @@ class Request:
         return self._result
 
 
+def traveltime_batch(items, *, extraparams=None, ctx=None):
+    """Several ``(model, coordsrc, coordrec)`` traveltime requests in shared launches
+    (``eik_traveltime_batch``): the models share grid kind and dimension; returns each request's
+    picks as ``eiktraveltime`` would."""
+    from .api import default_context
+
+    ctx = ctx or default_context()
+    n = len(items)
+    reqs = (_lib.EikRequest * max(n, 1))()
+    keep, outs = [], []
+    for i, (model, coordsrc, coordrec) in enumerate(items):
+        grd = model.grd
+        cs, off, packed = _pack(coordsrc, coordrec, grd.ndim)
+        picks = np.zeros(max(int(off[-1]), 1))
+        q = reqs[i]
+        q.type = REQ_TRAVELTIME
+        q.model = model.handle
+        q.nsrc = cs.shape[0]
+        q.coordsrc = _lib.ptr(cs)
+        q.rec_off = _lib.ptr(off, ctypes.c_int64)
+        q.coordrec = _lib.ptr(packed)
+        q.params = _params_struct(extraparams)
+        q.ttpicks = _lib.ptr(picks)
+        keep.append((cs, off, packed))
+        outs.append((picks, off, cs.shape[0]))
+    rc = _lib.load().eik_traveltime_batch(ctx.handle, n, reqs)
+    _lib.check(rc, ctx.handle)
+    return [[p[o[s]:o[s + 1]].copy() for s in range(ns)] for p, o, ns in outs]
+
+
 class Server:
     """Continuous-batching request loop (``eik_server``) on one device."""
 
@@ -136,9 +166,10 @@ class Server:
         return r
 
     def stats(self) -> dict:
-        out = np.zeros(4, dtype=np.int64)
-        _lib.load().eik_server_stats(self.handle, _lib.ptr(out, ctypes.c_int64), 4)
-        return {"queued": int(out[0]), "running": int(out[1]), "admitted_bytes": int(out[2]), "budget": int(out[3])}
+        out = np.zeros(6, dtype=np.int64)
+        _lib.load().eik_server_stats(self.handle, _lib.ptr(out, ctypes.c_int64), 6)
+        return {"queued": int(out[0]), "running": int(out[1]), "admitted_bytes": int(out[2]), "budget": int(out[3]),
+                "shared_batches": int(out[4]), "requests_in_shared_batches": int(out[5])}
 
     def close(self):
         """Wait for every submitted request, then release the slots."""

@@ -170,8 +170,16 @@ int eik_submit(eik_server* server, const eik_request* request, int64_t* id);
  * times_ms (optional, 3 doubles): latency from submission, queueing delay, run time; errbuf
  * (optional) receives the error message. */
 int eik_wait(eik_server* server, int64_t id, double* times_ms, char* errbuf, int64_t errlen);
-/* out[0] queued requests, [1] running, [2] admitted working-set bytes, [3] budget */
+/* out[0] queued requests, [1] running, [2] admitted working-set bytes, [3] budget,
+ * [4] shared launches run for two or more traveltime requests, [5] requests run in them */
 int eik_server_stats(eik_server* server, int64_t* out, int32_t nout);
+
+/* Several traveltime requests solved in shared launches (the server's continuous batching; no
+ * reference counterpart).  Every request must be EIK_REQ_TRAVELTIME with the same params; the
+ * models must share grid kind and dimension (sizes, spacings and velocities may differ).  Each
+ * request's picks go to its own ttpicks; results equal those of separate eik_traveltime calls
+ * (FP64 bit for bit).  An invalid request fails the whole call. */
+int eik_traveltime_batch(eik_ctx* ctx, int32_t nreq, const eik_request* reqs);
 
 #ifdef __cplusplus
 }

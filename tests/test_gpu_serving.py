@@ -44,11 +44,13 @@ def test_results_same_alone_and_in_the_mix(eik, precision):
                 handles.append(srv.submit_gradient(m, r["src"], r["recs"], r["pickobs"], r["stdobs"],
                                                    "gradvelandsrcloc", extraparams=ep))
         mixed = [h.result() for h in handles]
+        st = srv.stats()
     # run intervals on the server clock, relative to each submission
     spans = [(t + h.queue_ms / 1e3, t + (h.queue_ms + h.run_ms) / 1e3) for t, h in zip(t_sub, handles)]
     overlaps = sum(1 for i in range(NREQ) for j in range(i + 1, NREQ)
                    if spans[i][0] < spans[j][1] and spans[j][0] < spans[i][1])
-    print(f"  latencies ms: {[round(h.latency_ms, 1) for h in handles]}; overlapping pairs: {overlaps}")
+    print(f"  latencies ms: {[round(h.latency_ms, 1) for h in handles]}; overlapping pairs: {overlaps}; "
+          f"{st}")
     assert overlaps > 0, "requests never ran concurrently"
     for r, a, b in zip(reqs, alone, mixed):
         if r["kind"] == "traveltime":
@@ -79,5 +81,57 @@ def test_request_errors_are_reported_per_request(eik):
         with pytest.raises(AssertionError):
             h_bad.result()
         assert len(h_ok.result()) == len(reqs[1]["src"])
+    for m in models:
+        m.close()
+
+
+def _tt_requests(nd, count):
+    out = []
+    rid = 0
+    while len(out) < count:
+        r = c5_request(rid, small=True)
+        rid += 1
+        if r["kind"] == "traveltime" and r["grd"].ndim == nd:
+            out.append(r)
+    return out
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("nd", [2, 3])
+def test_traveltime_batch_equals_separate_calls(eik, precision, nd):
+    """Requests on different models (sizes and velocities differ) in one shared-launch call give
+    the picks of separate calls: FP64 identical, mixed within the receiver tolerance."""
+    ep = ExtraParams(precision=precision)
+    reqs = _tt_requests(nd, 5)
+    models = [eik.EikModel(r["vel"], r["grd"]) for r in reqs]
+    alone = [m.traveltime(r["src"], r["recs"], extraparams=ep) for r, m in zip(reqs, models)]
+    batched = eik.traveltime_batch([(m, r["src"], r["recs"]) for r, m in zip(reqs, models)], extraparams=ep)
+    print(f"{nd}D {precision}: grids {[r['grd'].grsize[0] for r in reqs]}, sources {[len(r['src']) for r in reqs]}")
+    for a, b in zip(alone, batched):
+        for pa, pb in zip(a, b):
+            if precision == "fp64":
+                np.testing.assert_array_equal(pb, pa)
+            else:
+                np.testing.assert_allclose(pb, pa, rtol=1e-5, atol=0)
+    for m in models:
+        m.close()
+
+
+def test_server_batches_queued_traveltime_requests(eik):
+    """A one-slot server admits the first request and takes the compatible queued ones along into
+    one shared-launch call; every result equals the request run alone."""
+    ep = ExtraParams()
+    reqs = _tt_requests(2, 8)
+    models = [eik.EikModel(r["vel"], r["grd"]) for r in reqs]
+    alone = [m.traveltime(r["src"], r["recs"], extraparams=ep) for r, m in zip(reqs, models)]
+    with eik.Server(nslots=1) as srv:
+        hs = [srv.submit_traveltime(m, r["src"], r["recs"], extraparams=ep) for r, m in zip(reqs, models)]
+        res = [h.result() for h in hs]
+        st = srv.stats()
+    print(f"server stats: {st}")
+    assert st["shared_batches"] >= 1 and st["requests_in_shared_batches"] >= 2
+    for a, b in zip(alone, res):
+        for pa, pb in zip(a, b):
+            np.testing.assert_array_equal(pb, pa)
     for m in models:
         m.close()
