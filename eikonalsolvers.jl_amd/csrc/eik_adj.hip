@@ -391,8 +391,21 @@ __device__ __forceinline__ int find_ainst(const InstDev* __restrict__ inst, int 
 __global__ void k_adj_band_max(const uint8_t* __restrict__ flags, const unsigned* __restrict__ apend, int64_t ntiles,
                                const InstDev* __restrict__ inst, int ninst, unsigned* __restrict__ wmax) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ntiles || !flags[t]) return;
-    atomicMax(&wmax[find_ainst(inst, ninst, t)], apend[t]);
+    const bool on = t < ntiles && flags[t];
+    const int i = on ? find_ainst(inst, ninst, t) : -1;
+    unsigned v = on ? apend[t] : 0u;
+    // one atomic per wave and instance (the tiles of a wave almost always share one)
+    unsigned long long m = __ballot(on);
+    const int lane = threadIdx.x & 63;
+    while (m) {
+        const int leader = __ffsll((long long)m) - 1;
+        const int li = __shfl(i, leader);
+        unsigned r = (i == li) ? v : 0u;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) r = max(r, (unsigned)__shfl_xor((int)r, o));
+        if (lane == leader) atomicMax(&wmax[li], r);
+        m &= ~__ballot(i == li);
+    }
 }
 
 // Pass 2: list the flagged tiles within the band below their instance's highest pending value and
