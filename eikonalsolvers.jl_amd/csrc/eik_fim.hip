@@ -35,6 +35,9 @@ struct TileShape<3> {
 };
 static_assert(TileShape<2>::TX == fwd_tile_edge(2) && TileShape<3>::TX == fwd_tile_edge(3), "tile edge");
 constexpr int kTileThreads = 64;
+// A run of consecutive list entries claimed by one block is at most 1/kRunDiv of what is left per
+// block (and at most the EIK_CLAIM maximum).
+constexpr unsigned kRunDiv = 4;
 // Waves per SIMD the register allocation targets.  The kernel is bound by VALU/SALU issue, not
 // by latency: forcing 3 waves (168 registers) on the FP32 3D kernel measured 8 % slower than 2.
 template <int ND, typename T>
@@ -146,7 +149,7 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(kTileThreads, kTileTh
     const uint4* __restrict__ list, unsigned* __restrict__ count, const InstDev* __restrict__ inst,
     const unsigned* __restrict__ wmin, T* __restrict__ ttpool, T* __restrict__ kpool, T* __restrict__ spool,
     uint8_t* __restrict__ dpool, const T* __restrict__ slowpool, unsigned* __restrict__ flags,
-    unsigned* __restrict__ pend, unsigned long long* __restrict__ counters, T settle) {
+    unsigned* __restrict__ pend, unsigned long long* __restrict__ counters, T settle, unsigned chunk) {
     using S = TileShape<ND>;
     using L = TileLds<ND>;
     constexpr int NN = S::NODES, NL = L::NL, E = L::E;
@@ -171,12 +174,27 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(kTileThreads, kTileTh
         s_chg[0][NN] = EIK_INFF;
         s_chg[1][NN] = EIK_INFF;
     }
-    unsigned claim = 0u;
-    if (n == 0) claim = atomicAdd(&count[kCntCursor], 1u);
-    unsigned bi = (unsigned)__builtin_amdgcn_readfirstlane((int)claim);
+    // Entries are claimed in runs of consecutive list entries, which are neighbouring tiles (the
+    // list is built in tile order): a block relaxing a tile right after its neighbour sees that
+    // neighbour's new values, which saves work.  Runs shrink towards the end of the list (at most
+    // `chunk`, at least 1, about a quarter of what is left per block) so the blocks finish
+    // together.  The next run is claimed while the last tile of the current one runs.
+    const unsigned gsz = gridDim.x;
+    auto run_len = [&](unsigned at) {
+        const unsigned left = at < nlist ? nlist - at : 0u;
+        const unsigned r = left / (kRunDiv * gsz);
+        return r < 1u ? 1u : (r > chunk ? chunk : r);
+    };
+    unsigned claim = 0u, want = run_len(0u);
+    if (n == 0) claim = atomicAdd(&count[kCntCursor], want);
+    unsigned bi = (unsigned)__builtin_amdgcn_readfirstlane((int)claim), bend = bi + want;
     while (bi < nlist) {
         const uint4 e = list[bi];
-        if (n == 0) claim = atomicAdd(&count[kCntCursor], 1u);  // the next entry, consumed after this tile
+        const bool last = bi + 1 >= bend || bi + 1 >= nlist;
+        if (last) {
+            want = run_len(bi);
+            if (n == 0) claim = atomicAdd(&count[kCntCursor], want);
+        }
         const InstDev& I = inst[e.y];
         const float pend0 = __uint_as_float(e.z);
         const unsigned fbits = e.w;
@@ -402,7 +420,12 @@ int it = 0;
         tot_sweeps += (unsigned)it;
         tot_act++;
         __syncthreads();  // LDS is reused by the next tile
-        bi = (unsigned)__builtin_amdgcn_readfirstlane((int)claim);
+        if (last) {
+            bi = (unsigned)__builtin_amdgcn_readfirstlane((int)claim);
+            bend = bi + want;
+        } else {
+            bi++;
+        }
     }
     // per-launch statistics: one atomic per counter and block
     unsigned long long upd = tot_upd, chg = (unsigned long long)tot_vchg + tot_kchg, kchg = tot_kchg;
@@ -727,6 +750,18 @@ __global__ void k_round_reset(unsigned* __restrict__ wmin, int ninst, unsigned* 
     }
 }
 
+
+
+// Longest run of consecutive list entries a block claims at once (EIK_CLAIM overrides).
+static unsigned claim_chunk() {
+    static const unsigned v = [] {
+        const char* e = getenv("EIK_CLAIM");
+        const int c = e ? atoi(e) : 8;
+        return (unsigned)(c < 1 ? 1 : (c > 64 ? 64 : c));
+    }();
+    return v;
+}
+
 // Relative size below which an FP32 change wakes no other node (EIK_F32_SETTLE overrides).
 static double f32_settle() {
     static const double v = [] {
@@ -801,7 +836,7 @@ static int fim_run_t(eik_ctx* ctx, const InstDev* d_inst, int ninst, int64_t nti
             CK(hipEventRecord(ctx->kev[2 * r], st));
             k_fim_round<ND, T, SPH><<<g, kTileThreads, 0, st>>>((const uint4*)list.p, d_cnt, d_inst, (const unsigned*)wm.p,
                                                           (T*)P.tt, (T*)P.k, (T*)P.s, P.d, slowpool, flags, pend,
-                                                          d_c64, settle);
+                                                          d_c64, settle, claim_chunk());
             CK(hipGetLastError());
             CK(hipEventRecord(ctx->kev[2 * r + 1], st));
         }
