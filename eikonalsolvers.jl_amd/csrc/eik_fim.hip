@@ -178,7 +178,8 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(kTileThreads, kTileTh
     // list is built in tile order): a block relaxing a tile right after its neighbour sees that
     // neighbour's new values, which saves work.  Runs shrink towards the end of the list (at most
     // `chunk`, at least 1, about a quarter of what is left per block) so the blocks finish
-    // together.  The next run is claimed while the last tile of the current one runs.
+    // together.  Inside a run the tiles go in increasing pending value (upstream first).  The next
+    // run is claimed while the last tile of the current one runs.
     const unsigned gsz = gridDim.x;
     auto run_len = [&](unsigned at) {
         const unsigned left = at < nlist ? nlist - at : 0u;
@@ -187,12 +188,26 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(kTileThreads, kTileTh
     };
     unsigned claim = 0u, want = run_len(0u);
     if (n == 0) claim = atomicAdd(&count[kCntCursor], want);
-    unsigned bi = (unsigned)__builtin_amdgcn_readfirstlane((int)claim), bend = bi + want;
-    while (bi < nlist) {
+    unsigned rbase = (unsigned)__builtin_amdgcn_readfirstlane((int)claim);
+    unsigned rlen = rbase < nlist ? min(want, nlist - rbase) : 0u, rdone = 0u;
+    while (rbase < nlist) {
+        // the run's next tile: lowest pending value among those not yet relaxed
+        unsigned bi = rbase;
+        {
+            unsigned best = 0xffffffffu;
+            for (unsigned k = 0; k < rlen; k++) {
+                const unsigned pv = list[rbase + k].z;
+                if (!((rdone >> k) & 1u) && pv < best) {
+                    best = pv;
+                    bi = rbase + k;
+                }
+            }
+        }
+        rdone |= 1u << (bi - rbase);
         const uint4 e = list[bi];
-        const bool last = bi + 1 >= bend || bi + 1 >= nlist;
+        const bool last = (unsigned)__popc(rdone) == rlen;
         if (last) {
-            want = run_len(bi);
+            want = run_len(rbase + rlen);
             if (n == 0) claim = atomicAdd(&count[kCntCursor], want);
         }
         const InstDev& I = inst[e.y];
@@ -421,10 +436,9 @@ int it = 0;
         tot_act++;
         __syncthreads();  // LDS is reused by the next tile
         if (last) {
-            bi = (unsigned)__builtin_amdgcn_readfirstlane((int)claim);
-            bend = bi + want;
-        } else {
-            bi++;
+            rbase = (unsigned)__builtin_amdgcn_readfirstlane((int)claim);
+            rlen = rbase < nlist ? min(want, nlist - rbase) : 0u;
+            rdone = 0u;
         }
     }
     // per-launch statistics: one atomic per counter and block
@@ -757,7 +771,7 @@ static unsigned claim_chunk() {
     static const unsigned v = [] {
         const char* e = getenv("EIK_CLAIM");
         const int c = e ? atoi(e) : 8;
-        return (unsigned)(c < 1 ? 1 : (c > 64 ? 64 : c));
+        return (unsigned)(c < 1 ? 1 : (c > 32 ? 32 : c));
     }();
     return v;
 }
