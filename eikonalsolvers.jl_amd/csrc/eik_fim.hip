@@ -554,18 +554,6 @@ __global__ void k_fill_u32(unsigned* __restrict__ p, int64_t n, unsigned v) {
     if (i < n) p[i] = v;
 }
 
-// gather the set flags into a list (order irrelevant) and clear them
-__global__ void k_compact(uint8_t* __restrict__ flags, int64_t ntiles, int64_t* __restrict__ list,
-                          unsigned* __restrict__ count) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ntiles) return;
-    if (flags[t]) {
-        flags[t] = 0;
-        const unsigned pos = atomicAdd(count, 1u);
-        list[pos] = t;
-    }
-}
-
 template <typename T>
 __global__ void k_fill(T* __restrict__ p, int64_t n, T v) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -411,11 +411,8 @@ inline int forward_picks(eik_ctx* ctx, eik_model* M, const SourceSet& S, int pre
 int model_slowness(eik_model* M);
 int gradient_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, const eik_params* P, int whichgrad,
                    const double* pickobs, const double* stdobs, double* gradvel, double* gradsrc, double* misf);
-// tile-flag compaction (eik_fim.hip), also used by the adjoint rounds
+// fill kernel (eik_fim.hip), also used by the adjoint rounds
 __global__ void k_fill_u32(unsigned* __restrict__ p, int64_t n, unsigned v);
-__global__ void k_compact(uint8_t* __restrict__ flags, int64_t ntiles, int64_t* __restrict__ list,
-                          unsigned* __restrict__ count);
-
 struct FimStats {
     int64_t rounds = 0;        // solver rounds (one launch each)
     double activations = 0;    // tile activations (blocks launched)
