@@ -249,6 +249,15 @@ int eik_wait(eik_server* S, int64_t id, double* times_ms, char* errbuf, int64_t 
     return st;
 }
 
+int eik_poll(eik_server* S, int64_t id, int32_t* done) {
+    if (!S || !done) return EIK_EINVAL;
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto it = S->reqs.find(id);
+    if (it == S->reqs.end()) return EIK_EINVAL;
+    *done = it->second.done ? 1 : 0;
+    return EIK_OK;
+}
+
 int eik_server_stats(eik_server* S, int64_t* out, int32_t nout) {
     if (!S || !out) return EIK_EINVAL;
     std::lock_guard<std::mutex> lk(S->mu);

@@ -28,7 +28,7 @@ EXPORTS = (
     "eik_model_create", "eik_model_update_vel", "eik_model_destroy",
     "eik_traveltime", "eik_gradient", "eik_misfit", "eik_last_stats",
     "eik_server_create", "eik_server_destroy", "eik_submit", "eik_wait", "eik_server_stats",
-    "eik_tracerays", "eik_traveltime_batch",
+    "eik_tracerays", "eik_traveltime_batch", "eik_poll",
 )
 
 
@@ -113,6 +113,8 @@ def load():
         L.eik_tracerays.restype = ctypes.c_int
         L.eik_traveltime_batch.argtypes = [vp, ctypes.c_int32, P(EikRequest)]
         L.eik_traveltime_batch.restype = ctypes.c_int
+        L.eik_poll.argtypes = [vp, ctypes.c_int64, P(ctypes.c_int32)]
+        L.eik_poll.restype = ctypes.c_int
         _lib = L
         return L
 

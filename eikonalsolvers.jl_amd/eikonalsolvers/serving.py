@@ -41,6 +41,14 @@ class Request:
         self._done = False
         self._result = None
 
+    def done(self) -> bool:
+        """True once the request has finished (non-blocking, ``eik_poll``)."""
+        if self._done:
+            return True
+        d = ctypes.c_int32(0)
+        _lib.check(_lib.load().eik_poll(self._server.handle, self.id, ctypes.byref(d)), None, "eik_poll failed")
+        return bool(d.value)
+
     def result(self):
         """Wait for completion and return what ``eiktraveltime`` / ``eikgradient`` return."""
         if self._done:

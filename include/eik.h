@@ -170,6 +170,9 @@ int eik_submit(eik_server* server, const eik_request* request, int64_t* id);
  * times_ms (optional, 3 doubles): latency from submission, queueing delay, run time; errbuf
  * (optional) receives the error message. */
 int eik_wait(eik_server* server, int64_t id, double* times_ms, char* errbuf, int64_t errlen);
+/* Non-blocking: *done = 1 when request id has finished (eik_wait then returns at once), else 0.
+ * EIK_EINVAL for an unknown id (never submitted, or already waited for). */
+int eik_poll(eik_server* server, int64_t id, int32_t* done);
 /* out[0] queued requests, [1] running, [2] admitted working-set bytes, [3] budget,
  * [4] shared launches run for two or more traveltime requests, [5] requests run in them */
 int eik_server_stats(eik_server* server, int64_t* out, int32_t nout);

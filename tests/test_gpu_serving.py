@@ -126,6 +126,10 @@ def test_server_batches_queued_traveltime_requests(eik):
     alone = [m.traveltime(r["src"], r["recs"], extraparams=ep) for r, m in zip(reqs, models)]
     with eik.Server(nslots=1) as srv:
         hs = [srv.submit_traveltime(m, r["src"], r["recs"], extraparams=ep) for r, m in zip(reqs, models)]
+        t0 = time.perf_counter()
+        while not all(h.done() for h in hs):  # eik_poll, without blocking
+            assert time.perf_counter() - t0 < 60.0
+            time.sleep(0.002)
         res = [h.result() for h in hs]
         st = srv.stats()
     print(f"server stats: {st}")
