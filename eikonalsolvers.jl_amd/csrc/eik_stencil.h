@@ -255,6 +255,13 @@ __device__ __forceinline__ double tie_below(double x) { return x; }
 __device__ __forceinline__ float tie_below(float x) {
     return fabsf(x) < EIK_INFF ? x - kTieUlps * 1.1920929e-7f * fabsf(x) : x;
 }
+// Whether an update `cur` triggered by a neighbour popping at `c` pops after it as an ordinary node
+// (key = its own value).  FP64: only when strictly above; an update landing exactly on its trigger
+// pops in the heap's next batch, i.e. right after the trigger's, which the cascade key (c, cur, 1)
+// expresses (an ordinary key equal to c would put it in the trigger's own batch).  FP32: within
+// the tie margin counts as above.
+__device__ __forceinline__ bool pops_after(double cur, double c) { return cur > c; }
+__device__ __forceinline__ bool pops_after(float cur, float c) { return cur >= tie_below(c); }
 
 template <int ND>
 __device__ __forceinline__ double eval_any(const double* v1, const double* v2, unsigned acc1, unsigned acc2, double s,
@@ -307,7 +314,7 @@ __device__ __forceinline__ T replay_plain(const T* v1, const T* v2, unsigned can
             acc2 |= (unsigned)a2 << k;
         }
         cur = eval_any<ND>(v1, v2, acc1, acc2, s, m, code);
-        const bool ord = cur >= tie_below(c);
+        const bool ord = pops_after(cur, c);
         kzK = ord ? cur : c;
         kzS = ord ? -tinf<T>() : cur;
         kzD = ord ? 0 : 1;
@@ -375,7 +382,7 @@ __device__ __forceinline__ T replay(const T* v1, const T* v2, KeyAt keys, unsign
             acc2 |= (unsigned)a2 << k;
         }
         cur = eval_any<ND>(v1, v2, acc1, acc2, s, m, code);
-        const bool ord = cur >= tie_below(c.K);
+        const bool ord = pops_after(cur, c.K);
         kz.K = ord ? cur : c.K;
         kz.S = ord ? -tinf<T>() : (cur > c.S ? cur : c.S);
         kz.D = ord ? 0 : c.D + 1;
