@@ -126,3 +126,24 @@ def test_symmetric_constant_velocity(eik, oracle, nd, refine):
         else:
             np.testing.assert_array_equal(pg[s], po[s])
         assert rel.max() <= 1e-12
+
+
+@pytest.mark.parametrize("refine", [False, True])
+def test_gradient_on_node_and_halfway_sources(eik, oracle, refine):
+    """The adjoint at on-node, border and halfway-between-fine-nodes sources (3D), against the
+    oracle's gradient (the derivative codes and the system order come from the same replay)."""
+    grd = Grid3DCart(hgrid=1.0, cooinit=(0.0, 0.0, 0.0), grsize=(20, 18, 16))
+    vel = smooth_field(grd.grsize, 25, passes=4)
+    src = np.array([[5.0, 4.3, 3.0], [0.0, 8.2, 7.1], [9.5, 8.5, 7.5], [11.3, 0.0, 15.0]])
+    rec = np.array([[19.0, 3.2, 0.0], [7.7, 17.0, 0.0], [12.3, 8.9, 4.4], [2.0, 2.0, 0.0]])
+    recs = [rec.copy() for _ in src]
+    ep = ExtraParams(refinearoundsrc=refine)
+    picks = oracle.traveltime(vel * 1.02, grd, src, recs, extraparams=ep)
+    std = [np.full(len(p), 0.01) for p in picks]
+    gv, gs, m = eik.eikgradient(vel, grd, src, recs, picks, std, "gradvelandsrcloc", extraparams=ep)
+    ov, os_, mo = oracle.gradient(vel, grd, src, recs, picks, std, "gradvelandsrcloc", extraparams=ep, nthreads=1)
+    ev = np.linalg.norm(gv - ov) / np.linalg.norm(ov)
+    es = np.linalg.norm(gs - os_) / np.linalg.norm(os_)
+    print(f"refine={refine}: misfit {m:.10e} vs {mo:.10e}, gradvel rel L2 {ev:.2e}, gradsrc rel L2 {es:.2e}")
+    assert m == pytest.approx(mo, rel=1e-12)
+    assert ev <= 1e-9 and es <= 1e-9
