@@ -147,3 +147,27 @@ def test_gradient_on_node_and_halfway_sources(eik, oracle, refine):
     print(f"refine={refine}: misfit {m:.10e} vs {mo:.10e}, gradvel rel L2 {ev:.2e}, gradsrc rel L2 {es:.2e}")
     assert m == pytest.approx(mo, rel=1e-12)
     assert ev <= 1e-9 and es <= 1e-9
+
+
+@pytest.mark.parametrize("df,radius", [(1, 3), (2, 0), (7, 2), (5, 8)])
+@pytest.mark.parametrize("nd", [2, 3])
+def test_refinement_parameters_and_small_grids(eik, oracle, nd, df, radius):
+    """Unusual GridRefinementPars (downscalefactor 1..7, noderadius 0..8: boxes clipped by the
+    model on both sides) on small grids with sharp velocity contrasts (many non-causal 2nd-order
+    updates, i.e. cascade keys): FP64 against the oracle, bit for bit."""
+    from eikonalsolvers.grids import GridRefinementPars
+    rng = np.random.default_rng(31 + df + 10 * radius + nd)
+    n = (9, 7) if nd == 2 else (7, 6, 5)
+    grd = (Grid2DCart(hgrid=1.0, cooinit=(0.0, 0.0), grsize=n) if nd == 2
+           else Grid3DCart(hgrid=1.0, cooinit=(0.0, 0.0, 0.0), grsize=n))
+    vel = np.where(rng.uniform(size=n) < 0.5, 1.5, 6.0)
+    L = np.array([k - 1 for k in n], dtype=float)
+    src = rng.uniform(0.0, 1.0, size=(3, nd)) * L
+    recs = [rng.uniform(0.0, 1.0, size=(5, nd)) * L for _ in range(3)]
+    ep = ExtraParams(refinearoundsrc=True, grdrefpars=GridRefinementPars(downscalefactor=df, noderadius=radius))
+    pg, tg = eik.eiktraveltime(vel, grd, src, recs, returntt=True, extraparams=ep)
+    po, to = oracle.traveltime(vel, grd, src, recs, returntt=True, extraparams=ep)
+    for s in range(3):
+        rel = np.abs(tg[s] - to[s]) / np.maximum(np.abs(to[s]), 1e-300)
+        assert rel.max() <= 1e-12, (s, rel.max())
+        np.testing.assert_allclose(pg[s], po[s], rtol=1e-12, atol=0)
