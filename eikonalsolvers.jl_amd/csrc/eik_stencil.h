@@ -274,6 +274,209 @@ __device__ __forceinline__ float eval_any(const float* v1, const float* v2, unsi
     return eval32<ND>(v1, v2, acc1, acc2, s, m, code);
 }
 
+// ---------------------------------------------------------------- replay by axes
+// Without init neighbours and cascade keys the candidates are accepted in increasing value order,
+// so once an axis enters the stencil its side (the smaller candidate, + side on ties) and its
+// 2nd-order choice stay fixed: a later candidate on that axis is never chosen, and the chosen
+// side's 2-away node is usable iff it is off the boundary planes and not above the chosen value
+// (it was then accepted no later than the chosen node).  The replay's evaluations therefore differ
+// only in which axes are in use.  Each axis' contribution to the quadratic is formed once, and the
+// replay takes one step per axis instead of one per candidate: a candidate that adds no axis
+// re-evaluates the same sums, so the value and the key stay as they are.
+//
+// One case breaks this (`odd`): the - side wins without a usable 2-away node of its own while the
+// + side is a candidate with one; pick_axis then takes the + side's 2nd-order choice once the + side
+// is accepted (calcttpt_2ndord! does not reset it).  A wave with such a lane, or with a lane next
+// to an init node, runs the per-candidate replay.
+template <typename T>
+struct AxisTerm {
+    T cv1;          // chosen neighbour's value: the axis enters at this candidate (+inf: never)
+    T ca, bt, gt;   // contributions to alpha, beta and gamma (eval64's or eval32's form)
+    int codeA, codeB;
+};
+
+template <int ND, typename T>
+__device__ __forceinline__ bool axis_terms(const T* v1, const T* v2, unsigned cand, unsigned ok2,
+                                           const Metric<T>& m, AxisTerm<T>* at) {
+#pragma clang fp contract(off)
+    bool odd = false;
+    bool sc[ND];
+    T w2[ND];
+    T r = tinf<T>();
+#pragma unroll
+    for (int ax = 0; ax < ND; ax++) {
+        const int kp = 2 * ax, km = 2 * ax + 1;
+        const bool cp = (cand >> kp) & 1u, cm = (cand >> km) & 1u;
+        const bool okp = (ok2 >> kp) & 1u, okm = (ok2 >> km) & 1u;
+        const T vp = cp ? v1[kp] : tinf<T>();
+        const bool useM = cm & (v1[km] < vp);
+        const T cv1 = useM ? v1[km] : vp;
+        w2[ax] = useM ? v2[km] : v2[kp];
+        sc[ax] = (useM ? okm : okp) & (w2[ax] <= cv1) & (cv1 < tinf<T>());
+        odd |= useM & !sc[ax] & cp & okp & (v2[kp] <= v1[kp]);
+        at[ax].cv1 = cv1;
+        at[ax].codeA = useM ? (sc[ax] ? 2 : 1) : (cp ? (sc[ax] ? -2 : -1) : 0);
+        at[ax].codeB = useM ? 1 : (cp ? -1 : 0);
+        r = (cv1 < r) ? cv1 : r;
+    }
+#pragma unroll
+    for (int ax = 0; ax < ND; ax++) {
+        const T cv1 = at[ax].cv1;
+        if constexpr (sizeof(T) == 8) {
+            const double tmpa2 = 1.0 / 3.0 * (4.0 * cv1 - w2[ax]);
+            const double ca = sc[ax] ? m.c2[ax] : m.c1[ax];
+            const double tv = sc[ax] ? tmpa2 : cv1;
+            at[ax].ca = ca;
+            at[ax].bt = -2.0 * ca * tv;
+            at[ax].gt = ca * (tv * tv);
+        } else {
+            const float al = sc[ax] ? m.c2[ax] : m.c1[ax];
+            const float d1 = cv1 - r;
+            const float d = sc[ax] ? d1 + (cv1 - w2[ax]) * (1.0f / 3.0f) : d1;
+            at[ax].ca = al;
+            at[ax].bt = al * d;
+            at[ax].gt = al * d * d;
+        }
+    }
+    return odd;
+}
+
+// eval64 / eval32 over the axes in `on` from their precomputed contributions (same operations in
+// the same order, so the same result).
+template <int ND>
+__device__ __forceinline__ double eval_on(const AxisTerm<double>* at, unsigned on, double s,
+                                          const Metric<double>& m, int* code) {
+#pragma clang fp contract(off)
+    double alpha = 0.0, beta = 0.0, gamma = -(s * s);
+#pragma unroll
+    for (int ax = 0; ax < ND; ax++) {
+        const bool u = (on >> ax) & 1u;
+        alpha = u ? alpha + at[ax].ca : alpha;
+        beta = u ? beta + at[ax].bt : beta;
+        gamma = u ? gamma + at[ax].gt : gamma;
+        code[ax] = u ? at[ax].codeA : 0;
+    }
+    const double sqA = beta * beta - 4.0 * alpha * gamma;
+    double a = alpha, b = beta, q = sqA, oned = 0.0;
+    bool useoned = false;
+    if (__ballot(sqA < 0.0) != 0ull) {
+        double alpha1 = 0.0, beta1 = 0.0, gamma1 = -(s * s);
+        double cmin = EIK_INF, hmin = m.dh[0];
+        int idc = 0;
+#pragma unroll
+        for (int ax = 0; ax < ND; ax++) {
+            const bool u = (on >> ax) & 1u;
+            const double cv1 = u ? at[ax].cv1 : EIK_INF;
+            const double cb = m.c1[ax];
+            alpha1 = u ? alpha1 + cb : alpha1;
+            beta1 = u ? beta1 + (-2.0 * cb * cv1) : beta1;
+            gamma1 = u ? gamma1 + cb * (cv1 * cv1) : gamma1;
+            const bool better = (ax == 0) || cv1 < cmin;
+            idc = better ? ax : idc;
+            cmin = better ? cv1 : cmin;
+            hmin = better ? m.dh[ax] : hmin;
+        }
+        const double sqB = beta1 * beta1 - 4.0 * alpha1 * gamma1;
+        const bool useA = !(sqA < 0.0);
+        const bool useB = !useA && !(sqB < 0.0);
+        a = useA ? alpha : alpha1;
+        b = useA ? beta : beta1;
+        q = useA ? sqA : sqB;
+        useoned = !useA && !useB;
+        oned = (hmin * s) + cmin;
+#pragma unroll
+        for (int d = 0; d < ND; d++) {
+            const int cB = ((on >> d) & 1u) ? at[d].codeB : 0;
+            code[d] = useA ? code[d] : (useB ? cB : (d == idc ? cB : 0));
+        }
+    }
+    const double t = (-b + sqrt(q)) / (2.0 * a);
+    return useoned ? oned : t;
+}
+template <int ND>
+__device__ __forceinline__ float eval_on(const AxisTerm<float>* at, unsigned on, float s, const Metric<float>& m,
+                                         int* code) {
+    float r = EIK_INFF;
+#pragma unroll
+    for (int ax = 0; ax < ND; ax++) r = ((on >> ax) & 1u) ? fminf(r, at[ax].cv1) : r;
+    float A = 0.f, B = 0.f, C = -(s * s);
+#pragma unroll
+    for (int ax = 0; ax < ND; ax++) {
+        const bool u = (on >> ax) & 1u;
+        A = u ? A + at[ax].ca : A;
+        B = u ? B + at[ax].bt : B;
+        C = u ? C + at[ax].gt : C;
+        code[ax] = u ? at[ax].codeA : 0;
+    }
+    const float discA = B * B - A * C;
+    float a = A, b = B, q = discA, oned = 0.f;
+    bool useoned = false;
+    if (__ballot(discA < 0.f) != 0ull) {
+        float A1 = 0.f, B1 = 0.f, C1 = -(s * s);
+        float cmin = EIK_INFF, hmin = m.dh[0];
+        int idc = 0;
+#pragma unroll
+        for (int ax = 0; ax < ND; ax++) {
+            const bool u = (on >> ax) & 1u;
+            const float cv1 = u ? at[ax].cv1 : EIK_INFF;
+            const float ih2 = m.c1[ax];
+            const float d1 = cv1 - r;
+            A1 = u ? A1 + ih2 : A1;
+            B1 = u ? B1 + ih2 * d1 : B1;
+            C1 = u ? C1 + ih2 * d1 * d1 : C1;
+            const bool better = (ax == 0) || cv1 < cmin;
+            idc = better ? ax : idc;
+            cmin = better ? cv1 : cmin;
+            hmin = better ? m.dh[ax] : hmin;
+        }
+        const float discB = B1 * B1 - A1 * C1;
+        const bool useA = !(discA < 0.f);
+        const bool useB = !useA && !(discB < 0.f);
+        a = useA ? A : A1;
+        b = useA ? B : B1;
+        q = useA ? discA : discB;
+        useoned = !useA && !useB;
+        oned = cmin + hmin * s;
+#pragma unroll
+        for (int d = 0; d < ND; d++) {
+            const int cB = ((on >> d) & 1u) ? at[d].codeB : 0;
+            code[d] = useA ? code[d] : (useB ? cB : (d == idc ? cB : 0));
+        }
+    }
+    const float t = r + (b + sqrtf(q)) / a;
+    return useoned ? oned : t;
+}
+
+// replay_plain's sequence, one step per axis (see above).
+template <int ND, typename T>
+__device__ __forceinline__ T replay_axes(const AxisTerm<T>* at, T s, const Metric<T>& m, int* code,
+                                         Key<T>* kout) {
+    T cur = tinf<T>();
+    T kzK = tinf<T>(), kzS = -tinf<T>();
+    int kzD = 0;
+#pragma unroll
+    for (int d = 0; d < ND; d++) code[d] = 0;
+    T prev = -tinf<T>();
+    unsigned on = 0u;
+#pragma unroll 1
+    for (int step = 0; step < ND; step++) {
+        T c = tinf<T>();
+#pragma unroll
+        for (int ax = 0; ax < ND; ax++) c = (at[ax].cv1 > prev && at[ax].cv1 < c) ? at[ax].cv1 : c;
+        if (!(c < tinf<T>()) || kzD != 0 || !(c < tie_below(kzK))) break;
+        prev = c;
+#pragma unroll
+        for (int ax = 0; ax < ND; ax++) on |= (unsigned)(at[ax].cv1 == c) << ax;
+        cur = eval_on<ND>(at, on, s, m, code);
+        const bool ord = pops_after(cur, c);
+        kzK = ord ? cur : c;
+        kzS = ord ? -tinf<T>() : cur;
+        kzD = ord ? 0 : 1;
+    }
+    *kout = Key<T>{kzK, kzS, kzD};
+    return cur;
+}
+
 // Local replay of the heap (see the file comment).  v1/v2: adjacent / 2-away values; k1/k2 their
 // keys; f1/f2 fixed (init) bits; ok1/ok2 grid-boundary bits.  Returns the value and sets *kout to
 // the node's pop key and code to the derivative codes of the last stencil evaluation.
@@ -344,7 +547,14 @@ __device__ __forceinline__ T replay(const T* v1, const T* v2, KeyAt keys, unsign
         const bool reached2 = (bool)((ok2 >> k) & 1u) & !((f2 >> k) & 1u) & (v2[k] < tinf<T>());
         keyed |= (reached1 & (bool)((d1 >> k) & 1u)) | (reached2 & (bool)((d2 >> k) & 1u));
     }
-    if (__ballot(keyed) == 0ull) return replay_plain<ND, T>(v1, v2, cand, haveinit, f1, f2, ok1, ok2, s, m, code, kout);
+    if (__ballot(keyed) == 0ull) {
+        if (__ballot(haveinit) == 0ull) {
+            AxisTerm<T> at[ND];
+            const bool odd = axis_terms<ND, T>(v1, v2, cand, ok2, m, at);
+            if (__ballot(odd) == 0ull) return replay_axes<ND, T>(at, s, m, code, kout);
+        }
+        return replay_plain<ND, T>(v1, v2, cand, haveinit, f1, f2, ok1, ok2, s, m, code, kout);
+    }
     T cur = tinf<T>();
     Key<T> kz{tinf<T>(), -tinf<T>(), 0};
 #pragma unroll
