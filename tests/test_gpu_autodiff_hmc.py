@@ -81,7 +81,7 @@ def test_misfit_torch_backward(eik, prob):
     v = torch.tensor(vel, dtype=torch.float64, requires_grad=True)
     c = torch.tensor(src, dtype=torch.float64, requires_grad=True)
     f = misfit_torch(v, c, m, recs, obs, std)
-    assert float(f) == pytest.approx(misf, rel=1e-12)
+    assert float(f.detach()) == pytest.approx(misf, rel=1e-12)
     (3.0 * f).backward()
     np.testing.assert_allclose(v.grad.numpy(), 3.0 * gv, rtol=1e-12, atol=0)
     np.testing.assert_allclose(c.grad.numpy(), 3.0 * gs, rtol=1e-12, atol=0)
