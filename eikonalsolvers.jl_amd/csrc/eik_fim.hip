@@ -791,13 +791,19 @@ static int fim_run_t(eik_ctx* ctx, const InstDev* d_inst, int ninst, int64_t nti
     unsigned long long* d_c64 = (unsigned long long*)(d_cnt + kCnt64);
     hipStream_t st = ctx->stream;
     // resident blocks of this instantiation on the device (EIK_GRID overrides); the static is
-    // initialised once, thread-safely, by the first caller
+    // initialised once, thread-safely, by the first caller.  More concurrent tiles are not
+    // faster: neighbouring tiles relaxed at the same time see each other's old values, and the
+    // runs each block claims get shorter.  Measured optimum per CU: 8 blocks in 3D (201x201x101
+    // and 120x150x150 grids: 6 blocks 5 % slower on the first, 2 % faster on the second), 4-6 in
+    // 2D (600x400 x 256 sources: 8 blocks 15 % slower in FP64; FP32 occupancy allows 12, 35 %
+    // slower than 6).
     static const int grid = [ctx] {
         int ncu = 0, occ = 0;
         if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess) ncu = 1;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_fim_round<ND, T, SPH>, kTileThreads, 0) != hipSuccess)
             occ = 1;
-        int gr = std::max(1, ncu * std::max(1, occ));
+        constexpr int kMaxPerCu = ND == 3 ? 8 : 5;
+        int gr = std::max(1, ncu * std::min(kMaxPerCu, std::max(1, occ)));
         if (const char* e = getenv("EIK_GRID")) gr = std::max(1, atoi(e));
         if (getenv("EIK_VERBOSE"))
             fprintf(stderr, "libeik: k_fim_round<%d,%s,%d> %d CUs x %d blocks -> grid %d\n", ND,
