@@ -31,8 +31,7 @@ __device__ __forceinline__ double axis_h(const InstDev& I, const int* g, int d) 
     if (I.kind == EIK_GRID2D_CART || I.kind == EIK_GRID3D_CART || d == 0) return I.h[0];
     const double r = fma((double)g[0], I.h[0], I.o[0]);
     if (d == 1) return r * I.dthrad;
-    const double th = fma((double)g[1], I.h[1], I.o[1]);
-    return r * sin(deg2rad(th)) * I.dphrad;
+    return r * I.sinth[g[1]] * I.dphrad;  // the forward metric's sin(theta_j) table
 }
 
 // coefficient of stencil position p (0 = the row's own node) for code c on axis d
@@ -653,7 +652,11 @@ __device__ __forceinline__ double taper_factor(const TaperDev& T, const InstDev&
     for (int d = 0; d < nd; d++)
         if (g[d] < T.lo[d] || g[d] > T.hi[d]) return 1.0;
     double r;
-    if (kind == EIK_GRID2D_SPHERE) {
+    if (kind == EIK_GRID3D_SPHERE) {  // chord distance, as the 3D spherical forward measures it
+        const double a[3] = {fma((double)g[0], I.h[0], I.o[0]), fma((double)g[1], I.h[1], I.o[1]),
+                             fma((double)g[2], I.h[2], I.o[2])};
+        r = distance2points(kind, 3, a, T.src);
+    } else if (kind == EIK_GRID2D_SPHERE) {
         const double xs = T.src[0] * cos(deg2rad(T.src[1])), ys = T.src[0] * sin(deg2rad(T.src[1]));
         const double rr = fma((double)g[0], I.h[0], I.o[0]), th = fma((double)g[1], I.h[1], I.o[1]);
         const double xc = rr * cos(deg2rad(th)), yc = rr * sin(deg2rad(th));
@@ -895,7 +898,10 @@ static void taper_box(const eik_grid& g, const double* src, int R, TaperDev& T) 
         T.hi[0] = c[1][0] + R;
         T.lo[1] = c[0][1] - R;
         T.hi[1] = c[2][1] + R;
-        if (g.ndim == 3) {  // the reference builds the k range from the i index (utils.jl:257-258)
+        if (g.ndim == 3 && g.kind == EIK_GRID3D_SPHERE) {  // no reference: the box's own k range
+            T.lo[2] = c[0][2] - R;
+            T.hi[2] = c[nc - 1][2] + R;
+        } else if (g.ndim == 3) {  // the reference builds the k range from the i index (utils.jl:257-258)
             T.lo[2] = c[0][0] - R;
             T.hi[2] = c[3][0] + R;
         }

@@ -392,9 +392,9 @@ int eik_gradient(eik_ctx* ctx, eik_model* model, int64_t nsrc, const double* coo
     const eik_params pe = effective_params(params_in);
     const eik_params* params = &pe;
     const int kind = model->g.kind;
-    if (kind == EIK_GRID3D_SPHERE)
-        return ctx->fail(EIK_EINVAL, "createsparsederivativematrices!(): Not yet implemented for Grid3DSphere.");
-    if (kind == EIK_GRID2D_SPHERE && whichgrad != EIK_GRADVEL)
+    // Grid3DSphere: the reference raises here (createsparsederivativematrices!, eikmainfwd.jl:885-887);
+    // this build differentiates its own 3D spherical forward (r sin(theta) dphi metric), :gradvel only
+    if ((kind == EIK_GRID2D_SPHERE || kind == EIK_GRID3D_SPHERE) && whichgrad != EIK_GRADVEL)
         return ctx->fail(EIK_EINVAL, "gradsrcloc is not available on spherical grids");
     if (S.nrec() > 0 && (!pickobs || !stdobs)) return ctx->fail(EIK_EINVAL, "null observations");
     if (whichgrad != EIK_GRADSRCLOC && !gradvel) return ctx->fail(EIK_EINVAL, "null gradvel buffer");

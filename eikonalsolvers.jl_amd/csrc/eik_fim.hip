@@ -923,11 +923,27 @@ static double adjband_mult() {
     return m;
 }
 
+// A node crossing is the mean node spacing over the axes times the mean slowness.  On spherical
+// grids the angular spacings are arcs at the middle radius (and middle colatitude for phi): on a
+// 120x150x150 grid with dr = 2 km and 0.1 degree steps at r ~ 6250 km they are ~11 km, and a band
+// counted in crossings of dr alone was 13 % (FP64) / 24 % (FP32) slower.
+static double node_spacing_mean(const InstDev& I, int nd) {
+    if (I.kind != EIK_GRID2D_SPHERE && I.kind != EIK_GRID3D_SPHERE) return I.h[0];
+    const double rm = I.o[0] + 0.5 * (I.n[0] - 1) * I.h[0];
+    double sum = I.h[0] + rm * I.dthrad;
+    if (nd == 3) {
+        const double thm = (I.o[1] + 0.5 * (I.n[1] - 1) * I.h[1]) * (M_PI / 180.0);
+        sum += rm * sin(thm) * I.dphrad;
+    }
+    return sum / nd;
+}
+
 static void set_band(InstDev& I, int nd, double smean) {
     const double m = band_mult(nd), mn = nodeband_mult(), ma = adjband_mult();
-    I.band = m > 0.0 ? m * I.h[0] * smean : INFINITY;
-    I.nband = mn > 0.0 ? mn * I.h[0] * smean : INFINITY;
-    I.aband = ma > 0.0 ? ma * I.h[0] * smean : INFINITY;
+    const double cross = node_spacing_mean(I, nd) * smean;
+    I.band = m > 0.0 ? m * cross : INFINITY;
+    I.nband = mn > 0.0 ? mn * cross : INFINITY;
+    I.aband = ma > 0.0 ? ma * cross : INFINITY;
 }
 
 // distance from a source-box corner to the source, on the host (sourceboxloctt!,

@@ -1223,10 +1223,25 @@ static void taper(const og_grid* g, double* grad, const double* src, int R) {
         imax = corn[1][0] + R;
         jmin = corn[0][1] - R;
         jmax = corn[2][1] + R;
-        if (nd == 3) {
+        if (nd == 3 && g->kind == OG_SPH3D) { /* no reference: the box's own k range */
+            kmin = corn[0][2] - R;
+            kmax = corn[nc - 1][2] + R;
+        } else if (nd == 3) {
             kmin = corn[0][0] - R;
             kmax = corn[3][0] + R;
         }
+    }
+    if (g->kind == OG_SPH3D) { /* chord distance, as the 3D spherical forward measures it */
+        rmax = R * g->d[0];
+        for (int64_t k = kmin; k <= kmax; k++)
+            for (int64_t j = jmin; j <= jmax; j++)
+                for (int64_t i = imin; i <= imax; i++) {
+                    if (i < 0 || i > g->n[0] - 1 || j < 0 || j > g->n[1] - 1 || k < 0 || k > g->n[2] - 1) continue;
+                    const double a[3] = {coord(g, 0, i), coord(g, 1, j), coord(g, 2, k)};
+                    const double r = distance2points(g, a, src);
+                    if (r <= rmax) grad[i + g->n[0] * (j + g->n[1] * k)] *= r / rmax;
+                }
+        return;
     }
     if (g->kind == OG_SPH2D) {
         rmax = R * g->d[0];
@@ -1593,7 +1608,8 @@ int og_gradient(const og_grid* g, const double* vel, int64_t nsrc, const double*
     if (check_inputs(g, vel, nsrc, coordsrc, rec_off, coordrec, p)) return -1;
     for (int64_t r = 0; r < rec_off[nsrc]; r++)
         if (!(stdobs[r] != 0.0)) return fail("stdobs must be non-zero");
-    if (g->kind == OG_SPH3D) return fail("createsparsederivativematrices!(): Not yet implemented for Grid3DSphere.");
+    /* OG_SPH3D: the reference raises (createsparsederivativematrices!, eikmainfwd.jl:885-887); this
+     * build differentiates its own 3D spherical forward, :gradvel only (as on 2D spheres) */
     if (is_sph(g) && whichgrad != OG_GRADVEL) return fail("gradsrcloc is not available on spherical grids");
     const int64_t N = npts(g);
     int64_t nt = p->nthreads > 1 ? p->nthreads : 1;

@@ -1,7 +1,7 @@
 """Deterministic test models shared by the CPU and GPU tests (seeded numpy generators)."""
 import numpy as np
 
-from eikonalsolvers.grids import Grid2DCart, Grid2DSphere, Grid3DCart
+from eikonalsolvers.grids import Grid3DSphere, Grid2DCart, Grid2DSphere, Grid3DCart
 
 
 def smooth_field(shape, seed, passes=8, lo=1.5, hi=6.0, base=3.0, amp=1.0):
@@ -50,6 +50,17 @@ def random3d(seed=2, n=(40, 36, 28), h=1.0, nsrc=2, nrec=10):
         r[:, 2] = 0.0  # top face
         recs.append(r)
     return grd, vel, src, recs
+
+
+def sphere3d(seed=11):
+    """Small 3D spherical case far from the equator (theta 30-34.6 deg), so the sin(theta) of the
+    phi metric is ~0.5 and matters."""
+    grd = Grid3DSphere(Δr=2.0, Δθ=0.2, Δφ=0.2, grsize=(20, 24, 22), cooinit=(6300.0, 30.0, 10.0))
+    vel = smooth_field(grd.grsize, seed, lo=5.0, hi=8.0, base=6.5, amp=0.4)
+    src = np.array([[6310.3, 31.13, 11.07], [6322.9, 33.41, 12.93]])
+    th, ph = np.meshgrid(np.linspace(30.4, 34.2, 4), np.linspace(10.4, 13.8, 3), indexing="ij")
+    rec = np.column_stack([np.full(th.size, 6336.0), th.ravel(), ph.ravel()])
+    return grd, vel, src, [rec, rec.copy()]
 
 
 def sphere2d(seed=3):
