@@ -255,6 +255,13 @@ def main() -> int:
                                "sample": f"{ns} of the 16 sources of the same workload, heap FMM with refinement "
                                          f"(C restatement of the reference), OpenMP by source chunk, {tcpu:.2f} s"}
         out["parity_fp64_picks_max_rel_vs_oracle"] = rel
+        # the mixed form's error against the same oracle picks (north-star receiver tolerance 1e-5)
+        picks_mx = model.traveltime(src[:ns], recs[:ns], extraparams=ExtraParams(precision="mixed"))
+        out["mixed_precision"]["picks_max_rel_vs_oracle"] = max(
+            float(np.max(np.abs(a - b) / np.abs(b))) for a, b in zip(picks_mx, picks_cpu))
+        picks_fast = model.traveltime(src[:ns], recs[:ns], extraparams=ExtraParams(fast=True))
+        out["fp64_fast"]["picks_max_rel_vs_oracle"] = max(
+            float(np.max(np.abs(a - b) / np.abs(b))) for a, b in zip(picks_fast, picks_cpu))
         # the same restatement on one core, two sources (the single-thread rate of the reference path)
         t0 = time.perf_counter()
         O.traveltime(vel, grd, src[:2], recs[:2], extraparams=ExtraParams(), nthreads=1)
