@@ -97,7 +97,10 @@ int eik_traveltime(eik_ctx* ctx, eik_model* model, int64_t nsrc, const double* c
                    double* ttpicks, double* ttgrid);
 
 /* eikgradient (src/EikSolv/eikmaingrad.jl:31-164).  gradvel (N doubles) and gradsrc (nsrc*ndim,
- * packed rows) may be NULL when not requested by whichgrad; misf = 0.5*sum((t-tobs)^2/std^2). */
+ * packed rows) may be NULL when not requested by whichgrad; misf = 0.5*sum((t-tobs)^2/std^2).
+ * Spherical grids: EIK_GRADVEL only (EIK_EINVAL otherwise).  Grid3DSphere, where the reference
+ * raises (createsparsederivativematrices!, eikmainfwd.jl:885-887), differentiates this library's
+ * own 3D spherical forward. */
 int eik_gradient(eik_ctx* ctx, eik_model* model, int64_t nsrc, const double* coordsrc,
                  const int64_t* rec_off, const double* coordrec, const double* pickobs,
                  const double* stdobs, int32_t whichgrad, const eik_params* params,
