@@ -78,6 +78,10 @@ __device__ __forceinline__ bool tiny_change(T v0, const Key<T>& k0, T v1, const 
 // keys conservatively, so rounding must never move them up).
 __device__ __forceinline__ float key_down(double k) { return __double2float_rd(k); }
 __device__ __forceinline__ float key_down(float k) { return k; }
+// ... and a node's own key as a float no smaller than it: a trigger c <= key_up(K) includes every
+// c <= K (at most a spurious re-evaluation, which leaves a node whose inputs did not change as it is)
+__device__ __forceinline__ float key_up(double k) { return __double2float_ru(k); }
+__device__ __forceinline__ float key_up(float k) { return k; }
 
 // Compact LDS image of a tile: the NODES core nodes (x fastest), then for each face k = 2*axis + l
 // (l = 0: + side, l = 1: - side; the neighbour-slot numbering of eik_stencil.h) its two halo
@@ -313,14 +317,14 @@ int it = 0;
             const int cb = it & 1, pb = cb ^ 1;
             // 1. collect the triggers of the previous sweep; a node within the band is updated
             if (act && it > 0) {
-                const T kx = s_k[n];
+                const float kx = key_up(s_k[n]);
                 float tr = ptr;
 #pragma unroll
                 for (int k = 0; k < 2 * ND; k++) {
 #pragma unroll
                     for (int d = 1; d <= 2; d++) {
                         const float c = s_chg[pb][L::core_nbr(n, k, d)];
-                        tr = ((T)c <= kx) ? fminf(tr, c) : tr;
+                        tr = (c <= kx && c < tr) ? c : tr;
                     }
                 }
                 ptr = tr;
@@ -334,13 +338,14 @@ int it = 0;
             if (need) {
                 unsigned f1 = 0u, f2 = 0u, d1 = 0u, d2 = 0u, dcap = 0u;
                 T v1[2 * ND], v2[2 * ND];
+                // Slots outside the grid are staged as +inf with depth 0, and every use of a
+                // neighbour below is masked by ok1 / ok2, so the reads need no bounds select.
 #pragma unroll
                 for (int k = 0; k < 2 * ND; k++) {
-                    const bool in1 = (ok1 >> k) & 1u, in2 = (ok2 >> k) & 1u;
                     const int o1 = L::nbr(n, k, 1), o2 = L::nbr(n, k, 2);
-                    const T a = in1 ? s_t[o1] : tinf<T>();
-                    const T b = in2 ? s_t[o2] : tinf<T>();
-                    const int da = in1 ? (int)s_d[o1] : 0, db = in2 ? (int)s_d[o2] : 0;
+                    const T a = s_t[o1];
+                    const T b = s_t[o2];
+                    const int da = (int)s_d[o1], db = (int)s_d[o2];
                     f1 |= (unsigned)is_fixed(a) << k;
                     f2 |= (unsigned)is_fixed(b) << k;
                     d1 |= (unsigned)(da != 0) << k;
