@@ -104,6 +104,46 @@ def _traffic_from_profile():
         return json.load(f).get("hbm_bytes_per_launch")
 
 
+def config_leg(name: str, steps: int) -> dict:
+    """Forward (and gradient) timings of one of the other workloads, FP64 and mixed, through the
+    public API: c1 = the README's 2D 300x220 case (4 sources), c3 = 2D 600x400 with 256 sources
+    (:gradvel), c4 = 3D spherical 120x150x150 with 32 sources (:gradvel, no refinement)."""
+    import eikonalsolvers as eik
+    from eikonalsolvers.grids import ExtraParams
+    from eikonalsolvers import synthetic as syn
+
+    def timed(f):
+        f()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            f()
+        return (time.perf_counter() - t0) / steps
+
+    if name == "c1":
+        grd, vel, src, recs = syn.c1_2d_forward()
+        grad, extra = False, {}
+    elif name == "c3":
+        grd, vel, src, recs = syn.c3_2d_batched()
+        grad, extra = True, {}
+    else:
+        grd, vel, src, recs = syn.c4_3d_sphere()
+        grad, extra = True, {"refinearoundsrc": False}
+    m = eik.EikModel(vel, grd)
+    out = {"config": name, "grid": list(grd.grsize), "nsrc": len(src), "steps": steps}
+    picks = m.traveltime(src, recs, extraparams=ExtraParams(**extra))
+    obs = [p + 0.01 * np.cos(np.arange(len(p))) for p in picks]
+    std = [np.full(len(p), 0.01) for p in picks]
+    for prec in ("fp64", "mixed"):
+        ep = ExtraParams(precision=prec, **extra)
+        dt = timed(lambda: m.traveltime(src, recs, extraparams=ep))
+        out[prec] = {"forward_ms": round(1e3 * dt, 2), "source_solves_per_s": round(len(src) / dt, 1)}
+        if grad:
+            dg = timed(lambda: m.gradient(src, recs, obs, std, "gradvel", extraparams=ep))
+            out[prec].update({"gradvel_ms": round(1e3 * dg, 2), "evals_per_s": round(len(src) / dg, 1)})
+    m.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -118,6 +158,8 @@ def main() -> int:
     ap.add_argument("--serving-slots", type=int, default=4)
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain device copy of 512 MiB (calibrates HBM byte counters)")
+    ap.add_argument("--config", choices=["c1", "c3", "c4"], default=None,
+                    help="print the timings of another workload instead of the benchmark line")
     args = ap.parse_args()
     if args.copy_probe:
         import torch
@@ -132,6 +174,9 @@ def main() -> int:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it "
                                    "(sources would shard by chunk, see DESIGN.md)", "n_gpus": args.gpus}))
         return 2
+    if args.config:
+        print(json.dumps(config_leg(args.config, max(1, args.steps))), flush=True)
+        return 0
 
     import eikonalsolvers as eik
     from eikonalsolvers.grids import ExtraParams
