@@ -172,7 +172,7 @@ def main() -> int:
         return 0
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it "
-                                   "(sources would shard by chunk, see DESIGN.md)", "n_gpus": args.gpus}))
+                                   "(the sources of a call would shard across GPUs by chunk)", "n_gpus": args.gpus}))
         return 2
     if args.config:
         print(json.dumps(config_leg(args.config, max(1, args.steps))), flush=True)

@@ -17,7 +17,7 @@ def smooth_field(shape, seed, passes=8, lo=1.5, hi=6.0, base=3.0, amp=1.0):
 
 
 def c1_model():
-    """The README/BASELINE 2D forward case: 300x220, h=0.5, v = 2.5 + 0.034*j, 4 sources."""
+    """The reference README's 2D forward case: 300x220, h=0.5, v = 2.5 + 0.034*j, 4 sources."""
     grd = Grid2DCart(hgrid=0.5, cooinit=(0.0, 0.0), grsize=(300, 220))
     nx, ny = grd.grsize
     vel = np.zeros((nx, ny))
