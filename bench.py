@@ -171,7 +171,7 @@ def main() -> int:
         print(json.dumps({"copy_probe_bytes_read": x.numel(), "copy_probe_bytes_written": y.numel(), "launches": 5}))
         return 0
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it "
+        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part of it "
                                    "(the sources of a call would shard across GPUs by chunk)", "n_gpus": args.gpus}))
         return 2
     if args.config:

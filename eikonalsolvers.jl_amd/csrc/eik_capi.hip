@@ -393,7 +393,7 @@ int eik_gradient(eik_ctx* ctx, eik_model* model, int64_t nsrc, const double* coo
     const eik_params* params = &pe;
     const int kind = model->g.kind;
     // Grid3DSphere: the reference raises here (createsparsederivativematrices!, eikmainfwd.jl:885-887);
-    // this build differentiates its own 3D spherical forward (r sin(theta) dphi metric), :gradvel only
+    // this library differentiates its own 3D spherical forward (r sin(theta) dphi metric), :gradvel only
     if ((kind == EIK_GRID2D_SPHERE || kind == EIK_GRID3D_SPHERE) && whichgrad != EIK_GRADVEL)
         return ctx->fail(EIK_EINVAL, "gradsrcloc is not available on spherical grids");
     if (S.nrec() > 0 && (!pickobs || !stdobs)) return ctx->fail(EIK_EINVAL, "null observations");

@@ -96,7 +96,7 @@ __host__ __device__ __forceinline__ int nn_coarse(int i0, int df) {
 
 __host__ __device__ __forceinline__ double deg2rad(double x) { return x * (M_PI / 180.0); }
 
-// distance2points (src/EikSolv/eikutils.jl:47-73); 3D sphere: chord length (this build's semantics)
+// distance2points (src/EikSolv/eikutils.jl:47-73); 3D sphere: chord length (this library's semantics)
 __host__ __device__ __forceinline__ double distance2points(int kind, int nd, const double* a, const double* b) {
 #pragma clang fp contract(off)
     if (kind == EIK_GRID2D_SPHERE) {

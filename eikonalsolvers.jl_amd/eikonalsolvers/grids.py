@@ -156,7 +156,7 @@ class Grid2DSphere(AbstractGridEik):
 class Grid3DSphere(AbstractGridEik):
     """3D spherical grid in (r, theta[deg], phi[deg]) (eikstructs.jl:182-219).
 
-    The reference constructs this type but its solvers do not run on it; this build gives it its
+    The reference constructs this type but its solvers do not run on it; this package gives it its
     own semantics: the phi spacing of the stencil is r*sin(theta)*dphi and distances are chords.
     """
 

@@ -68,7 +68,7 @@ static inline double deg2rad(double x) { return x * (M_PI / 180.0); }
 static inline int is_sph(const og_grid* g) { return g->kind == OG_SPH2D || g->kind == OG_SPH3D; }
 
 /* per-node spacing of the stencil (eikforward2D.jl:115-119, eikforward3D.jl:168-172).
- * OG_SPH3D uses this build's own metric r*sin(theta)*dphi (the reference's 3D sphere is
+ * OG_SPH3D uses the engine's own metric r*sin(theta)*dphi (the reference's 3D sphere is
  * non-functional). */
 static inline void node_spacing(const og_grid* g, const int64_t* ijk, double* dh) {
     if (!is_sph(g)) {
@@ -450,7 +450,7 @@ static int findenclosingbox(const og_grid* g, const double* src, int64_t corn[8]
     return nc;
 }
 
-/* distance2points (eikutils.jl:47-73); OG_SPH3D: this build's great-circle chord metric */
+/* distance2points (eikutils.jl:47-73); OG_SPH3D: the engine's chord metric */
 static double distance2points(const og_grid* g, const double* a, const double* b) {
     if (g->kind == OG_SPH2D) {
         double r1 = a[0], t1 = a[1], r2 = b[0], t2 = b[1];

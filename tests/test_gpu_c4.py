@@ -3,7 +3,7 @@ dtheta = dphi = 0.1 deg, outer radius 6371 km, 32 sources at 10-150 km depth, 20
 receivers each, FP64 and mixed precision.
 
 Grid3DSphere does not run in the reference (its fine grid and derivative matrices raise), so
-this build gives it its own forward semantics (r sin(theta) dphi metric, chord distances,
+the engine gives it its own forward semantics (r sin(theta) dphi metric, chord distances,
 no refinement) and the oracle restates those: parity unpinned against the reference, pinned
 against the oracle here.  Refinement and :gradsrcloc raise; :gradvel differentiates this
 forward and is checked against the oracle (below), whose adjoint finite differences pin.
@@ -75,7 +75,7 @@ def test_c4_refinement_and_gradsrcloc_raise(eik, c4):
 
 
 # ---------------------------------------------------------------- :gradvel on Grid3DSphere
-# The reference raises here (createsparsederivativematrices!); this build differentiates its own
+# The reference raises here (createsparsederivativematrices!); the engine differentiates its own
 # 3D spherical forward.  The oracle's adjoint for it is pinned by finite differences
 # (tests/test_oracle_sphere3d_gradient.py); the GPU is checked against the oracle as the Cartesian
 # gradients are (FP64 1e-7, mixed 1e-3 relative L2), and at the C4 size mixed against FP64.

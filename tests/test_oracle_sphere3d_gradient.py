@@ -1,7 +1,7 @@
 """The oracle's :gradvel on Grid3DSphere, pinned by central finite differences of its own misfit.
 
 The reference raises for Grid3DSphere (createsparsederivativematrices!, eikmainfwd.jl:885-887); the
-oracle and the GPU differentiate this build's own 3D spherical forward (r sin(theta) dphi metric,
+oracle and the GPU differentiate the engine's own 3D spherical forward (r sin(theta) dphi metric,
 chord distances), with the reference's discrete adjoint otherwise unchanged (eikmaingrad.jl).
 That adjoint is not the exact derivative of the discrete misfit (the heap's acceptance order
 and the derivative codes are held fixed), so finite differences pin it only to ~1e-3: on a
