@@ -897,15 +897,19 @@ int fim_run(eik_ctx* ctx, int nd, int prec, int kind, const InstDev* d_inst, int
 // Band width of the tile scheduler, in node crossing times (spacing x mean slowness).  A round
 // runs only the active tiles whose pending traveltime lies within the band of the lowest pending
 // one, so tiles far ahead of the front are not relaxed against values that will still change.
-// Measured optimum: about one tile edge in 3D (4; 3 and 6 are slower on 201x201x101), 10-16 in 2D
-// (600x400 x 256 sources: 12 is 17 % faster than 4 in FP64, 20 % in FP32; 8x8 tiles are twice as
-// wide).  EIK_BAND overrides the multiplier (0 or less: no band, every active tile runs).
-static double band_mult(int nd) {
+// Measured optima: FP64 about one tile edge in 3D (4: 3 is slower on 201x201x101, 6-8 slower on the
+// 120x150x150 spherical grid), 10-16 in 2D (600x400 x 256 sources: 12 is 17 % faster than 4; 8x8
+// tiles are twice as wide).  FP32 has fewer rounding-level ripples and prefers a wider band: 6 in
+// 3D (6-10 flat on 201x201x101, 6 % faster than 4; 3 % on the spherical grid), 16 in 2D.
+// EIK_BAND overrides the multiplier (0 or less: no band, every active tile runs).
+static double band_mult(int nd, int prec) {
     static const double env = [] {
         const char* e = getenv("EIK_BAND");
         return e ? atof(e) : NAN;
     }();
-    return env == env ? env : (nd == 3 ? 4.0 : 12.0);
+    if (env == env) return env;
+    const bool f32 = prec != EIK_PREC_FP64;
+    return nd == 3 ? (f32 ? 6.0 : 4.0) : (f32 ? 16.0 : 12.0);
 }
 
 // EIK_NODEBAND: the same for the nodes inside a tile (default 0: a listed tile relaxes all of
@@ -944,8 +948,8 @@ static double node_spacing_mean(const InstDev& I, int nd) {
     return sum / nd;
 }
 
-static void set_band(InstDev& I, int nd, double smean) {
-    const double m = band_mult(nd), mn = nodeband_mult(), ma = adjband_mult();
+static void set_band(InstDev& I, int nd, int prec, double smean) {
+    const double m = band_mult(nd, prec), mn = nodeband_mult(), ma = adjband_mult();
     const double cross = node_spacing_mean(I, nd) * smean;
     I.band = m > 0.0 ? m * cross : INFINITY;
     I.nband = mn > 0.0 ? mn * cross : INFINITY;
@@ -1015,7 +1019,7 @@ int forward_solve(eik_ctx* ctx, const std::vector<FwdGroup>& G, int prec, const 
                 I.s_off = soff[k];
                 I.tile_base = ntiles_c;
                 I.atile_base = antiles_c;
-                set_band(I, nd, M->smean);
+                set_band(I, nd, prec, M->smean);
                 totN += M->N;
                 ntiles_c += (int64_t)I.t[0] * I.t[1] * I.t[2];
                 antiles_c += (int64_t)I.at[0] * I.at[1] * I.at[2];
@@ -1049,7 +1053,7 @@ int forward_solve(eik_ctx* ctx, const std::vector<FwdGroup>& G, int prec, const 
             finst[s].s_off = nfine_pts;
             finst[s].tile_base = ntiles_f;
             finst[s].atile_base = antiles_f;
-            set_band(finst[s], nd, smod[s]->smean);
+            set_band(finst[s], nd, EIK_PREC_FP64, smod[s]->smean);  // refined boxes solve in FP64
             nfine_pts += (int64_t)finst[s].n[0] * finst[s].n[1] * finst[s].n[2];
             ntiles_f += (int64_t)finst[s].t[0] * finst[s].t[1] * finst[s].t[2];
             antiles_f += (int64_t)finst[s].at[0] * finst[s].at[1] * finst[s].at[2];
