@@ -240,6 +240,10 @@ void eik_model_destroy(eik_model* M) {
     delete M;
 }
 
+// Receivers per call beyond which rec_off is taken as corrupt (2^32 receivers would need 32 GiB of
+// coordinates alone in 3D).
+static const int64_t kMaxReceivers = (int64_t)1 << 32;
+
 // checksrcrecposition (src/EikSolv/eikchecks.jl:4-50) and ExtraParams checks (eikstructs.jl:270-277)
 static int load_sources(eik_ctx* ctx, eik_model* M, int64_t nsrc, const double* coordsrc, const int64_t* rec_off,
                         const double* coordrec, const eik_params* P, SourceSet& S) {
@@ -258,14 +262,27 @@ static int load_sources(eik_ctx* ctx, eik_model* M, int64_t nsrc, const double* 
         return ctx->fail(EIK_EINVAL, "createfinegrid(): spherical coordinates in 3D still work in progress...");
     S.nsrc = nsrc;
     S.nd = nd;
-    S.src.assign(coordsrc, coordsrc + nsrc * nd);
-    S.rec_off.assign(rec_off, rec_off + nsrc + 1);
-    if (nsrc > 0 && S.rec_off[0] != 0) return ctx->fail(EIK_EINVAL, "rec_off[0] must be 0");
-    for (int64_t s = 0; s < nsrc; s++)
-        if (S.rec_off[s + 1] < S.rec_off[s]) return ctx->fail(EIK_EINVAL, "rec_off must be non-decreasing");
+    // rec_off is read only when there are sources (it may be NULL otherwise); its prefix sums are
+    // checked before anything is sized from them, and no allocation failure crosses the ABI
+    if (nsrc > 0) {
+        if (rec_off[0] != 0) return ctx->fail(EIK_EINVAL, "rec_off[0] must be 0");
+        for (int64_t s = 0; s < nsrc; s++)
+            if (rec_off[s + 1] < rec_off[s]) return ctx->fail(EIK_EINVAL, "rec_off must be non-decreasing");
+        if (rec_off[nsrc] > kMaxReceivers) return ctx->fail(EIK_EINVAL, "too many receivers (%lld)", (long long)rec_off[nsrc]);
+    }
+    try {
+        S.src.assign(coordsrc, coordsrc + nsrc * nd);
+        if (nsrc > 0)
+            S.rec_off.assign(rec_off, rec_off + nsrc + 1);
+        else
+            S.rec_off.assign(1, 0);
+        const int64_t nrec = S.nrec();
+        if (nrec > 0 && !coordrec) return ctx->fail(EIK_EINVAL, "null receivers");
+        S.rec.assign(coordrec, coordrec + nrec * nd);
+    } catch (const std::bad_alloc&) {
+        return ctx->fail(EIK_ENOMEM, "out of host memory (sources and receivers)");
+    }
     const int64_t nrec = S.nrec();
-    if (nrec > 0 && !coordrec) return ctx->fail(EIK_EINVAL, "null receivers");
-    S.rec.assign(coordrec, coordrec + nrec * nd);
     for (int d = 0; d < nd; d++) {
         const double lo = gcoord(M->g, d, 0), hi = gcoord(M->g, d, M->g.n[d] - 1);
         for (int64_t s = 0; s < nsrc; s++) {
@@ -361,7 +378,16 @@ int eik_misfit(eik_ctx* ctx, eik_model* model, int64_t nsrc, const double* coord
                const double* coordrec, const double* pickobs, const double* stdobs, const eik_params* params,
                double* misf) {
     if (!ctx || !model || !misf) return EIK_EINVAL;
-    const int64_t nrec = nsrc > 0 && rec_off ? rec_off[nsrc] : 0;
+    int64_t nrec = 0;
+    {
+        // validate everything (sources, rec_off, observations) before reading any of it
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        SourceSet S;
+        const int rc = load_sources(ctx, model, nsrc, coordsrc, rec_off, coordrec, params, S);
+        if (rc) return rc;
+        nrec = S.nrec();
+        if (nrec > 0 && (!pickobs || !stdobs)) return ctx->fail(EIK_EINVAL, "null observations");
+    }
     std::vector<double> picks((size_t)std::max<int64_t>(nrec, 1));
     int rc = eik_traveltime(ctx, model, nsrc, coordsrc, rec_off, coordrec, params, picks.data(), nullptr);
     if (rc) return rc;

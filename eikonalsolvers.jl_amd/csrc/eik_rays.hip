@@ -219,9 +219,32 @@ __global__ void __launch_bounds__(kRayThreads) k_trace(const double* __restrict_
 
 }  // namespace
 
+// Step cap of one ray: a descending ray is never much longer than the domain, so 20 domain
+// lengths of steps (plus 1000) are allowed.  The domain length is the diagonal of the Cartesian
+// box (`diag`), or on a Grid2DSphere the outer circle's diameter 2 r_max, which bounds every
+// chord inside the annulus sector.  Both are lengths; rstep is the step length in the same unit.
+static int64_t ray_max_steps(int kind, double diag, double rmax, double rstep) {
+    const double len = kind == EIK_GRID2D_SPHERE ? 2.0 * rmax : diag;
+    return (int64_t)std::min(4.0e9, 20.0 * len / rstep + 1000.0);
+}
+
 }  // namespace eik
 
 using namespace eik;
+
+extern "C" int eik_tracerays_max_steps(const eik_grid* grid, double steplen, int64_t* out) {
+    if (!grid || !out || grid->ndim < 2 || grid->ndim > 3) return EIK_EINVAL;
+    if (!(steplen > 0.0)) return EIK_EASSERT;
+    double diag = 0.0;
+    for (int d = 0; d < grid->ndim; d++) {
+        const double ext = (double)(grid->n[d] - 1) * grid->d[d];
+        diag += ext * ext;
+    }
+    const bool sph = grid->kind == EIK_GRID2D_SPHERE;
+    const double rmax = grid->o[0] + (double)(grid->n[0] - 1) * grid->d[0];
+    *out = ray_max_steps(grid->kind, sqrt(diag), rmax, sph ? steplen : steplen * grid->d[0]);
+    return EIK_OK;
+}
 
 extern "C" int eik_tracerays(eik_ctx* ctx, const eik_grid* grid, int64_t nsrc, const double* ttime,
                              const double* coordsrc, const int64_t* rec_off, const double* coordrec, double steplen,
@@ -260,9 +283,7 @@ extern "C" int eik_tracerays(eik_ctx* ctx, const eik_grid* grid, int64_t nsrc, c
     const bool sph = grid->kind == EIK_GRID2D_SPHERE;
     G.rstep = sph ? steplen : steplen * grid->d[0];
     G.thr = (sph ? grid->d[0] : grid->d[0]) * sqrt(2.0);
-    if (sph) diag = 2.0 * G.end[0];  // a chord of the outer circle bounds every distance
-    // a descending ray is never much longer than the domain: allow 20 diagonals
-    G.max_steps = (int64_t)std::min(4.0e9, 20.0 * sqrt(diag) / G.rstep + 1000.0);
+    G.max_steps = ray_max_steps(grid->kind, sqrt(diag), G.end[0], G.rstep);
     const int64_t nrec = nsrc > 0 ? rec_off[nsrc] : 0;
     ray_off[0] = 0;
     if (nsrc == 0 || nrec == 0) {

@@ -28,7 +28,7 @@ EXPORTS = (
     "eik_model_create", "eik_model_update_vel", "eik_model_destroy",
     "eik_traveltime", "eik_gradient", "eik_misfit", "eik_last_stats",
     "eik_server_create", "eik_server_destroy", "eik_submit", "eik_wait", "eik_server_stats",
-    "eik_tracerays", "eik_traveltime_batch", "eik_poll",
+    "eik_tracerays", "eik_traveltime_batch", "eik_poll", "eik_tracerays_max_steps",
 )
 
 
@@ -115,6 +115,8 @@ def load():
         L.eik_traveltime_batch.restype = ctypes.c_int
         L.eik_poll.argtypes = [vp, ctypes.c_int64, P(ctypes.c_int32)]
         L.eik_poll.restype = ctypes.c_int
+        L.eik_tracerays_max_steps.argtypes = [P(EikGrid), ctypes.c_double, ip]
+        L.eik_tracerays_max_steps.restype = ctypes.c_int
         _lib = L
         return L
 

@@ -200,8 +200,12 @@ class EikModel:
     def misfit(self, coordsrc, coordrec, pickobs, stdobs, *, extraparams=None):
         grd = self.grd
         cs, off, packed = _pack(coordsrc, coordrec, grd.ndim)
-        po = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.float64).ravel() for x in pickobs]))
-        so = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.float64).ravel() for x in stdobs]))
+        assert len(coordrec) == len(pickobs) == len(stdobs), "length(coordrec)==length(pickobs)"
+        po = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.float64).ravel() for x in pickobs]
+                                                 or [np.zeros(0)]))
+        so = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.float64).ravel() for x in stdobs]
+                                                 or [np.zeros(0)]))
+        assert po.size == so.size == int(off[-1]), "size(pickobs)==size(stdobs)==number of receivers"
         p = _params_struct(extraparams)
         m = np.zeros(1)
         rc = _lib.load().eik_misfit(self.ctx.handle, self.handle, cs.shape[0], _lib.ptr(cs),

@@ -136,6 +136,12 @@ int eik_tracerays(eik_ctx* ctx, const eik_grid* grid, int64_t nsrc, const double
                   const int64_t* rec_off, const double* coordrec, double steplen, int64_t* ray_off, double* rays,
                   int64_t cap_points);
 
+/* The step cap eik_tracerays applies to one ray on this grid (host-only query, no device work; no
+ * reference counterpart, the reference's loop has no cap): 20 domain lengths of steps plus 1000,
+ * where the domain length is the box diagonal (Cartesian) or the outer diameter 2 r_max
+ * (Grid2DSphere) and the step is steplen*hgrid (Cartesian) or steplen (Grid2DSphere). */
+int eik_tracerays_max_steps(const eik_grid* grid, double steplen, int64_t* out);
+
 /* ---------------------------------------------------------------- request loop (serving)
  * No reference counterpart: the reference evaluates one call at a time.  A server runs
  * traveltime and gradient requests with continuous batching: nslots slots (each a context with

@@ -67,6 +67,15 @@ static inline double coord(const og_grid* g, int d, int64_t i0) { return fma((do
 static inline double deg2rad(double x) { return x * (M_PI / 180.0); }
 static inline int is_sph(const og_grid* g) { return g->kind == OG_SPH2D || g->kind == OG_SPH3D; }
 
+/* Test hook (og_plant_mistake): a known mistake planted in the spherical metric so that the
+ * tests can show that their pins detect it.  0 = none (the only value any real use sees). */
+static int planted_mistake = 0;
+int og_plant_mistake(int which) {
+    const int old = planted_mistake;
+    planted_mistake = which;
+    return old;
+}
+
 /* per-node spacing of the stencil (eikforward2D.jl:115-119, eikforward3D.jl:168-172).
  * OG_SPH3D uses the engine's own metric r*sin(theta)*dphi (the reference's 3D sphere is
  * non-functional). */
@@ -78,9 +87,14 @@ static inline void node_spacing(const og_grid* g, const int64_t* ijk, double* dh
     double r = coord(g, 0, ijk[0]);
     dh[0] = g->d[0];
     dh[1] = r * deg2rad(g->d[1]);
+    if (planted_mistake == OG_MISTAKE_DEGREES) dh[1] = r * g->d[1];
+    if (planted_mistake == OG_MISTAKE_NO_RADIUS) dh[1] = deg2rad(g->d[1]);
     if (g->ndim == 3) {
         double th = coord(g, 1, ijk[1]);
         dh[2] = r * sin(deg2rad(th)) * deg2rad(g->d[2]);
+        if (planted_mistake == OG_MISTAKE_DEGREES) dh[2] = r * sin(deg2rad(th)) * g->d[2];
+        if (planted_mistake == OG_MISTAKE_NO_RADIUS) dh[2] = sin(deg2rad(th)) * deg2rad(g->d[2]);
+        if (planted_mistake == OG_MISTAKE_NO_SINTHETA) dh[2] = r * deg2rad(g->d[2]);
     }
 }
 
@@ -454,6 +468,7 @@ static int findenclosingbox(const og_grid* g, const double* src, int64_t corn[8]
 static double distance2points(const og_grid* g, const double* a, const double* b) {
     if (g->kind == OG_SPH2D) {
         double r1 = a[0], t1 = a[1], r2 = b[0], t2 = b[1];
+        if (planted_mistake == OG_MISTAKE_INIT_DEGREES) return sqrt(r1 * r1 + r2 * r2 - 2.0 * r1 * r2 * cos(t1 - t2));
         return sqrt(r1 * r1 + r2 * r2 - 2.0 * r1 * r2 * cos(deg2rad(t1 - t2)));
     }
     if (g->kind == OG_SPH3D) {

@@ -85,6 +85,17 @@ double og_stencil(const og_grid* g, const double* vel, const double* tt, const u
 /* Gaussian smoothing of a gradient (utils.jl:22-96), out-of-place. */
 int og_smoothgradient(const og_grid* g, int32_t l, const double* in, double* out);
 
+/* TEST HOOK: plant a known mistake in the spherical forward so that a test can show that its pins
+ * detect it (returns the previous setting).  Never set outside tests; 0 = the oracle as written. */
+enum {
+    OG_MISTAKE_NONE = 0,
+    OG_MISTAKE_DEGREES = 1,      /* angular spacings taken in degrees, not radians */
+    OG_MISTAKE_NO_RADIUS = 2,    /* arc length without the node's radius r_i */
+    OG_MISTAKE_NO_SINTHETA = 3,  /* 3D: phi arc without sin(theta) */
+    OG_MISTAKE_INIT_DEGREES = 4  /* 2D: source-box distance (law of cosines) in degrees */
+};
+int og_plant_mistake(int which);
+
 #ifdef __cplusplus
 }
 #endif

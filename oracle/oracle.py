@@ -58,6 +58,8 @@ def lib():
         L.og_stencil.argtypes = [P(OgGrid), dp, dp, P(ctypes.c_uint8), ip, P(ctypes.c_int32)]
         L.og_stencil.restype = ctypes.c_double
         L.og_smoothgradient.argtypes = [P(OgGrid), ctypes.c_int32, dp, dp]
+        L.og_plant_mistake.argtypes = [ctypes.c_int]
+        L.og_plant_mistake.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -244,3 +246,26 @@ def warning_count() -> int:
 
 def reset_warnings():
     lib().og_reset_warnings()
+
+
+# Test hook: known mistakes that can be planted in the oracle's spherical forward, so that a test
+# can show that its pins detect each of them (eik_oracle.h, og_plant_mistake).
+MISTAKE_DEGREES = 1
+MISTAKE_NO_RADIUS = 2
+MISTAKE_NO_SINTHETA = 3
+MISTAKE_INIT_DEGREES = 4
+
+
+class planted_mistake:
+    """Context manager: run the oracle with one planted mistake, then restore it."""
+
+    def __init__(self, which: int):
+        self.which = which
+
+    def __enter__(self):
+        self.prev = lib().og_plant_mistake(self.which)
+        return self
+
+    def __exit__(self, *exc):
+        lib().og_plant_mistake(self.prev)
+        return False

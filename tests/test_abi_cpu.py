@@ -53,3 +53,28 @@ def test_grid_validation_matches_reference():
     g = Grid2DCart(hgrid=0.5, cooinit=(1.0, 2.0), grsize=(4, 3))
     np.testing.assert_array_equal(g.x, [1.0, 1.5, 2.0, 2.5])
     assert g.cooend == (2.5, 3.0)
+
+
+def test_ray_step_cap_is_a_length_on_wide_spheres():
+    """The ray step cap counts 20 domain lengths of steps on every grid kind: on a Grid2DSphere the
+    length is the outer diameter (a length in km, not its square root).  A 40 degree sector at
+    r = 6371 km with 0.5 km steps must allow the ~4500 km chord between its ends many times over."""
+    import ctypes
+    from eikonalsolvers import _lib
+    from eikonalsolvers.api import _grid_struct
+    from eikonalsolvers.grids import Grid2DCart, Grid2DSphere
+    if not os.path.exists(_lib.LIBPATH):
+        pytest.skip("libeik.so not built (run __graft_entry__.build())")
+    L = _lib.load()
+    out = ctypes.c_int64(0)
+    grd = Grid2DSphere(Δr=2.0, Δθ=0.1, grsize=(120, 401), cooinit=(6133.0, 40.0))
+    assert L.eik_tracerays_max_steps(ctypes.byref(_grid_struct(grd)), 0.5, ctypes.byref(out)) == 0
+    rmax = 6133.0 + 119 * 2.0
+    chord = 2 * rmax * np.sin(np.deg2rad(40.0) / 2)  # the longest straight path in the sector
+    assert out.value == int(20.0 * 2 * rmax / 0.5 + 1000.0)
+    assert out.value * 0.5 > 20 * chord
+    # Cartesian: 20 box diagonals of steplen*hgrid
+    g2 = Grid2DCart(hgrid=0.25, cooinit=(0.0, 0.0), grsize=(600, 400))
+    assert L.eik_tracerays_max_steps(ctypes.byref(_grid_struct(g2)), 0.1, ctypes.byref(out)) == 0
+    diag = np.hypot(599 * 0.25, 399 * 0.25)
+    assert out.value == int(20.0 * diag / (0.1 * 0.25) + 1000.0)
