@@ -8,7 +8,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "eikonalsolvers", "libeik.so")
-SOURCES = ["eik_capi.hip", "eik_fim.hip", "eik_adj.hip", "eik_serve.hip", "eik_rays.hip"]
+SOURCES = ["eik_capi.hip", "eik_fim.hip", "eik_adj.hip", "eik_serve.hip", "eik_rays.hip", "eik_noalloc.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

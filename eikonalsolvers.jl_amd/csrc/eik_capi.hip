@@ -103,6 +103,79 @@ void build_fine(const eik_grid& g, const double* src, int df, int noderadius, Fi
     F.ncorn = find_enclosing_box(gf, src, F.corn);
 }
 
+// Receivers per call beyond which rec_off is taken as corrupt (2^32 receivers would need 32 GiB of
+// coordinates alone in 3D).
+static const int64_t kMaxReceivers = (int64_t)1 << 32;
+
+// checksrcrecposition (src/EikSolv/eikchecks.jl:4-50) and ExtraParams checks (eikstructs.jl:270-277)
+int load_sources(eik_ctx* ctx, eik_model* M, int64_t nsrc, const double* coordsrc, const int64_t* rec_off,
+                 const double* coordrec, const eik_params* P, SourceSet& S) {
+    const int nd = M->g.ndim;
+    if (nsrc < 0 || (nsrc > 0 && (!coordsrc || !rec_off))) return ctx->fail(EIK_EINVAL, "bad source arguments");
+    if (!P) return ctx->fail(EIK_EINVAL, "null params");
+    if (P->radiussmoothgradsrc < 0) return ctx->fail(EIK_EASSERT, "radiussmoothgradsrc>= 0");
+    if (P->smoothgradkern < 0) return ctx->fail(EIK_EASSERT, "smoothgradkern>=0");
+    if (P->smoothgradkern > 0 && P->smoothgradkern % 2 == 0)
+        return ctx->fail(EIK_EINVAL, "ExtraParams: smoothgradkern must be an odd number");
+    if (P->precision != EIK_PREC_FP64 && P->precision != EIK_PREC_MIXED) return ctx->fail(EIK_EINVAL, "bad precision");
+    if (P->fast != 0 && P->fast != 1) return ctx->fail(EIK_EINVAL, "fast must be 0 or 1");
+    if (P->refinearoundsrc && (P->downscalefactor < 1 || P->noderadius < 0))
+        return ctx->fail(EIK_EINVAL, "bad grid refinement parameters");
+    if (P->refinearoundsrc && M->g.kind == EIK_GRID3D_SPHERE)
+        return ctx->fail(EIK_EINVAL, "createfinegrid(): spherical coordinates in 3D still work in progress...");
+    S.nsrc = nsrc;
+    S.nd = nd;
+    // rec_off is read only when there are sources (it may be NULL otherwise); its prefix sums are
+    // checked before anything is sized from them, and no allocation failure crosses the ABI
+    if (nsrc > 0) {
+        if (rec_off[0] != 0) return ctx->fail(EIK_EINVAL, "rec_off[0] must be 0");
+        for (int64_t s = 0; s < nsrc; s++)
+            if (rec_off[s + 1] < rec_off[s]) return ctx->fail(EIK_EINVAL, "rec_off must be non-decreasing");
+        if (rec_off[nsrc] > kMaxReceivers) return ctx->fail(EIK_EINVAL, "too many receivers (%lld)", (long long)rec_off[nsrc]);
+    }
+    try {
+        S.src.assign(coordsrc, coordsrc + nsrc * nd);
+        if (nsrc > 0)
+            S.rec_off.assign(rec_off, rec_off + nsrc + 1);
+        else
+            S.rec_off.assign(1, 0);
+        const int64_t nrec = S.nrec();
+        if (nrec > 0 && !coordrec) return ctx->fail(EIK_EINVAL, "null receivers");
+        S.rec.assign(coordrec, coordrec + nrec * nd);
+    } catch (const std::bad_alloc&) {
+        return ctx->fail(EIK_ENOMEM, "out of host memory (sources and receivers)");
+    }
+    const int64_t nrec = S.nrec();
+    for (int d = 0; d < nd; d++) {
+        const double lo = gcoord(M->g, d, 0), hi = gcoord(M->g, d, M->g.n[d] - 1);
+        for (int64_t s = 0; s < nsrc; s++) {
+            const double x = S.src[s * nd + d];
+            if (!(lo <= x && x <= hi)) return ctx->fail(EIK_EASSERT, "source %lld outside the grid", (long long)s);
+        }
+        for (int64_t r = 0; r < nrec; r++) {
+            const double x = S.rec[r * nd + d];
+            if (!(lo <= x && x <= hi)) return ctx->fail(EIK_EASSERT, "receiver %lld outside the grid", (long long)r);
+        }
+    }
+    return EIK_OK;
+}
+
+// Mixed precision keeps the solve near the source in FP64.  Without refinement that region is an
+// FP64 box of +-kNearSourceRadius nodes at the coarse spacing, solved with the refinement machinery
+// at downscalefactor 1: the nodes it hands to the FP32 solve are those the heap accepts before the
+// front reaches the box faces, whose values do not depend on the box, so the coarse problem is
+// the reference's unrefined one.
+static const int32_t kNearSourceRadius = 4;
+eik_params effective_params(const eik_params* p) {
+    eik_params e = *p;
+    if (e.precision == EIK_PREC_MIXED && !e.refinearoundsrc) {
+        e.refinearoundsrc = 1;
+        e.downscalefactor = 1;
+        e.noderadius = kNearSourceRadius;
+    }
+    return e;
+}
+
 }  // namespace eik
 
 using namespace eik;
@@ -133,6 +206,8 @@ int eik_ctx_create(int32_t device, eik_ctx** out) {
             eik_ctx_destroy(ctx);
             return EIK_EDEVICE;
         }
+    // the no-allocation entry point's scratch block: this device's address, resolved once
+    scratch_setup(device);
     *out = ctx;
     return EIK_OK;
 }
@@ -238,79 +313,6 @@ void eik_model_destroy(eik_model* M) {
     if (M->d_slow32) (void)hipFree(M->d_slow32);
     if (M->d_sinth) (void)hipFree(M->d_sinth);
     delete M;
-}
-
-// Receivers per call beyond which rec_off is taken as corrupt (2^32 receivers would need 32 GiB of
-// coordinates alone in 3D).
-static const int64_t kMaxReceivers = (int64_t)1 << 32;
-
-// checksrcrecposition (src/EikSolv/eikchecks.jl:4-50) and ExtraParams checks (eikstructs.jl:270-277)
-static int load_sources(eik_ctx* ctx, eik_model* M, int64_t nsrc, const double* coordsrc, const int64_t* rec_off,
-                        const double* coordrec, const eik_params* P, SourceSet& S) {
-    const int nd = M->g.ndim;
-    if (nsrc < 0 || (nsrc > 0 && (!coordsrc || !rec_off))) return ctx->fail(EIK_EINVAL, "bad source arguments");
-    if (!P) return ctx->fail(EIK_EINVAL, "null params");
-    if (P->radiussmoothgradsrc < 0) return ctx->fail(EIK_EASSERT, "radiussmoothgradsrc>= 0");
-    if (P->smoothgradkern < 0) return ctx->fail(EIK_EASSERT, "smoothgradkern>=0");
-    if (P->smoothgradkern > 0 && P->smoothgradkern % 2 == 0)
-        return ctx->fail(EIK_EINVAL, "ExtraParams: smoothgradkern must be an odd number");
-    if (P->precision != EIK_PREC_FP64 && P->precision != EIK_PREC_MIXED) return ctx->fail(EIK_EINVAL, "bad precision");
-    if (P->fast != 0 && P->fast != 1) return ctx->fail(EIK_EINVAL, "fast must be 0 or 1");
-    if (P->refinearoundsrc && (P->downscalefactor < 1 || P->noderadius < 0))
-        return ctx->fail(EIK_EINVAL, "bad grid refinement parameters");
-    if (P->refinearoundsrc && M->g.kind == EIK_GRID3D_SPHERE)
-        return ctx->fail(EIK_EINVAL, "createfinegrid(): spherical coordinates in 3D still work in progress...");
-    S.nsrc = nsrc;
-    S.nd = nd;
-    // rec_off is read only when there are sources (it may be NULL otherwise); its prefix sums are
-    // checked before anything is sized from them, and no allocation failure crosses the ABI
-    if (nsrc > 0) {
-        if (rec_off[0] != 0) return ctx->fail(EIK_EINVAL, "rec_off[0] must be 0");
-        for (int64_t s = 0; s < nsrc; s++)
-            if (rec_off[s + 1] < rec_off[s]) return ctx->fail(EIK_EINVAL, "rec_off must be non-decreasing");
-        if (rec_off[nsrc] > kMaxReceivers) return ctx->fail(EIK_EINVAL, "too many receivers (%lld)", (long long)rec_off[nsrc]);
-    }
-    try {
-        S.src.assign(coordsrc, coordsrc + nsrc * nd);
-        if (nsrc > 0)
-            S.rec_off.assign(rec_off, rec_off + nsrc + 1);
-        else
-            S.rec_off.assign(1, 0);
-        const int64_t nrec = S.nrec();
-        if (nrec > 0 && !coordrec) return ctx->fail(EIK_EINVAL, "null receivers");
-        S.rec.assign(coordrec, coordrec + nrec * nd);
-    } catch (const std::bad_alloc&) {
-        return ctx->fail(EIK_ENOMEM, "out of host memory (sources and receivers)");
-    }
-    const int64_t nrec = S.nrec();
-    for (int d = 0; d < nd; d++) {
-        const double lo = gcoord(M->g, d, 0), hi = gcoord(M->g, d, M->g.n[d] - 1);
-        for (int64_t s = 0; s < nsrc; s++) {
-            const double x = S.src[s * nd + d];
-            if (!(lo <= x && x <= hi)) return ctx->fail(EIK_EASSERT, "source %lld outside the grid", (long long)s);
-        }
-        for (int64_t r = 0; r < nrec; r++) {
-            const double x = S.rec[r * nd + d];
-            if (!(lo <= x && x <= hi)) return ctx->fail(EIK_EASSERT, "receiver %lld outside the grid", (long long)r);
-        }
-    }
-    return EIK_OK;
-}
-
-// Mixed precision keeps the solve near the source in FP64.  Without refinement that region is an
-// FP64 box of +-kNearSourceRadius nodes at the coarse spacing, solved with the refinement machinery
-// at downscalefactor 1: the nodes it hands to the FP32 solve are those the heap accepts before the
-// front reaches the box faces, whose values do not depend on the box, so the coarse problem is
-// the reference's unrefined one.
-static const int32_t kNearSourceRadius = 4;
-static eik_params effective_params(const eik_params* p) {
-    eik_params e = *p;
-    if (e.precision == EIK_PREC_MIXED && !e.refinearoundsrc) {
-        e.refinearoundsrc = 1;
-        e.downscalefactor = 1;
-        e.noderadius = kNearSourceRadius;
-    }
-    return e;
 }
 
 int eik_traveltime(eik_ctx* ctx, eik_model* model, int64_t nsrc, const double* coordsrc, const int64_t* rec_off,
@@ -435,6 +437,17 @@ int eik_gradient(eik_ctx* ctx, eik_model* model, int64_t nsrc, const double* coo
     rc = gradient_solve(ctx, model, S, params, whichgrad, pickobs, stdobs, gradvel, gradsrc, misf);
     ctx->stats[0] = total.mark();
     return rc;
+}
+
+int eik_mem_info(eik_ctx* ctx, int64_t* free_bytes, int64_t* total_bytes) {
+    if (!ctx || !free_bytes || !total_bytes) return EIK_EINVAL;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    size_t f = 0, t = 0;
+    CK(hipMemGetInfo(&f, &t));
+    *free_bytes = (int64_t)f;
+    *total_bytes = (int64_t)t;
+    return EIK_OK;
 }
 
 int eik_last_stats(const eik_ctx* ctx, double* out, int32_t nout) {

@@ -287,15 +287,37 @@ __host__ __device__ __forceinline__ double interp_receiver(const T* tt, const In
 }  // namespace eik
 
 // ------------------------------------------------------------------ host-side objects
+// A fixed block of device memory handed out in 256-byte aligned pieces and never freed piecewise
+// (the no-allocation entry point's scratch); reset() starts a call over from an empty block.
+struct Arena {
+    char* base = nullptr;
+    size_t cap = 0;
+    size_t used = 0;
+    void* carve(size_t n) {
+        const size_t at = (used + 255) & ~(size_t)255;
+        if (!base || at > cap || n > cap - at) return nullptr;
+        used = at + n;
+        return base + at;
+    }
+};
+
+// A named device buffer of a context: grows with hipMalloc, or, when the context runs in arena
+// mode, is carved from the arena (no allocation).
 struct DevBuf {
     void* p = nullptr;
     size_t cap = 0;
+    Arena* arena = nullptr;
     int ensure(size_t n) {
         if (n <= cap && p) return 0;
+        const size_t want = n ? n : 16;
+        if (arena) {
+            p = arena->carve(want);
+            cap = p ? want : 0;
+            return p ? 0 : -1;
+        }
         if (p) (void)hipFree(p);
         p = nullptr;
         cap = 0;
-        size_t want = n ? n : 16;
         if (hipMalloc(&p, want) != hipSuccess) {
             p = nullptr;
             return -1;
@@ -304,7 +326,7 @@ struct DevBuf {
         return 0;
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p && !arena) (void)hipFree(p);
         p = nullptr;
         cap = 0;
     }
@@ -331,7 +353,15 @@ struct eik_ctx {
     double stats[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     int xcap = 1;  // capacity per source of the transferred-point list
     std::mutex mu;
-    DevBuf& buf(const char* name) { return bufs[name]; }
+    // arena mode (the no-allocation entry point): buffers are carved from this block instead
+    Arena* arena = nullptr;
+    std::map<std::string, DevBuf> abufs;
+    DevBuf& buf(const char* name) {
+        if (!arena) return bufs[name];
+        DevBuf& b = abufs[name];
+        b.arena = arena;
+        return b;
+    }
     int fail(int code, const char* fmt, ...) {
         char b[512];
         va_list ap;
@@ -432,5 +462,15 @@ struct TTPools {
 };
 int fim_run(eik_ctx* ctx, int nd, int prec, int kind, const InstDev* d_inst, int ninst, int64_t ntiles,
             const TTPools& pools, const void* slowpool, bool exact, FimStats* fs);
+
+// no-allocation scratch (eik_noalloc.hip): resolve this device's copy once (context creation)
+void scratch_setup(int device);
+
+// argument checks shared by the entry points (eik_capi.hip): checksrcrecposition and the
+// ExtraParams checks; the parameters a call actually runs with (mixed precision's FP64 box)
+int load_sources(eik_ctx* ctx, eik_model* M, int64_t nsrc, const double* coordsrc, const int64_t* rec_off,
+                 const double* coordrec, const eik_params* P, SourceSet& S);
+eik_params effective_params(const eik_params* p);
+
 
 }  // namespace eik

@@ -29,6 +29,7 @@ EXPORTS = (
     "eik_traveltime", "eik_gradient", "eik_misfit", "eik_last_stats",
     "eik_server_create", "eik_server_destroy", "eik_submit", "eik_wait", "eik_server_stats",
     "eik_tracerays", "eik_traveltime_batch", "eik_poll", "eik_tracerays_max_steps",
+    "eik_traveltime_noalloc", "eik_mem_info",
 )
 
 
@@ -91,6 +92,10 @@ def load():
         L.eik_model_destroy.restype = None
         L.eik_traveltime.argtypes = [vp, vp, ctypes.c_int64, dp, ip, dp, P(EikParams), dp, dp]
         L.eik_traveltime.restype = ctypes.c_int
+        L.eik_traveltime_noalloc.argtypes = [vp, vp, ctypes.c_int64, dp, ip, dp, P(EikParams), dp, dp]
+        L.eik_traveltime_noalloc.restype = ctypes.c_int
+        L.eik_mem_info.argtypes = [vp, ip, ip]
+        L.eik_mem_info.restype = ctypes.c_int
         L.eik_gradient.argtypes = [vp, vp, ctypes.c_int64, dp, ip, dp, dp, dp, ctypes.c_int32, P(EikParams),
                                    dp, dp, dp]
         L.eik_gradient.restype = ctypes.c_int

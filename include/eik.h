@@ -111,6 +111,22 @@ int eik_misfit(eik_ctx* ctx, eik_model* model, int64_t nsrc, const double* coord
                const int64_t* rec_off, const double* coordrec, const double* pickobs,
                const double* stdobs, const eik_params* params, double* misf);
 
+/* eik_traveltime without any device allocation.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Same arguments, results and errors as eik_traveltime (FP64: bit for bit);
+ * every device buffer of the call is carved from a fixed 128 MiB scratch block per GPU (a __device__
+ * array held from library load until the process ends), and a call whose buffers would not fit is
+ * rejected with EIK_ENOMEM before any device work (about one 201x201x101 source in FP64).  The block
+ * keeps the last call's contents; calls on one GPU from any context of the process run one at a
+ * time (the library serialises them; overlapping calls would otherwise race on the block). */
+int eik_traveltime_noalloc(eik_ctx* ctx, eik_model* model, int64_t nsrc, const double* coordsrc,
+                           const int64_t* rec_off, const double* coordrec, const eik_params* params,
+                           double* ttpicks, double* ttgrid);
+
+/* Free and total memory of the context's GPU (hipMemGetInfo through the HIP runtime this library uses),
+ * for callers that size their calls; no reference counterpart. */
+int eik_mem_info(eik_ctx* ctx, int64_t* free_bytes, int64_t* total_bytes);
+
 /* Timing of the last call's device work (ms), split by phase; no reference counterpart.
  * out[0] = total, [1] = near-source (fine grid) solve, [2] = coarse solve, [3] = adjoint,
  * [4] = tile activations of the coarse solve (node-visits = activations x tile nodes),
