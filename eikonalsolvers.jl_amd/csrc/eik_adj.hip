@@ -71,7 +71,7 @@ __device__ __forceinline__ bool before_stop(const InstDev& I, const Key<T>& k) {
 
 // Codes and G_d of every node: replay of the final stencil evaluation (eikmainfwd.jl:794-809)
 // and 2*(D_d t)_i accumulated over the stencil in acceptance order (far node first).
-template <int ND, typename T>
+template <int ND, typename T, bool MIX>
 __global__ void __launch_bounds__(kAB) k_adj_prep(const InstDev* __restrict__ inst, TTPools pools,
                                                   const T* __restrict__ slowpool, uint16_t* __restrict__ code_out,
                                                   double* __restrict__ G, int64_t gstride) {
@@ -129,12 +129,25 @@ __global__ void __launch_bounds__(kAB) k_adj_prep(const InstDev* __restrict__ in
                     v2[k] = fabs(b);
                 }
             }
-            Metric<T> mt;
-            node_spacing<T>(I, g, mt.dh);
-            metric_coefs(mt, ND);
+            using A = Arith<T, MIX>;
             int code[3] = {0, 0, 0};
             Key<T> kz;
-            (void)replay<ND, T>(v1, v2, k1, k2, f1, f2, ok1, ok2, slow[l], mt, code, &kz);
+            bool near64 = false;  // the forward's FP64 near-source stencil
+            if constexpr (MIX) {
+                near64 = in_fp64_region(I, g);
+                if (near64) {
+                    Metric<double> m64;
+                    node_spacing<double>(I, g, m64.dh);
+                    metric_coefs(m64, ND);
+                    (void)replay<ND, T, false>(v1, v2, k1, k2, f1, f2, ok1, ok2, (double)slow[l], m64, code, &kz);
+                }
+            }
+            if (!near64) {
+                Metric<A> mt;
+                node_spacing<A>(I, g, mt.dh);
+                metric_coefs(mt, ND);
+                (void)replay<ND, T, MIX>(v1, v2, k1, k2, f1, f2, ok1, ok2, (A)slow[l], mt, code, &kz);
+            }
 #pragma unroll
             for (int d = 0; d < ND; d++) {
                 const int c = code[d];
@@ -861,7 +874,7 @@ static int adj_run_t(eik_ctx* ctx, const InstDev* d_inst, int ninst, int64_t nti
 static int adj_run(eik_ctx* ctx, int nd, int prec, const InstDev* d_inst, int ninst, int64_t ntiles,
                    const uint16_t* code, const double* G, int64_t gstride, const double* b, double* lam,
                    const void* ttpool, FimStats* fs) {
-    if (prec == EIK_PREC_FP64)
+    if (prec != EIK_PREC_FP32)
         return nd == 2 ? adj_run_t<2, double>(ctx, d_inst, ninst, ntiles, code, G, gstride, b, lam,
                                               (const double*)ttpool, fs)
                        : adj_run_t<3, double>(ctx, d_inst, ninst, ntiles, code, G, gstride, b, lam,
@@ -870,14 +883,14 @@ static int adj_run(eik_ctx* ctx, int nd, int prec, const InstDev* d_inst, int ni
                    : adj_run_t<3, float>(ctx, d_inst, ninst, ntiles, code, G, gstride, b, lam, (const float*)ttpool, fs);
 }
 
-template <typename T>
+template <typename T, bool MIX>
 static void launch_prep(int nd, hipStream_t st, const InstDev* inst, int ninst, const TTPools& pools,
                         const void* slow, uint16_t* code, double* G, int64_t gstride) {
     dim3 grid(256, (unsigned)ninst);
     if (nd == 2)
-        k_adj_prep<2, T><<<grid, kAB, 0, st>>>(inst, pools, (const T*)slow, code, G, gstride);
+        k_adj_prep<2, T, MIX><<<grid, kAB, 0, st>>>(inst, pools, (const T*)slow, code, G, gstride);
     else
-        k_adj_prep<3, T><<<grid, kAB, 0, st>>>(inst, pools, (const T*)slow, code, G, gstride);
+        k_adj_prep<3, T, MIX><<<grid, kAB, 0, st>>>(inst, pools, (const T*)slow, code, G, gstride);
 }
 
 static void taper_box(const eik_grid& g, const double* src, int R, TaperDev& T) {
@@ -919,7 +932,7 @@ int gradient_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, const eik_par
     Timer tm(ctx);
     const InstDev* dci = (const InstDev*)ctx->buf("inst_coarse").p;
     void* ttc = ctx->buf("tt_coarse").p;
-    const void* slow = prec == EIK_PREC_FP64 ? (const void*)M->d_slow64 : (const void*)M->d_slow32;
+    const void* slow = prec == EIK_PREC_FP32 ? (const void*)M->d_slow32 : (const void*)M->d_slow64;
     const int TX = adj_tile_edge(nd), TZ = nd == 2 ? 1 : TX;
     const int64_t tiles_per = (int64_t)((M->g.n[0] + TX - 1) / TX) * ((M->g.n[1] + TX - 1) / TX) *
                               (nd == 3 ? (M->g.n[2] + TZ - 1) / TZ : 1);
@@ -934,9 +947,11 @@ int gradient_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, const eik_par
     CK(hipMemsetAsync(b_c.p, 0, (size_t)P_c * sizeof(double), st));
     const TTPools cpools{ttc, ctx->buf("key_K").p, ctx->buf("key_S").p, (uint8_t*)ctx->buf("key_D").p};
     if (prec == EIK_PREC_FP64)
-        launch_prep<double>(nd, st, dci, (int)nsrc, cpools, slow, (uint16_t*)code_c.p, (double*)G_c.p, P_c);
+        launch_prep<double, false>(nd, st, dci, (int)nsrc, cpools, slow, (uint16_t*)code_c.p, (double*)G_c.p, P_c);
+    else if (prec == EIK_PREC_MIXED)
+        launch_prep<double, true>(nd, st, dci, (int)nsrc, cpools, slow, (uint16_t*)code_c.p, (double*)G_c.p, P_c);
     else
-        launch_prep<float>(nd, st, dci, (int)nsrc, cpools, slow, (uint16_t*)code_c.p, (double*)G_c.p, P_c);
+        launch_prep<float, false>(nd, st, dci, (int)nsrc, cpools, slow, (uint16_t*)code_c.p, (double*)G_c.p, P_c);
     CK(hipGetLastError());
 
     // receivers: rhs, misfit
@@ -962,7 +977,7 @@ int gradient_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, const eik_par
     unsigned* apend = (unsigned*)ctx->buf("adj_pend").p;
     uint8_t* flags = (uint8_t*)ctx->buf("fim_flags").p;
     if (nrec > 0) {
-        if (prec == EIK_PREC_FP64)
+        if (prec != EIK_PREC_FP32)
             k_adj_rhs<double><<<nb(nrec, 128), 128, 0, st>>>((const double*)drec.p, (const int*)drs.p, nrec, dci,
                                                              (const double*)ttc, nd, kind, (const double*)dpo.p,
                                                              (const double*)dso.p, (double*)b_c.p, (double*)dmt.p, flags, apend);
@@ -1072,7 +1087,7 @@ int gradient_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, const eik_par
         CK(hipMemsetAsync(dgi.p, 0, (size_t)P_f * sizeof(double), st));
         CK(hipMemcpyAsync(dfp.p, fpts.data(), nfp * sizeof(SrcPt), hipMemcpyHostToDevice, st));
         const TTPools fpools{(void*)ttf, ctx->buf("keyf_K").p, ctx->buf("keyf_S").p, (uint8_t*)ctx->buf("keyf_D").p};
-        launch_prep<double>(nd, st, dfi, (int)nsrc, fpools, slf, (uint16_t*)code_f.p, (double*)G_f.p, P_f);
+        launch_prep<double, false>(nd, st, dfi, (int)nsrc, fpools, slf, (uint16_t*)code_f.p, (double*)G_f.p, P_f);
         if (npts > 0)
             k_fine_rhs<<<nb(npts, 64), 64, 0, st>>>((const SrcPt*)dpts.p, npts, dfi, ttf, (const double*)dup.p,
                                                     (const double*)dus.p, (double*)b_f.p, (double*)haq.p, flags, apend,
@@ -1137,7 +1152,7 @@ int gradient_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, const eik_par
         for (int64_t s = 0; s < nsrc; s++) taper_box(M->g, S.src.data() + s * nd, P->radiussmoothgradsrc, tap[s]);
         CK(hipMemcpyAsync(dtap.p, tap.data(), nsrc * sizeof(TaperDev), hipMemcpyHostToDevice, st));
         dim3 gi(512, (unsigned)nsrc);
-        if (prec == EIK_PREC_FP64)
+        if (prec != EIK_PREC_FP32)
             k_grad_img<double><<<gi, 256, 0, st>>>(dci, (const double*)ttc, (const double*)lam_c.p, M->d_vel,
                                                    (double*)img.p, N);
         else

@@ -18,6 +18,10 @@ void make_inst(const eik_grid& g, InstDev& I) {
     I.kind = g.kind;
     I.dthrad = deg2rad(g.d[1]);
     I.dphrad = deg2rad(g.d[2]);
+    for (int d = 0; d < 3; d++) {
+        I.xlo[d] = 1;  // no FP64 region
+        I.xhi[d] = 0;
+    }
     I.tstopK = INFINITY;
     I.tstopS = -INFINITY;
     I.tstopD = 0;
@@ -117,7 +121,8 @@ int load_sources(eik_ctx* ctx, eik_model* M, int64_t nsrc, const double* coordsr
     if (P->smoothgradkern < 0) return ctx->fail(EIK_EASSERT, "smoothgradkern>=0");
     if (P->smoothgradkern > 0 && P->smoothgradkern % 2 == 0)
         return ctx->fail(EIK_EINVAL, "ExtraParams: smoothgradkern must be an odd number");
-    if (P->precision != EIK_PREC_FP64 && P->precision != EIK_PREC_MIXED) return ctx->fail(EIK_EINVAL, "bad precision");
+    if (P->precision != EIK_PREC_FP64 && P->precision != EIK_PREC_MIXED && P->precision != EIK_PREC_FP32)
+        return ctx->fail(EIK_EINVAL, "bad precision");
     if (P->fast != 0 && P->fast != 1) return ctx->fail(EIK_EINVAL, "fast must be 0 or 1");
     if (P->refinearoundsrc && (P->downscalefactor < 1 || P->noderadius < 0))
         return ctx->fail(EIK_EINVAL, "bad grid refinement parameters");
@@ -160,15 +165,18 @@ int load_sources(eik_ctx* ctx, eik_model* M, int64_t nsrc, const double* coordsr
     return EIK_OK;
 }
 
-// Mixed precision keeps the solve near the source in FP64.  Without refinement that region is an
-// FP64 box of +-kNearSourceRadius nodes at the coarse spacing, solved with the refinement machinery
-// at downscalefactor 1: the nodes it hands to the FP32 solve are those the heap accepts before the
-// front reaches the box faces, whose values do not depend on the box, so the coarse problem is
-// the reference's unrefined one.
-static const int32_t kNearSourceRadius = 4;
-eik_params effective_params(const eik_params* p) {
+// The FP32 precision keeps the solve near the source in FP64.  Without refinement that region is
+// an FP64 box of +-kNearSourceRadius nodes at the coarse spacing, solved with the refinement
+// machinery at downscalefactor 1: the nodes it hands to the coarse solve are those the heap
+// accepts before the front reaches the box faces, whose values do not depend on the box, so the
+// coarse traveltimes are the reference's unrefined ones (the gradient then follows the refined
+// formulas near the source).  The mixed precision needs no box: its traveltimes are FP64, so the
+// nodes within kNearSourceRadius of the source box simply use the FP64 stencil in the coarse solve
+// (forward_solve), and the problem, gradient included, stays the unrefined one.
+eik_params effective_params(const eik_params* p, int kind) {
     eik_params e = *p;
-    if (e.precision == EIK_PREC_MIXED && !e.refinearoundsrc) {
+    e.precision = run_precision(e.precision, kind);
+    if (e.precision == EIK_PREC_FP32 && !e.refinearoundsrc) {
         e.refinearoundsrc = 1;
         e.downscalefactor = 1;
         e.noderadius = kNearSourceRadius;
@@ -323,7 +331,7 @@ int eik_traveltime(eik_ctx* ctx, eik_model* model, int64_t nsrc, const double* c
     SourceSet S;
     int rc = load_sources(ctx, model, nsrc, coordsrc, rec_off, coordrec, params_in, S);
     if (rc) return rc;
-    const eik_params pe = effective_params(params_in);
+    const eik_params pe = effective_params(params_in, model->g.kind);
     const eik_params* params = &pe;
     if (S.nrec() > 0 && !ttpicks) return ctx->fail(EIK_EINVAL, "null picks buffer");
     if (nsrc == 0) return EIK_OK;
@@ -358,7 +366,7 @@ int eik_traveltime_batch(eik_ctx* ctx, int32_t nreq, const eik_request* reqs) {
         nrec += SS[i].nrec();
     }
     if (G.empty()) return EIK_OK;
-    const eik_params pe = effective_params(&reqs[0].params);
+    const eik_params pe = effective_params(&reqs[0].params, reqs[0].model->g.kind);
     for (double& x : ctx->stats) x = 0.0;
     Timer total(ctx);
     int rc = forward_solve(ctx, G, pe.precision, &pe);
@@ -417,7 +425,7 @@ int eik_gradient(eik_ctx* ctx, eik_model* model, int64_t nsrc, const double* coo
     SourceSet S;
     int rc = load_sources(ctx, model, nsrc, coordsrc, rec_off, coordrec, params_in, S);
     if (rc) return rc;
-    const eik_params pe = effective_params(params_in);
+    const eik_params pe = effective_params(params_in, model->g.kind);
     const eik_params* params = &pe;
     const int kind = model->g.kind;
     // Grid3DSphere: the reference raises here (createsparsederivativematrices!, eikmainfwd.jl:885-887);

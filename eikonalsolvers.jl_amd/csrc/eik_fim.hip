@@ -146,8 +146,9 @@ struct TileLds {
 // face's bit and the smallest changed key as its pending value.  Blocks take list entries
 // (local tile, instance, pending value, face bits) from a cursor, claiming the next entry while
 // the current tile runs.  SPH: spherical grids (per-node metric); Cartesian grids share one
-// metric, held in scalar registers.
-template <int ND, typename T, bool SPH>
+// metric, held in scalar registers.  MIX: the mixed precision (FP64 values, keys and comparisons,
+// the quadratic's increment in FP32; eik_stencil.h).
+template <int ND, typename T, bool SPH, bool MIX>
 __global__ void __attribute__((amdgpu_flat_work_group_size(kTileThreads, kTileThreads),
                                amdgpu_waves_per_eu(RoundOcc<ND, T>::value))) k_fim_round(
     const uint4* __restrict__ list, unsigned* __restrict__ count, const InstDev* __restrict__ inst,
@@ -156,6 +157,7 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(kTileThreads, kTileTh
     unsigned* __restrict__ pend, unsigned long long* __restrict__ counters, T settle, unsigned chunk) {
     using S = TileShape<ND>;
     using L = TileLds<ND>;
+    using A = Arith<T, MIX>;  // arithmetic of the quadratic
     constexpr int NN = S::NODES, NL = L::NL, E = L::E;
     static_assert(NN == kTileThreads, "one node per lane");
     static_assert(NL % NN == 0, "staging loop");
@@ -308,10 +310,11 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(kTileThreads, kTileTh
             }
         }
 
-        Metric<T> mcart;
+        Metric<A> mcart;
 #pragma unroll
-        for (int ax = 0; ax < 3; ax++) mcart.dh[ax] = (T)I.h[0];
+        for (int ax = 0; ax < 3; ax++) mcart.dh[ax] = (A)I.h[0];
         metric_coefs(mcart, ND);
+        const A sa = (A)sl;
 int it = 0;
         for (; it < S::MAXIT; it++) {
             const int cb = it & 1, pb = cb ^ 1;
@@ -361,14 +364,26 @@ int it = 0;
                 };
                 int code[3];
                 Key<T> nk;
-                T nv;
-                if constexpr (SPH) {
-                    Metric<T> mt;
-                    node_spacing<T>(I, g, mt.dh);
+                T nv = tinf<T>();
+                bool near64 = false;  // mixed precision near the source: the FP64 stencil
+                if constexpr (MIX) {
+                    near64 = in_fp64_region(I, g);
+                    if (near64) {
+                        Metric<double> m64;
+                        node_spacing<double>(I, g, m64.dh);
+                        metric_coefs(m64, ND);
+                        nv = replay<ND, T, false>(v1, v2, keys, d1, d2, dcap, f1, f2, ok1, ok2, (double)sl, m64, code,
+                                                  &nk);
+                    }
+                }
+                if (near64) {
+                } else if constexpr (SPH) {
+                    Metric<A> mt;
+                    node_spacing<A>(I, g, mt.dh);
                     metric_coefs(mt, ND);
-                    nv = replay<ND, T>(v1, v2, keys, d1, d2, dcap, f1, f2, ok1, ok2, sl, mt, code, &nk);
+                    nv = replay<ND, T, MIX>(v1, v2, keys, d1, d2, dcap, f1, f2, ok1, ok2, sa, mt, code, &nk);
                 } else {
-                    nv = replay<ND, T>(v1, v2, keys, d1, d2, dcap, f1, f2, ok1, ok2, sl, mcart, code, &nk);
+                    nv = replay<ND, T, MIX>(v1, v2, keys, d1, d2, dcap, f1, f2, ok1, ok2, sa, mcart, code, &nk);
                 }
                 tot_upd++;
                 const T cur = s_t[n];
@@ -778,11 +793,21 @@ static double f32_settle() {
     return v;
 }
 
+// Relative size below which a mixed-precision change wakes no other node (EIK_MIX_SETTLE overrides;
+// default 0: every change propagates and the mixed solve replays the mixed heap FMM exactly).
+static double mix_settle() {
+    static const double v = [] {
+        const char* e = getenv("EIK_MIX_SETTLE");
+        return e ? atof(e) : 0.0;
+    }();
+    return v;
+}
+
 // Rounds are enqueued in batches of kRoundBatch without host synchronisation: each round resets
 // the per-instance minimum, selects the tiles within the band, and runs the round kernel on a
 // fixed grid of resident blocks that take tiles from the device-side list.  After a batch the
 // host reads how many tiles are still flagged; none means converged.
-template <int ND, typename T, bool SPH>
+template <int ND, typename T, bool SPH, bool MIX>
 static int fim_run_t(eik_ctx* ctx, const InstDev* d_inst, int ninst, int64_t ntiles, const TTPools& P,
                      const T* slowpool, bool exact, FimStats* fs) {
     DevBuf& list = ctx->buf("fim_tile_list");
@@ -805,14 +830,14 @@ static int fim_run_t(eik_ctx* ctx, const InstDev* d_inst, int ninst, int64_t nti
     static const int grid = [ctx] {
         int ncu = 0, occ = 0;
         if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess) ncu = 1;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_fim_round<ND, T, SPH>, kTileThreads, 0) != hipSuccess)
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_fim_round<ND, T, SPH, MIX>, kTileThreads, 0) != hipSuccess)
             occ = 1;
         constexpr int kMaxPerCu = ND == 3 ? 8 : 5;
         int gr = std::max(1, ncu * std::min(kMaxPerCu, std::max(1, occ)));
         if (const char* e = getenv("EIK_GRID")) gr = std::max(1, atoi(e));
         if (getenv("EIK_VERBOSE"))
-            fprintf(stderr, "libeik: k_fim_round<%d,%s,%d> %d CUs x %d blocks -> grid %d\n", ND,
-                    sizeof(T) == 8 ? "double" : "float", (int)SPH, ncu, occ, gr);
+            fprintf(stderr, "libeik: k_fim_round<%d,%s,%d,%d> %d CUs x %d blocks -> grid %d\n", ND,
+                    sizeof(T) == 8 ? "double" : "float", (int)SPH, (int)MIX, ncu, occ, gr);
         return gr;
     }();
     const int g = (int)std::min<int64_t>(grid, ntiles);
@@ -829,7 +854,7 @@ static int fim_run_t(eik_ctx* ctx, const InstDev* d_inst, int ninst, int64_t nti
     // (mixed precision): changes below f32_settle() relative (a few ulps) never propagate; they
     // are rounding ripples, and at heap near-ties within FP32 resolution they can otherwise
     // circulate for thousands of rounds before the settling stage ends them.
-    const T base_settle = sizeof(T) == 8 ? (!exact ? settle_tol : T(0)) : T(f32_settle());
+    const T base_settle = MIX ? T(mix_settle()) : (sizeof(T) == 8 ? (!exact ? settle_tol : T(0)) : T(f32_settle()));
     for (int64_t batch = 0;; batch++) {
         if (batch >= maxbatches) return ctx->fail(EIK_ENOCONV, "traveltime solve did not converge");
         // the tolerance grows tenfold every 500 rounds while the cycle persists
@@ -847,7 +872,7 @@ static int fim_run_t(eik_ctx* ctx, const InstDev* d_inst, int ninst, int64_t nti
                                                                (const unsigned*)wm.p, (uint4*)list.p,
                                                                d_cnt + kCntList);
             CK(hipEventRecord(ctx->kev[2 * r], st));
-            k_fim_round<ND, T, SPH><<<g, kTileThreads, 0, st>>>((const uint4*)list.p, d_cnt, d_inst, (const unsigned*)wm.p,
+            k_fim_round<ND, T, SPH, MIX><<<g, kTileThreads, 0, st>>>((const uint4*)list.p, d_cnt, d_inst, (const unsigned*)wm.p,
                                                           (T*)P.tt, (T*)P.k, (T*)P.s, P.d, slowpool, flags, pend,
                                                           d_c64, settle, claim_chunk());
             CK(hipGetLastError());
@@ -876,21 +901,26 @@ static int fim_run_t(eik_ctx* ctx, const InstDev* d_inst, int ninst, int64_t nti
     return EIK_OK;
 }
 
-template <int ND, typename T>
+template <int ND, typename T, bool MIX>
 static int fim_run_k(eik_ctx* ctx, bool sph, const InstDev* d_inst, int ninst, int64_t ntiles, const TTPools& P,
                      const void* slowpool, bool exact, FimStats* fs) {
-    return sph ? fim_run_t<ND, T, true>(ctx, d_inst, ninst, ntiles, P, (const T*)slowpool, exact, fs)
-               : fim_run_t<ND, T, false>(ctx, d_inst, ninst, ntiles, P, (const T*)slowpool, exact, fs);
+    return sph ? fim_run_t<ND, T, true, MIX>(ctx, d_inst, ninst, ntiles, P, (const T*)slowpool, exact, fs)
+               : fim_run_t<ND, T, false, MIX>(ctx, d_inst, ninst, ntiles, P, (const T*)slowpool, exact, fs);
+}
+
+template <int ND>
+static int fim_run_nd(eik_ctx* ctx, int prec, bool sph, const InstDev* d_inst, int ninst, int64_t ntiles,
+                      const TTPools& P, const void* slowpool, bool exact, FimStats* fs) {
+    if (prec == EIK_PREC_FP32) return fim_run_k<ND, float, false>(ctx, sph, d_inst, ninst, ntiles, P, slowpool, exact, fs);
+    if (prec == EIK_PREC_MIXED) return fim_run_k<ND, double, true>(ctx, sph, d_inst, ninst, ntiles, P, slowpool, exact, fs);
+    return fim_run_k<ND, double, false>(ctx, sph, d_inst, ninst, ntiles, P, slowpool, exact, fs);
 }
 
 int fim_run(eik_ctx* ctx, int nd, int prec, int kind, const InstDev* d_inst, int ninst, int64_t ntiles,
             const TTPools& pools, const void* slowpool, bool exact, FimStats* fs) {
     const bool sph = kind == EIK_GRID2D_SPHERE || kind == EIK_GRID3D_SPHERE;
-    if (nd == 2)
-        return prec == EIK_PREC_FP64 ? fim_run_k<2, double>(ctx, sph, d_inst, ninst, ntiles, pools, slowpool, exact, fs)
-                                     : fim_run_k<2, float>(ctx, sph, d_inst, ninst, ntiles, pools, slowpool, exact, fs);
-    return prec == EIK_PREC_FP64 ? fim_run_k<3, double>(ctx, sph, d_inst, ninst, ntiles, pools, slowpool, exact, fs)
-                                 : fim_run_k<3, float>(ctx, sph, d_inst, ninst, ntiles, pools, slowpool, exact, fs);
+    return nd == 2 ? fim_run_nd<2>(ctx, prec, sph, d_inst, ninst, ntiles, pools, slowpool, exact, fs)
+                   : fim_run_nd<3>(ctx, prec, sph, d_inst, ninst, ntiles, pools, slowpool, exact, fs);
 }
 
 // ----------------------------------------------------------------- forward pass of one call
@@ -908,8 +938,10 @@ static double band_mult(int nd, int prec) {
         return e ? atof(e) : NAN;
     }();
     if (env == env) return env;
-    const bool f32 = prec != EIK_PREC_FP64;
-    return nd == 3 ? (f32 ? 6.0 : 4.0) : (f32 ? 16.0 : 12.0);
+    if (prec == EIK_PREC_FP32) return nd == 3 ? 6.0 : 16.0;
+    // mixed on 201x201x101 x 16 sources: 4 -> 6 crossings 4 % faster, 8 the same as 6
+    if (prec == EIK_PREC_MIXED) return nd == 3 ? 6.0 : 12.0;
+    return nd == 3 ? 4.0 : 12.0;
 }
 
 // EIK_NODEBAND: the same for the nodes inside a tile (default 0: a listed tile relaxes all of
@@ -983,8 +1015,9 @@ int forward_solve(eik_ctx* ctx, const std::vector<FwdGroup>& G, int prec, const 
     Timer tm(ctx);
 
     // slowness of the models: a single model's own arrays, or the models' arrays side by side
-    const size_t esz = prec == EIK_PREC_FP64 ? sizeof(double) : sizeof(float);
-    const void* slow = prec == EIK_PREC_FP64 ? (const void*)M0->d_slow64 : (const void*)M0->d_slow32;
+    const bool f32 = prec == EIK_PREC_FP32;  // FP32 traveltime and key pools
+    const size_t esz = f32 ? sizeof(float) : sizeof(double);
+    const void* slow = f32 ? (const void*)M0->d_slow32 : (const void*)M0->d_slow64;
     std::vector<int64_t> soff(G.size(), 0);
     if (G.size() > 1) {
         int64_t tot = 0;
@@ -995,7 +1028,7 @@ int forward_solve(eik_ctx* ctx, const std::vector<FwdGroup>& G, int prec, const 
         DevBuf& sb = ctx->buf("slow_batch");
         if (sb.ensure((size_t)tot * esz)) return ctx->fail(EIK_ENOMEM, "out of device memory (batched models)");
         for (size_t k = 0; k < G.size(); k++) {
-            const void* src = prec == EIK_PREC_FP64 ? (const void*)G[k].M->d_slow64 : (const void*)G[k].M->d_slow32;
+            const void* src = f32 ? (const void*)G[k].M->d_slow32 : (const void*)G[k].M->d_slow64;
             CK(hipMemcpyAsync((char*)sb.p + soff[k] * esz, src, (size_t)G[k].M->N * esz, hipMemcpyDeviceToDevice, st));
         }
         slow = sb.p;
@@ -1020,6 +1053,21 @@ int forward_solve(eik_ctx* ctx, const std::vector<FwdGroup>& G, int prec, const 
                 I.tile_base = ntiles_c;
                 I.atile_base = antiles_c;
                 set_band(I, nd, prec, M->smean);
+                if (prec == EIK_PREC_MIXED && !P->refinearoundsrc) {
+                    // unrefined mixed solve: the stencil stays FP64 within kNearSourceRadius nodes of
+                    // the source box
+                    int corn[8][3];
+                    const int nc = find_enclosing_box(M->g, sxyz.data() + s * nd, corn);
+                    for (int d = 0; d < 3; d++) {
+                        int lo = corn[0][d], hi = corn[0][d];
+                        for (int c = 1; c < nc; c++) {
+                            lo = std::min(lo, corn[c][d]);
+                            hi = std::max(hi, corn[c][d]);
+                        }
+                        I.xlo[d] = d < nd ? lo - kNearSourceRadius : 0;
+                        I.xhi[d] = d < nd ? hi + kNearSourceRadius : 0;
+                    }
+                }
                 totN += M->N;
                 ntiles_c += (int64_t)I.t[0] * I.t[1] * I.t[2];
                 antiles_c += (int64_t)I.at[0] * I.at[1] * I.at[2];
@@ -1090,7 +1138,7 @@ int forward_solve(eik_ctx* ctx, const std::vector<FwdGroup>& G, int prec, const 
     k_fill_u32<<<nblocks(maxt, 256), 256, 0, st>>>((unsigned*)pend.p, maxt, 0x7f800000u);
 
     // coarse grids start at +inf
-    if (prec == EIK_PREC_FP64)
+    if (prec != EIK_PREC_FP32)
         k_fill<double><<<nblocks(totN, 256), 256, 0, st>>>((double*)ttc.p, totN, EIK_INF);
     else
         k_fill<float><<<nblocks(totN, 256), 256, 0, st>>>((float*)ttc.p, totN, EIK_INFF);
@@ -1151,7 +1199,7 @@ int forward_solve(eik_ctx* ctx, const std::vector<FwdGroup>& G, int prec, const 
         ctx->xcap = cap;
         if (ntr.ensure((size_t)nsrc * sizeof(int)) || xl.ensure((size_t)nsrc * cap * sizeof(SrcPt)))
             return ctx->fail(EIK_ENOMEM, "out of device memory");
-        if (prec == EIK_PREC_FP64)
+        if (prec != EIK_PREC_FP32)
             k_transfer<double><<<(unsigned)nsrc, 256, 0, st>>>((const FineDev*)dfd.p, (InstDev*)dfi.p, fpools,
                                                               (const InstDev*)dci.p, (double*)ttc.p, (unsigned*)flags.p,
                                                               (unsigned*)pend.p, (int*)ntr.p, (SrcPt*)xl.p, cap, nd);
@@ -1177,7 +1225,7 @@ int forward_solve(eik_ctx* ctx, const std::vector<FwdGroup>& G, int prec, const 
         DevBuf& dcs = ctx->buf("corners");
         if (dcs.ensure(corners.size() * sizeof(CornerDev))) return ctx->fail(EIK_ENOMEM, "out of device memory");
         CK(hipMemcpyAsync(dcs.p, corners.data(), corners.size() * sizeof(CornerDev), hipMemcpyHostToDevice, st));
-        if (prec == EIK_PREC_FP64)
+        if (prec != EIK_PREC_FP32)
             k_corners<double><<<nblocks((int64_t)corners.size(), 64), 64, 0, st>>>(
                 (const CornerDev*)dcs.p, (int)corners.size(), (const InstDev*)dci.p, (double*)ttc.p, nullptr,
                 (unsigned*)flags.p, (unsigned*)pend.p, nd, kind);
@@ -1233,7 +1281,7 @@ int forward_picks(eik_ctx* ctx, const std::vector<FwdGroup>& G, int prec, double
         CK(hipMemcpyAsync(drs.p, rs.data(), (size_t)nrec * sizeof(int), hipMemcpyHostToDevice, st));
         const InstDev* dci = (const InstDev*)ctx->buf("inst_coarse").p;
         void* ttc = ctx->buf("tt_coarse").p;
-        if (prec == EIK_PREC_FP64)
+        if (prec != EIK_PREC_FP32)
             k_receivers<double><<<nblocks(nrec, 128), 128, 0, st>>>((const double*)drec.p, (const int*)drs.p, nrec, dci,
                                                                     (const double*)ttc, nd, kind, (double*)dpk.p);
         else
@@ -1250,7 +1298,7 @@ int forward_picks(eik_ctx* ctx, const std::vector<FwdGroup>& G, int prec, double
         void* ttc = ctx->buf("tt_coarse").p;
         int64_t off = 0;
         for (size_t s = 0; s < N.size(); s++) {
-            if (prec == EIK_PREC_FP64)
+            if (prec != EIK_PREC_FP32)
                 k_export<double><<<nblocks(N[s], 256), 256, 0, st>>>((const double*)ttc + off, (double*)ex.p, N[s]);
             else
                 k_export<float><<<nblocks(N[s], 256), 256, 0, st>>>((const float*)ttc + off, (double*)ex.p, N[s]);

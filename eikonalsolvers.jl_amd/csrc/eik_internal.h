@@ -44,7 +44,16 @@ struct InstDev {
     double band;         // band scheduler: tiles within this traveltime of the lowest pending one run
     double nband;        // the same for the nodes updated inside a tile
     double aband;        // adjoint rounds: tiles within this traveltime of the highest pending one run
+    // mixed precision: nodes inside [xlo, xhi] (per axis) use the FP64 stencil (the near-source
+    // region of an unrefined solve); empty when xlo > xhi
+    int32_t xlo[3], xhi[3];
 };
+
+// whether node g lies in the instance's FP64 region
+__host__ __device__ __forceinline__ bool in_fp64_region(const InstDev& I, const int* g) {
+    return g[0] >= I.xlo[0] && g[0] <= I.xhi[0] && g[1] >= I.xlo[1] && g[1] <= I.xhi[1] && g[2] >= I.xlo[2] &&
+           g[2] <= I.xhi[2];
+}
 
 // Refined box around one source (createfinegrid, src/EikSolv/eikmainfwd.jl:950-1104)
 struct FineDev {
@@ -470,7 +479,14 @@ void scratch_setup(int device);
 // ExtraParams checks; the parameters a call actually runs with (mixed precision's FP64 box)
 int load_sources(eik_ctx* ctx, eik_model* M, int64_t nsrc, const double* coordsrc, const int64_t* rec_off,
                  const double* coordrec, const eik_params* P, SourceSet& S);
-eik_params effective_params(const eik_params* p);
-
+eik_params effective_params(const eik_params* p, int kind);
+// nodes within this many of the source box stay FP64 in the reduced precisions
+constexpr int32_t kNearSourceRadius = 4;
+// The precision a grid kind runs with: Grid2DSphere keeps the FP64 stencil under the mixed
+// precision (its discrete adjoint amplifies rounding-level traveltime changes ~1e4-fold, so FP32
+// increments move its gradients beyond 1e-3).
+inline int run_precision(int prec, int kind) {
+    return (prec == EIK_PREC_MIXED && kind == EIK_GRID2D_SPHERE) ? EIK_PREC_FP64 : prec;
+}
 
 }  // namespace eik

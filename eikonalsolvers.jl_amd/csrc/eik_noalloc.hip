@@ -37,7 +37,7 @@ size_t al(size_t n) { return ((n ? n : 16) + 255) & ~(size_t)255; }
 size_t forward_bytes(const eik_model* M, const SourceSet& S, const eik_params& P, bool ttgrid) {
     const int nd = M->g.ndim;
     const int64_t nsrc = S.nsrc;
-    const size_t esz = P.precision == EIK_PREC_FP64 ? sizeof(double) : sizeof(float);
+    const size_t esz = P.precision == EIK_PREC_FP32 ? sizeof(float) : sizeof(double);
     InstDev I;
     make_inst(M->g, I);
     set_tiles(I, nd);
@@ -112,7 +112,7 @@ extern "C" int eik_traveltime_noalloc(eik_ctx* ctx, eik_model* model, int64_t ns
     SourceSet S;
     int rc = load_sources(ctx, model, nsrc, coordsrc, rec_off, coordrec, params_in, S);
     if (rc) return rc;
-    const eik_params pe = effective_params(params_in);
+    const eik_params pe = effective_params(params_in, model->g.kind);
     if (S.nrec() > 0 && !ttpicks) return ctx->fail(EIK_EINVAL, "null picks buffer");
     if (nsrc == 0) return EIK_OK;
     const size_t need = forward_bytes(model, S, pe, ttgrid != nullptr);

@@ -69,9 +69,9 @@ static int64_t request_bytes(const eik_request& r) {
     const eik_model* M = r.model;
     const int nd = M->g.ndim;
     const int64_t N = M->N;
-    const int64_t esz = r.params.precision == EIK_PREC_MIXED ? 4 : 8;
+    const int64_t esz = r.params.precision == EIK_PREC_FP32 ? 4 : 8;
     int64_t b = r.nsrc * N * (3 * esz + 1);
-    const bool refine = r.params.refinearoundsrc || r.params.precision == EIK_PREC_MIXED;
+    const bool refine = r.params.refinearoundsrc || r.params.precision == EIK_PREC_FP32;
     if (refine) {
         const int64_t df = r.params.refinearoundsrc ? std::max(1, r.params.downscalefactor) : 1;
         const int64_t rad = r.params.refinearoundsrc ? std::max(0, r.params.noderadius) : 4;

@@ -18,6 +18,10 @@
 // FMM gives it; in FP64 the arithmetic below follows the reference's operation order so the
 // results agree bit for bit.
 //
+// Arithmetic.  FP64 (eval64) follows the reference bit for bit.  The mixed precision (MIX, eval_mix)
+// keeps values, pop keys and every comparison of the replay in FP64 and forms only the quadratic's
+// increment over the smallest chosen neighbour in FP32; FP32 storage (eval32) does everything in FP32.
+//
 // Neighbour slot k = 2*axis + l, l = 0 for the + side and 1 for the - side (the reference scans
 // the + side first).  ok1 bit k: the adjacent node is inside the grid (isonb1st false); ok2 bit k:
 // that adjacent node is off every boundary plane (isonb2nd false, eikutils.jl:129-162), so the
@@ -42,6 +46,18 @@ __device__ __forceinline__ float tinf<float>() {
     return EIK_INFF;
 }
 
+
+// Arithmetic type of the quadratic: the storage type, except FP32 for the mixed precision.
+template <typename T, bool MIX>
+struct ArithOf {
+    using type = T;
+};
+template <>
+struct ArithOf<double, true> {
+    using type = float;
+};
+template <typename T, bool MIX>
+using Arith = typename ArithOf<T, MIX>::type;
 
 // Per-node stencil metric: spacing per axis and the quadratic coefficients of the 1st-order
 // (1/h^2) and 2nd-order (9/(4 h^2)) one-sided differences, computed once per node update rather
@@ -219,6 +235,73 @@ __device__ __forceinline__ float eval32(const float* v1, const float* v2, unsign
     return useoned ? oned : t;
 }
 
+// ---------------------------------------------------------------- mixed (FP64 values, FP32 increment)
+// Same neighbour selection as eval64 (in FP64).  The quadratic is eval32's shifted form, solved in
+// FP32 for delta = T - r with the chosen values' offsets c_a - r formed in FP64 and rounded once
+// to FP32; the result r + delta is formed in FP64, so a value carries the FP64 precision of its
+// smallest neighbour plus an FP32-accurate increment (~1e-7 of a node crossing), and the replay's
+// comparisons, which decide the heap order, stay exact.  The 1-D fallback is cmin + (h s in FP32).
+template <int ND>
+__device__ __forceinline__ double eval_mix(const double* v1, const double* v2, unsigned acc1, unsigned acc2, float s,
+                                           const Metric<float>& m, int* code) {
+#pragma clang fp contract(off)
+    AxisPick<double> p[3];
+    double r = EIK_INF;
+#pragma unroll
+    for (int ax = 0; ax < ND; ax++) {
+        p[ax] = pick_axis<double>(v1, v2, acc1, acc2, ax);
+        r = (p[ax].use1 && p[ax].cv1 < r) ? p[ax].cv1 : r;
+    }
+    float A = 0.f, B = 0.f, C = -(s * s);
+    float d1v[3];
+#pragma unroll
+    for (int ax = 0; ax < ND; ax++) {
+        const bool sec = p[ax].use1 && p[ax].use2;
+        const float al = sec ? m.c2[ax] : m.c1[ax];
+        const float d1 = (float)(p[ax].cv1 - r);
+        const float d = sec ? d1 + (float)(p[ax].cv1 - p[ax].cv2) * (1.0f / 3.0f) : d1;
+        d1v[ax] = d1;
+        A = p[ax].use1 ? A + al : A;
+        B = p[ax].use1 ? B + al * d : B;
+        C = p[ax].use1 ? C + al * d * d : C;
+        code[ax] = p[ax].codeA;
+    }
+    const float discA = B * B - A * C;
+    float a = A, b = B, q = discA, hs = 0.f;
+    double cmin = EIK_INF;
+    bool useoned = false;
+    if (__ballot(discA < 0.f) != 0ull) {
+        float A1 = 0.f, B1 = 0.f, C1 = -(s * s);
+        float hmin = m.dh[0];
+        int idc = 0;
+#pragma unroll
+        for (int ax = 0; ax < ND; ax++) {
+            const float ih2 = m.c1[ax];
+            const float d1 = d1v[ax];
+            A1 = p[ax].use1 ? A1 + ih2 : A1;
+            B1 = p[ax].use1 ? B1 + ih2 * d1 : B1;
+            C1 = p[ax].use1 ? C1 + ih2 * d1 * d1 : C1;
+            const bool better = (ax == 0) || p[ax].cv1 < cmin;
+            idc = better ? ax : idc;
+            cmin = better ? p[ax].cv1 : cmin;
+            hmin = better ? m.dh[ax] : hmin;
+        }
+        const float discB = B1 * B1 - A1 * C1;
+        const bool useA = !(discA < 0.f);
+        const bool useB = !useA && !(discB < 0.f);
+        a = useA ? A : A1;
+        b = useA ? B : B1;
+        q = useA ? discA : discB;
+        useoned = !useA && !useB;
+        hs = hmin * s;
+#pragma unroll
+        for (int d = 0; d < ND; d++)
+            code[d] = useA ? p[d].codeA : (useB ? p[d].codeB : (d == idc ? p[d].codeB : 0));
+    }
+    const double t = r + (double)((b + sqrtf(q)) / a);
+    return useoned ? cmin + (double)hs : t;
+}
+
 // ---------------------------------------------------------------- heap pop keys
 // The heap pops nodes in value order except after a non-causal 2nd-order update: a node whose
 // new value falls below the value of the neighbour whose acceptance triggered it is popped right
@@ -263,12 +346,15 @@ __device__ __forceinline__ float tie_below(float x) {
 __device__ __forceinline__ bool pops_after(double cur, double c) { return cur > c; }
 __device__ __forceinline__ bool pops_after(float cur, float c) { return cur >= tie_below(c); }
 
-template <int ND>
-__device__ __forceinline__ double eval_any(const double* v1, const double* v2, unsigned acc1, unsigned acc2, double s,
-                                           const Metric<double>& m, int* code) {
-    return eval64<ND>(v1, v2, acc1, acc2, s, m, code);
+template <int ND, bool MIX>
+__device__ __forceinline__ double eval_any(const double* v1, const double* v2, unsigned acc1, unsigned acc2,
+                                           Arith<double, MIX> s, const Metric<Arith<double, MIX>>& m, int* code) {
+    if constexpr (MIX)
+        return eval_mix<ND>(v1, v2, acc1, acc2, s, m, code);
+    else
+        return eval64<ND>(v1, v2, acc1, acc2, s, m, code);
 }
-template <int ND>
+template <int ND, bool MIX>
 __device__ __forceinline__ float eval_any(const float* v1, const float* v2, unsigned acc1, unsigned acc2, float s,
                                           const Metric<float>& m, int* code) {
     return eval32<ND>(v1, v2, acc1, acc2, s, m, code);
@@ -288,16 +374,17 @@ __device__ __forceinline__ float eval_any(const float* v1, const float* v2, unsi
 // + side is a candidate with one; pick_axis then takes the + side's 2nd-order choice once the + side
 // is accepted (calcttpt_2ndord! does not reset it).  A wave with such a lane, or with a lane next
 // to an init node, runs the per-candidate replay.
-template <typename T>
+template <typename T, typename A = T>
 struct AxisTerm {
     T cv1;          // chosen neighbour's value: the axis enters at this candidate (+inf: never)
-    T ca, bt, gt;   // contributions to alpha, beta and gamma (eval64's or eval32's form)
+    A ca, bt, gt;   // contributions to alpha, beta and gamma (eval64's, eval32's or eval_mix's form)
+    A d1;           // eval32 / eval_mix: cv1 - r (the all-1st-order fallback's offset)
     int codeA, codeB;
 };
 
-template <int ND, typename T>
+template <int ND, typename T, bool MIX = false>
 __device__ __forceinline__ bool axis_terms(const T* v1, const T* v2, unsigned cand, unsigned ok2,
-                                           const Metric<T>& m, AxisTerm<T>* at) {
+                                           const Metric<Arith<T, MIX>>& m, AxisTerm<T, Arith<T, MIX>>* at) {
 #pragma clang fp contract(off)
     bool odd = false;
     bool sc[ND];
@@ -322,13 +409,23 @@ __device__ __forceinline__ bool axis_terms(const T* v1, const T* v2, unsigned ca
 #pragma unroll
     for (int ax = 0; ax < ND; ax++) {
         const T cv1 = at[ax].cv1;
-        if constexpr (sizeof(T) == 8) {
+        if constexpr (sizeof(T) == 8 && !MIX) {
             const double tmpa2 = 1.0 / 3.0 * (4.0 * cv1 - w2[ax]);
             const double ca = sc[ax] ? m.c2[ax] : m.c1[ax];
             const double tv = sc[ax] ? tmpa2 : cv1;
             at[ax].ca = ca;
             at[ax].bt = -2.0 * ca * tv;
             at[ax].gt = ca * (tv * tv);
+            at[ax].d1 = 0.0;
+        } else if constexpr (MIX) {
+            // eval_mix's terms: offsets from r formed in FP64, rounded once to FP32
+            const float al = sc[ax] ? m.c2[ax] : m.c1[ax];
+            const float d1 = (float)(cv1 - r);
+            const float d = sc[ax] ? d1 + (float)(cv1 - w2[ax]) * (1.0f / 3.0f) : d1;
+            at[ax].ca = al;
+            at[ax].bt = al * d;
+            at[ax].gt = al * d * d;
+            at[ax].d1 = d1;
         } else {
             const float al = sc[ax] ? m.c2[ax] : m.c1[ax];
             const float d1 = cv1 - r;
@@ -336,6 +433,7 @@ __device__ __forceinline__ bool axis_terms(const T* v1, const T* v2, unsigned ca
             at[ax].ca = al;
             at[ax].bt = al * d;
             at[ax].gt = al * d * d;
+            at[ax].d1 = d1;
         }
     }
     return odd;
@@ -344,7 +442,7 @@ __device__ __forceinline__ bool axis_terms(const T* v1, const T* v2, unsigned ca
 // eval64 / eval32 over the axes in `on` from their precomputed contributions (same operations in
 // the same order, so the same result).
 template <int ND>
-__device__ __forceinline__ double eval_on(const AxisTerm<double>* at, unsigned on, double s,
+__device__ __forceinline__ double eval_on(const AxisTerm<double, double>* at, unsigned on, double s,
                                           const Metric<double>& m, int* code) {
 #pragma clang fp contract(off)
     double alpha = 0.0, beta = 0.0, gamma = -(s * s);
@@ -394,7 +492,7 @@ __device__ __forceinline__ double eval_on(const AxisTerm<double>* at, unsigned o
     return useoned ? oned : t;
 }
 template <int ND>
-__device__ __forceinline__ float eval_on(const AxisTerm<float>* at, unsigned on, float s, const Metric<float>& m,
+__device__ __forceinline__ float eval_on(const AxisTerm<float, float>* at, unsigned on, float s, const Metric<float>& m,
                                          int* code) {
     float r = EIK_INFF;
 #pragma unroll
@@ -447,10 +545,68 @@ __device__ __forceinline__ float eval_on(const AxisTerm<float>* at, unsigned on,
     return useoned ? oned : t;
 }
 
+// eval_mix over the axes in `on` from their precomputed contributions (the first axis in use is
+// the smallest candidate, so r is the same shift axis_terms used).
+template <int ND>
+__device__ __forceinline__ double eval_on(const AxisTerm<double, float>* at, unsigned on, float s,
+                                          const Metric<float>& m, int* code) {
+#pragma clang fp contract(off)
+    double r = EIK_INF;
+#pragma unroll
+    for (int ax = 0; ax < ND; ax++) r = (((on >> ax) & 1u) && at[ax].cv1 < r) ? at[ax].cv1 : r;
+    float A = 0.f, B = 0.f, C = -(s * s);
+#pragma unroll
+    for (int ax = 0; ax < ND; ax++) {
+        const bool u = (on >> ax) & 1u;
+        A = u ? A + at[ax].ca : A;
+        B = u ? B + at[ax].bt : B;
+        C = u ? C + at[ax].gt : C;
+        code[ax] = u ? at[ax].codeA : 0;
+    }
+    const float discA = B * B - A * C;
+    float a = A, b = B, q = discA, hs = 0.f;
+    double cmin = EIK_INF;
+    bool useoned = false;
+    if (__ballot(discA < 0.f) != 0ull) {
+        float A1 = 0.f, B1 = 0.f, C1 = -(s * s);
+        float hmin = m.dh[0];
+        int idc = 0;
+#pragma unroll
+        for (int ax = 0; ax < ND; ax++) {
+            const bool u = (on >> ax) & 1u;
+            const double cv1 = u ? at[ax].cv1 : EIK_INF;
+            const float ih2 = m.c1[ax];
+            const float d1 = at[ax].d1;
+            A1 = u ? A1 + ih2 : A1;
+            B1 = u ? B1 + ih2 * d1 : B1;
+            C1 = u ? C1 + ih2 * d1 * d1 : C1;
+            const bool better = (ax == 0) || cv1 < cmin;
+            idc = better ? ax : idc;
+            cmin = better ? cv1 : cmin;
+            hmin = better ? m.dh[ax] : hmin;
+        }
+        const float discB = B1 * B1 - A1 * C1;
+        const bool useA = !(discA < 0.f);
+        const bool useB = !useA && !(discB < 0.f);
+        a = useA ? A : A1;
+        b = useA ? B : B1;
+        q = useA ? discA : discB;
+        useoned = !useA && !useB;
+        hs = hmin * s;
+#pragma unroll
+        for (int d = 0; d < ND; d++) {
+            const int cB = ((on >> d) & 1u) ? at[d].codeB : 0;
+            code[d] = useA ? code[d] : (useB ? cB : (d == idc ? cB : 0));
+        }
+    }
+    const double t = r + (double)((b + sqrtf(q)) / a);
+    return useoned ? cmin + (double)hs : t;
+}
+
 // replay_plain's sequence, one step per axis (see above).
-template <int ND, typename T>
-__device__ __forceinline__ T replay_axes(const AxisTerm<T>* at, T s, const Metric<T>& m, int* code,
-                                         Key<T>* kout) {
+template <int ND, typename T, bool MIX = false>
+__device__ __forceinline__ T replay_axes(const AxisTerm<T, Arith<T, MIX>>* at, Arith<T, MIX> s,
+                                         const Metric<Arith<T, MIX>>& m, int* code, Key<T>* kout) {
     T cur = tinf<T>();
     T kzK = tinf<T>(), kzS = -tinf<T>();
     int kzD = 0;
@@ -485,17 +641,17 @@ __device__ __forceinline__ T replay_axes(const AxisTerm<T>* at, T s, const Metri
 // neighbour of any lane in the wave carries a cascade key the replay compares values only
 // (replay_plain); otherwise the full key comparison runs (replay_keys).  Both give the same
 // result on such neighbourhoods.
-template <int ND, typename T>
+template <int ND, typename T, bool MIX = false>
 __device__ __forceinline__ T replay_plain(const T* v1, const T* v2, unsigned cand, bool haveinit, unsigned f1,
-                                          unsigned f2, unsigned ok1, unsigned ok2, T s, const Metric<T>& m,
-                                          int* code, Key<T>* kout) {
+                                          unsigned f2, unsigned ok1, unsigned ok2, Arith<T, MIX> s,
+                                          const Metric<Arith<T, MIX>>& m, int* code, Key<T>* kout) {
     T cur = tinf<T>();
     T kzK = tinf<T>(), kzS = -tinf<T>();
     int kzD = 0;
 #pragma unroll
     for (int d = 0; d < ND; d++) code[d] = 0;
     if (haveinit) {
-        cur = eval_any<ND>(v1, v2, f1 & ok1, f2 & ok2, s, m, code);
+        cur = eval_any<ND, MIX>(v1, v2, f1 & ok1, f2 & ok2, s, m, code);
         kzK = cur;
     }
     // distinct candidate values in increasing order; after a step that ends below its trigger
@@ -516,7 +672,7 @@ __device__ __forceinline__ T replay_plain(const T* v1, const T* v2, unsigned can
             const bool a2 = (bool)((ok2 >> k) & 1u) & ((bool)((f2 >> k) & 1u) | (v2[k] <= c));
             acc2 |= (unsigned)a2 << k;
         }
-        cur = eval_any<ND>(v1, v2, acc1, acc2, s, m, code);
+        cur = eval_any<ND, MIX>(v1, v2, acc1, acc2, s, m, code);
         const bool ord = pops_after(cur, c);
         kzK = ord ? cur : c;
         kzS = ord ? -tinf<T>() : cur;
@@ -530,10 +686,10 @@ __device__ __forceinline__ T replay_plain(const T* v1, const T* v2, unsigned can
 // node; it is called only on the rare cascade path, so callers can read keys from memory there
 // instead of holding them in registers.  d1/d2: slots whose adjacent / 2-away node has a cascade
 // key (D != 0); dcap: adjacent nodes at the cascade depth cap.
-template <int ND, typename T, class KeyAt>
+template <int ND, typename T, bool MIX, class KeyAt>
 __device__ __forceinline__ T replay(const T* v1, const T* v2, KeyAt keys, unsigned d1, unsigned d2, unsigned dcap,
-                                    unsigned f1, unsigned f2, unsigned ok1, unsigned ok2, T s, const Metric<T>& m,
-                                    int* code, Key<T>* kout) {
+                                    unsigned f1, unsigned f2, unsigned ok1, unsigned ok2, Arith<T, MIX> s,
+                                    const Metric<Arith<T, MIX>>& m, int* code, Key<T>* kout) {
     // candidates: adjacent nodes inside the grid, not init, reached, cascade depth below the cap
     unsigned cand = 0u;
     bool haveinit = false;
@@ -549,18 +705,18 @@ __device__ __forceinline__ T replay(const T* v1, const T* v2, KeyAt keys, unsign
     }
     if (__ballot(keyed) == 0ull) {
         if (__ballot(haveinit) == 0ull) {
-            AxisTerm<T> at[ND];
-            const bool odd = axis_terms<ND, T>(v1, v2, cand, ok2, m, at);
-            if (__ballot(odd) == 0ull) return replay_axes<ND, T>(at, s, m, code, kout);
+            AxisTerm<T, Arith<T, MIX>> at[ND];
+            const bool odd = axis_terms<ND, T, MIX>(v1, v2, cand, ok2, m, at);
+            if (__ballot(odd) == 0ull) return replay_axes<ND, T, MIX>(at, s, m, code, kout);
         }
-        return replay_plain<ND, T>(v1, v2, cand, haveinit, f1, f2, ok1, ok2, s, m, code, kout);
+        return replay_plain<ND, T, MIX>(v1, v2, cand, haveinit, f1, f2, ok1, ok2, s, m, code, kout);
     }
     T cur = tinf<T>();
     Key<T> kz{tinf<T>(), -tinf<T>(), 0};
 #pragma unroll
     for (int d = 0; d < ND; d++) code[d] = 0;
     if (haveinit) {
-        cur = eval_any<ND>(v1, v2, f1 & ok1, f2 & ok2, s, m, code);
+        cur = eval_any<ND, MIX>(v1, v2, f1 & ok1, f2 & ok2, s, m, code);
         kz = Key<T>{cur, -tinf<T>(), 0};
     }
     // distinct candidate keys in increasing order (equal keys pop as one batch); each step scans
@@ -591,7 +747,7 @@ __device__ __forceinline__ T replay(const T* v1, const T* v2, KeyAt keys, unsign
             acc1 |= (unsigned)a1 << k;
             acc2 |= (unsigned)a2 << k;
         }
-        cur = eval_any<ND>(v1, v2, acc1, acc2, s, m, code);
+        cur = eval_any<ND, MIX>(v1, v2, acc1, acc2, s, m, code);
         const bool ord = pops_after(cur, c.K);
         kz.K = ord ? cur : c.K;
         kz.S = ord ? -tinf<T>() : (cur > c.S ? cur : c.S);
@@ -602,10 +758,10 @@ __device__ __forceinline__ T replay(const T* v1, const T* v2, KeyAt keys, unsign
 }
 
 // Same with the keys held in arrays (k1: adjacent, k2: 2-away).
-template <int ND, typename T>
+template <int ND, typename T, bool MIX = false>
 __device__ __forceinline__ T replay(const T* v1, const T* v2, const Key<T>* k1, const Key<T>* k2, unsigned f1,
-                                    unsigned f2, unsigned ok1, unsigned ok2, T s, const Metric<T>& m, int* code,
-                                    Key<T>* kout) {
+                                    unsigned f2, unsigned ok1, unsigned ok2, Arith<T, MIX> s,
+                                    const Metric<Arith<T, MIX>>& m, int* code, Key<T>* kout) {
     unsigned d1 = 0u, d2 = 0u, dcap = 0u;
 #pragma unroll
     for (int k = 0; k < 2 * ND; k++) {
@@ -614,7 +770,7 @@ __device__ __forceinline__ T replay(const T* v1, const T* v2, const Key<T>* k1, 
         dcap |= (unsigned)(k1[k].D >= kMaxCascade) << k;
     }
     auto keys = [&](int k, bool two) { return two ? k2[k] : k1[k]; };
-    return replay<ND, T>(v1, v2, keys, d1, d2, dcap, f1, f2, ok1, ok2, s, m, code, kout);
+    return replay<ND, T, MIX>(v1, v2, keys, d1, d2, dcap, f1, f2, ok1, ok2, s, m, code, kout);
 }
 
 }  // namespace eik

@@ -19,7 +19,7 @@ import threading
 import numpy as np
 
 from . import _lib
-from .grids import ExtraParams, PRECISION_FP64, PRECISION_MIXED
+from .grids import PRECISIONS, ExtraParams
 
 _WHICH = {"gradvel": 1, "gradsrcloc": 2, "gradvelandsrcloc": 3}
 
@@ -87,7 +87,7 @@ def _params_struct(extraparams) -> _lib.EikParams:
     p.noderadius = int(ep.grdrefpars.noderadius)
     p.radiussmoothgradsrc = int(ep.radiussmoothgradsrc)
     p.smoothgradkern = int(ep.smoothgradkern)
-    p.precision = PRECISION_MIXED if ep.precision == "mixed" else PRECISION_FP64
+    p.precision = PRECISIONS[ep.precision]
     p.fast = int(bool(ep.fast))
     return p
 

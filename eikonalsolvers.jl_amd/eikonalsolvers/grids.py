@@ -203,7 +203,9 @@ class GridRefinementPars:
 
 
 PRECISION_FP64 = 0
-PRECISION_MIXED = 1   # FP32 stencil; FP64 near-source init, adjoint and reductions
+PRECISION_MIXED = 1   # FP32 stencil increments on FP64 traveltimes; FP64 near-source init, adjoint, reductions
+PRECISION_FP32 = 2    # FP32 traveltimes and stencil; FP64 near-source init, adjoint and reductions
+PRECISIONS = {"fp64": PRECISION_FP64, "mixed": PRECISION_MIXED, "fp32": PRECISION_FP32}
 
 
 @dataclass(frozen=True)
@@ -211,7 +213,10 @@ class ExtraParams:
     """Extra parameters (eikstructs.jl:246-282) plus the device-side ``precision`` knob.
 
     ``parallelkind`` is accepted and validated like the reference but every source runs batched on
-    the one GPU.  ``precision`` is ``"fp64"`` (default, the reference's arithmetic) or ``"mixed"``.
+    the one GPU.  ``precision`` is ``"fp64"`` (default, the reference's arithmetic), ``"mixed"``
+    (FP64 traveltimes, the stencil's quadratic increment in FP32; the heap's order is decided in
+    FP64) or ``"fp32"`` (FP32 traveltimes and stencil: a throughput mode whose results can leave the
+    stated tolerances downstream of heap near-ties within FP32 resolution).
     ``fast`` (FP64): False (default) reproduces the heap FMM bit for bit; True lets changes below
     1e-12 relative stop propagating (~1/3 faster; a suppressed change can flip a decision at an
     exact heap tie, measured up to 5e-5 relative on a grid, inside the stated tolerances).
@@ -235,5 +240,5 @@ class ExtraParams:
             raise ValueError("ExtraParams: smoothgradkern must be an odd number")
         if pk not in ("serial", "sharedmem", "distribmem"):
             raise ValueError("ExtraParams(): 'parallelkind' must be one of :serial, :sharedmem or :distribmem")
-        if self.precision not in ("fp64", "mixed"):
-            raise ValueError("ExtraParams(): 'precision' must be 'fp64' or 'mixed'")
+        if self.precision not in PRECISIONS:
+            raise ValueError("ExtraParams(): 'precision' must be 'fp64', 'mixed' or 'fp32'")

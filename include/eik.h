@@ -42,7 +42,11 @@ extern "C" {
 
 /* arithmetic of the traveltime stencil */
 #define EIK_PREC_FP64 0   /* FP64 everywhere: the reference's arithmetic */
-#define EIK_PREC_MIXED 1  /* FP32 coarse stencil; FP64 near-source solve, adjoint and reductions */
+#define EIK_PREC_MIXED 1  /* FP32 stencil increments on FP64 traveltimes (the heap order is decided in
+                             FP64); FP64 near-source solve, adjoint and reductions */
+#define EIK_PREC_FP32 2   /* FP32 traveltimes and stencil (throughput mode: heap near-ties within FP32
+                             resolution can change a cone downstream); FP64 near-source solve, adjoint
+                             and reductions */
 
 /* whichgrad (src/EikSolv/eikmaingrad.jl:40) */
 #define EIK_GRADVEL 1

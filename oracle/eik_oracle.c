@@ -239,10 +239,122 @@ static double stencil(const og_grid* g, const double* vel, const double* tt, con
     return fmax(t1, t2);
 }
 
+/* The mixed precision's stencil (the engine's EIK_PREC_MIXED; no reference counterpart): the same
+ * neighbour choice as stencil() in FP64, the quadratic solved in FP32 for the increment over the
+ * smallest chosen value r, with the offsets c_a - r formed in FP64 and rounded once to FP32, the
+ * result r + increment formed in FP64.  1st-order fallback in FP32; 1-D fallback
+ * cmin + (float)h * s.  Slowness and metric are the FP64 values rounded to FP32; the metric
+ * coefficients are 1/h^2 and 2.25/h^2 in FP32. */
+static double stencil_mix(const og_grid* g, const double* vel, const double* tt, const uint8_t* st,
+                          const int64_t* ijk, int32_t* code) {
+    const int nd = g->ndim;
+    double dh[3];
+    node_spacing(g, ijk, dh);
+    const float s = (float)(1.0 / vel[lin(g, ijk)]);
+    int use1v[3] = {0, 0, 0}, use2v[3] = {0, 0, 0};
+    double cv1v[3] = {HUGEV, HUGEV, HUGEV}, cv2v[3] = {HUGEV, HUGEV, HUGEV};
+    int32_t codeA[3] = {0, 0, 0}, codeB[3] = {0, 0, 0};
+    for (int ax = 0; ax < nd; ax++) {
+        for (int l = 0; l < 2; l++) {
+            const int sh = (l == 0) ? 1 : -1;
+            int64_t nb[3] = {ijk[0], ijk[1], ijk[2]};
+            nb[ax] += sh;
+            int onb1, onb2;
+            isonbord(g, nb, &onb1, &onb2);
+            if (!onb1 && st[lin(g, nb)] == 2) {
+                double tv1 = tt[lin(g, nb)];
+                if (tv1 < cv1v[ax]) {
+                    cv1v[ax] = tv1;
+                    use1v[ax] = 1;
+                    codeA[ax] = -sh;
+                    codeB[ax] = -sh;
+                    int64_t nb2[3] = {ijk[0], ijk[1], ijk[2]};
+                    nb2[ax] += 2 * sh;
+                    if (!onb2 && st[lin(g, nb2)] == 2) {
+                        double tv2 = tt[lin(g, nb2)];
+                        if (tv2 <= cv1v[ax]) {
+                            cv2v[ax] = tv2;
+                            use2v[ax] = 1;
+                            codeA[ax] = -2 * sh;
+                        } else {
+                            cv2v[ax] = HUGEV;
+                            use2v[ax] = 0;
+                            codeA[ax] = -sh;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    float c1[3], c2[3], hf[3];
+    for (int ax = 0; ax < nd; ax++) {
+        hf[ax] = (float)dh[ax];
+        c1[ax] = 1.0f / (hf[ax] * hf[ax]);
+        c2[ax] = 2.25f * c1[ax];
+    }
+    double r = HUGEV;
+    for (int ax = 0; ax < nd; ax++)
+        if (use1v[ax] && cv1v[ax] < r) r = cv1v[ax];
+    float A = 0.f, B = 0.f, C = -(s * s), d1v[3] = {0.f, 0.f, 0.f};
+    for (int ax = 0; ax < nd; ax++) {
+        if (!use1v[ax]) continue;
+        const int sec = use2v[ax];
+        const float al = sec ? c2[ax] : c1[ax];
+        const float d1 = (float)(cv1v[ax] - r);
+        const float d = sec ? d1 + (float)(cv1v[ax] - cv2v[ax]) * (1.0f / 3.0f) : d1;
+        d1v[ax] = d1;
+        A = A + al;
+        B = B + al * d;
+        C = C + al * d * d;
+    }
+    for (int d = 0; d < nd; d++) code[d] = codeA[d];
+    float disc = B * B - A * C;
+    if (!(disc < 0.f)) return r + (double)((B + sqrtf(disc)) / A);
+    float A1 = 0.f, B1 = 0.f, C1 = -(s * s);
+    for (int ax = 0; ax < nd; ax++) {
+        if (!use1v[ax]) continue;
+        A1 = A1 + c1[ax];
+        B1 = B1 + c1[ax] * d1v[ax];
+        C1 = C1 + c1[ax] * d1v[ax] * d1v[ax];
+    }
+    for (int d = 0; d < nd; d++) code[d] = codeB[d];
+    disc = B1 * B1 - A1 * C1;
+    if (!(disc < 0.f)) return r + (double)((B1 + sqrtf(disc)) / A1);
+    warn_hit();
+    int idc = 0; /* first axis with the smallest adjacent value */
+    for (int ax = 1; ax < nd; ax++)
+        if (cv1v[ax] < cv1v[idc]) idc = ax;
+    for (int d = 0; d < nd; d++) code[d] = d == idc ? codeB[d] : 0;
+    return cv1v[idc] + (double)(hf[idc] * s);
+}
+
+/* The mixed precision's arithmetic for node ijk: FP64 inside the near-source region xb (the source
+ * box +- 4 nodes; NULL: none), the mixed stencil elsewhere. */
+typedef struct {
+    int64_t lo[3], hi[3];
+} xbox_t;
+static inline int in_xbox(const xbox_t* xb, const int64_t* ijk, int nd) {
+    if (!xb) return 0;
+    for (int d = 0; d < nd; d++)
+        if (ijk[d] < xb->lo[d] || ijk[d] > xb->hi[d]) return 0;
+    return 1;
+}
+static inline double stencil_any(const og_grid* g, const double* vel, const double* tt, const uint8_t* st,
+                                 const int64_t* ijk, int32_t* code, int mix, const xbox_t* xb) {
+    return (mix && !in_xbox(xb, ijk, g->ndim)) ? stencil_mix(g, vel, tt, st, ijk, code)
+                                                : stencil(g, vel, tt, st, ijk, code);
+}
+
 double og_stencil(const og_grid* g, const double* vel, const double* tt, const uint8_t* status,
                   const int64_t* ijk, int32_t* codes) {
     int64_t p[3] = {ijk[0], ijk[1], g->ndim == 3 ? ijk[2] : 0};
     return stencil(g, vel, tt, status, p, codes);
+}
+
+double og_stencil_mixed(const og_grid* g, const double* vel, const double* tt, const uint8_t* status,
+                        const int64_t* ijk, int32_t* codes) {
+    int64_t p[3] = {ijk[0], ijk[1], g->ndim == 3 ? ijk[2] : 0};
+    return stencil_mix(g, vel, tt, status, p, codes);
 }
 
 /* ------------------------------------------------------------------ binary min-heap */
@@ -670,7 +782,7 @@ static inline int is_coarse_node(const refvars_t* R, const int64_t* fijk, int nd
 
 /* ttFMM_core! (eikmainfwd.jl:427-823) */
 static int fmm_core(const og_grid* g, const double* vel, fmm_t* F, const int64_t* init, int64_t ninit,
-                    adj_t* A, fine_ctx_t* fc) {
+                    adj_t* A, fine_ctx_t* fc, int mix, const xbox_t* xb) {
     const int nd = g->ndim;
     const int nnb = 2 * nd;
     const int64_t N = npts(g);
@@ -711,7 +823,7 @@ static int fmm_core(const og_grid* g, const double* vel, fmm_t* F, const int64_t
             if (oob) continue;
             int64_t lc = lin(g, c);
             if (F->st[lc] == 0) {
-                double t = stencil(g, vel, F->tt, F->st, c, idD);
+                double t = stencil_any(g, vel, F->tt, F->st, c, idD, mix, xb);
                 hp_insert(&F->hp, t, lc);
                 F->st[lc] = 1;
                 if (A)
@@ -804,13 +916,13 @@ static int fmm_core(const og_grid* g, const double* vel, fmm_t* F, const int64_t
                 if (oob) continue;
                 int64_t lc = lin(g, c);
                 if (F->st[lc] == 0) {
-                    double t = stencil(g, vel, F->tt, F->st, c, idD);
+                    double t = stencil_any(g, vel, F->tt, F->st, c, idD, mix, xb);
                     hp_insert(&F->hp, t, lc);
                     F->st[lc] = 1;
                     if (A)
                         for (int d = 0; d < nd; d++) A->code[lc * nd + d] = (int8_t)idD[d];
                 } else if (F->st[lc] == 1) {
-                    double t = stencil(g, vel, F->tt, F->st, c, idD);
+                    double t = stencil_any(g, vel, F->tt, F->st, c, idD, mix, xb);
                     hp_update(&F->hp, t, lc);
                     if (A)
                         for (int d = 0; d < nd; d++) A->code[lc * nd + d] = (int8_t)idD[d];
@@ -872,14 +984,30 @@ static int ttfmm_hiord(const og_grid* g, const double* vel, const double* src, c
         fc.Fc = F;
         fc.Ac = A;
         fc.cinit = cinit;
-        if (fmm_core(&fg->R.gf, fg->R.velfine, &fg->Ff, finit, fg->sbf.ncorn, Af, &fc)) return -1;
+        /* the refined box is solved in FP64 in both precisions */
+        if (fmm_core(&fg->R.gf, fg->R.velfine, &fg->Ff, finit, fg->sbf.ncorn, Af, &fc, 0, NULL)) return -1;
         *ncinit = fc.ncinit;
     } else {
         sourceboxloctt(g, vel, src, F, sb);
         for (int c = 0; c < sb->ncorn; c++) cinit[c] = lin(g, sb->ijk[c]);
         *ncinit = sb->ncorn;
     }
-    return fmm_core(g, vel, F, cinit, *ncinit, A, NULL);
+    /* mixed precision without refinement: FP64 stencil within +-4 nodes of the source box */
+    xbox_t xb;
+    const int mix = p->precision == OG_PREC_MIXED;
+    const int usebox = mix && !p->refine;
+    if (usebox) {
+        for (int d = 0; d < 3; d++) {
+            xb.lo[d] = INT64_MAX;
+            xb.hi[d] = INT64_MIN;
+        }
+        for (int c = 0; c < sb->ncorn; c++)
+            for (int d = 0; d < g->ndim; d++) {
+                if (sb->ijk[c][d] - OG_MIXED_FP64_RADIUS < xb.lo[d]) xb.lo[d] = sb->ijk[c][d] - OG_MIXED_FP64_RADIUS;
+                if (sb->ijk[c][d] + OG_MIXED_FP64_RADIUS > xb.hi[d]) xb.hi[d] = sb->ijk[c][d] + OG_MIXED_FP64_RADIUS;
+            }
+    }
+    return fmm_core(g, vel, F, cinit, *ncinit, A, NULL, mix, usebox ? &xb : NULL);
 }
 
 /* ------------------------------------------------------------------ receivers */
@@ -1010,6 +1138,17 @@ static int64_t distribsrcs(int64_t nsrc, int64_t nw, int64_t* lo, int64_t* hi) {
     return nch;
 }
 
+/* The parameters a call runs with: as given, except that Grid2DSphere keeps the FP64 stencil under
+ * the mixed precision (the engine's rule: its adjoint amplifies rounding-level changes ~1e4-fold). */
+static og_params effective_params(const og_grid* g, const og_params* p) {
+    og_params e = *p;
+    if (e.precision == OG_PREC_MIXED && g->kind == OG_SPH2D) e.precision = OG_PREC_FP64;
+    return e;
+}
+static int sph3d_refine_unsupported(const og_grid* g, const og_params* p) {
+    return g->kind == OG_SPH3D && p->refine;
+}
+
 /* ------------------------------------------------------------------ forward API */
 
 /* ttforwsomesrc (eikmainfwd.jl:152-199) */
@@ -1039,10 +1178,12 @@ static int fwd_chunk(const og_grid* g, const double* vel, int64_t s0, int64_t s1
 }
 
 int og_traveltime(const og_grid* g, const double* vel, int64_t nsrc, const double* coordsrc,
-                  const int64_t* rec_off, const double* coordrec, const og_params* p, double* ttpicks,
+                  const int64_t* rec_off, const double* coordrec, const og_params* p_in, double* ttpicks,
                   double* ttgrid) {
+    const og_params pe = effective_params(g, p_in);
+    const og_params* p = &pe;
     if (check_inputs(g, vel, nsrc, coordsrc, rec_off, coordrec, p)) return -1;
-    if (g->kind == OG_SPH3D && p->refine)
+    if (sph3d_refine_unsupported(g, p))
         return fail("createfinegrid(): spherical coordinates in 3D still work in progress...");
     int64_t nt = p->nthreads > 1 ? p->nthreads : 1;
     int64_t* lo = (int64_t*)malloc((size_t)(nt + 1) * sizeof(int64_t));
@@ -1617,7 +1758,11 @@ int og_smoothgradient(const og_grid* g, int32_t l, const double* in, double* out
 
 int og_gradient(const og_grid* g, const double* vel, int64_t nsrc, const double* coordsrc,
                 const int64_t* rec_off, const double* coordrec, const double* pickobs, const double* stdobs,
-                int whichgrad, const og_params* p, double* gradvel, double* gradsrc, double* misf) {
+                int whichgrad, const og_params* p_in, double* gradvel, double* gradsrc, double* misf) {
+    const og_params pe = effective_params(g, p_in);
+    const og_params* p = &pe;
+    if (sph3d_refine_unsupported(g, p))
+        return fail("createfinegrid(): spherical coordinates in 3D still work in progress...");
     if (whichgrad < 1 || whichgrad > 3)
         return fail("AssertionError: Symbol 'whichgrad' not in (:gradvel, :gradsrcloc, :gradvelandsrcloc)");
     if (check_inputs(g, vel, nsrc, coordsrc, rec_off, coordrec, p)) return -1;
@@ -1669,8 +1814,10 @@ int og_gradient(const og_grid* g, const double* vel, int64_t nsrc, const double*
 
 /* ------------------------------------------------------------------ test hooks */
 
-int64_t og_fmm_record(const og_grid* g, const double* vel, const double* src, const og_params* p, double* tt,
+int64_t og_fmm_record(const og_grid* g, const double* vel, const double* src, const og_params* p_in, double* tt,
                       int64_t* order, int8_t* codes, int64_t* nsrcpts) {
+    const og_params pe = effective_params(g, p_in);
+    const og_params* p = &pe;
     const int64_t N = npts(g);
     fmm_t F;
     adj_t A;

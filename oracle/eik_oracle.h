@@ -41,7 +41,15 @@ typedef struct {
     int32_t radiussmooth;  /* radiussmoothgradsrc (default 3) */
     int32_t smoothkern;    /* smoothgradkern (default 0, odd) */
     int32_t nthreads;      /* 1 = serial (:serial); >1 = source chunks as distribsrcs (:sharedmem) */
+    int32_t precision;     /* OG_PREC_FP64 (the reference) or OG_PREC_MIXED (the engine's mixed mode) */
 } og_params;
+
+/* OG_PREC_MIXED restates the engine's mixed precision (no reference counterpart): the coarse stencil
+ * keeps FP64 values and comparisons and solves the quadratic's increment in FP32; near the source it
+ * stays FP64 (the refined box, or without refinement the nodes within OG_MIXED_FP64_RADIUS of the
+ * source box).  The problem (init set, adjoint) is the one the parameters define, as in FP64. */
+enum { OG_PREC_FP64 = 0, OG_PREC_MIXED = 1 };
+#define OG_MIXED_FP64_RADIUS 4
 
 /* whichgrad (eikmaingrad.jl:40) */
 enum { OG_GRADVEL = 1, OG_GRADSRCLOC = 2, OG_GRADBOTH = 3 };
@@ -81,6 +89,10 @@ int og_fmm_bruteforce(const og_grid* g, const double* vel, const double* src, do
  * eikforward3D.jl:151-532).  ijk 0-based; status: 2 = accepted.  Writes codes[ndim]. */
 double og_stencil(const og_grid* g, const double* vel, const double* tt, const uint8_t* status,
                   const int64_t* ijk, int32_t* codes);
+
+/* The same with the mixed precision's arithmetic (OG_PREC_MIXED). */
+double og_stencil_mixed(const og_grid* g, const double* vel, const double* tt, const uint8_t* status,
+                        const int64_t* ijk, int32_t* codes);
 
 /* Gaussian smoothing of a gradient (utils.jl:22-96), out-of-place. */
 int og_smoothgradient(const og_grid* g, int32_t l, const double* in, double* out);

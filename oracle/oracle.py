@@ -28,7 +28,7 @@ class OgGrid(ctypes.Structure):
 class OgParams(ctypes.Structure):
     _fields_ = [("refine", ctypes.c_int32), ("df", ctypes.c_int32), ("noderadius", ctypes.c_int32),
                 ("radiussmooth", ctypes.c_int32), ("smoothkern", ctypes.c_int32),
-                ("nthreads", ctypes.c_int32)]
+                ("nthreads", ctypes.c_int32), ("precision", ctypes.c_int32)]
 
 
 def build() -> str:
@@ -83,7 +83,13 @@ def grid_struct(grd) -> OgGrid:
     return g
 
 
-def params_struct(extraparams=None, nthreads: int = 1) -> OgParams:
+PRECISIONS = {"fp64": 0, "mixed": 1}
+
+
+def params_struct(extraparams=None, nthreads: int = 1, precision: str = "fp64") -> OgParams:
+    """og_params from ExtraParams.  The oracle's arithmetic is chosen by `precision` alone ("fp64",
+    the reference, by default; "mixed", the engine's mixed precision restated), never by
+    ExtraParams.precision, so a test states which oracle it compares against."""
     p = OgParams()
     if extraparams is None:
         p.refine, p.df, p.noderadius, p.radiussmooth, p.smoothkern = 1, 5, 3, 3, 0
@@ -94,6 +100,7 @@ def params_struct(extraparams=None, nthreads: int = 1) -> OgParams:
         p.radiussmooth = int(extraparams.radiussmoothgradsrc)
         p.smoothkern = int(extraparams.smoothgradkern)
     p.nthreads = int(nthreads)
+    p.precision = PRECISIONS[precision]
     return p
 
 
@@ -119,10 +126,10 @@ def _err():
     raise OracleError(lib().og_last_error().decode())
 
 
-def traveltime(vel, grd, coordsrc, coordrec, returntt=False, extraparams=None, nthreads=1):
+def traveltime(vel, grd, coordsrc, coordrec, returntt=False, extraparams=None, nthreads=1, precision="fp64"):
     L = lib()
     g = grid_struct(grd)
-    p = params_struct(extraparams, nthreads)
+    p = params_struct(extraparams, nthreads, precision)
     v = _vel(vel, grd)
     cs, off, packed = _pack(coordsrc, coordrec, grd.ndim)
     nsrc = cs.shape[0]
@@ -140,11 +147,12 @@ def traveltime(vel, grd, coordsrc, coordrec, returntt=False, extraparams=None, n
     return out
 
 
-def gradient(vel, grd, coordsrc, coordrec, pickobs, stdobs, whichgrad, extraparams=None, nthreads=1):
+def gradient(vel, grd, coordsrc, coordrec, pickobs, stdobs, whichgrad, extraparams=None, nthreads=1,
+             precision="fp64"):
     L = lib()
     wg = {"gradvel": 1, "gradsrcloc": 2, "gradvelandsrcloc": 3}[str(whichgrad).lstrip(":")]
     g = grid_struct(grd)
-    p = params_struct(extraparams, nthreads)
+    p = params_struct(extraparams, nthreads, precision)
     v = _vel(vel, grd)
     cs, off, packed = _pack(coordsrc, coordrec, grd.ndim)
     nsrc = cs.shape[0]
@@ -166,10 +174,10 @@ def gradient(vel, grd, coordsrc, coordrec, pickobs, stdobs, whichgrad, extrapara
     return gvel, gsrc, float(misf[0])
 
 
-def misfit(vel, grd, coordsrc, coordrec, pickobs, stdobs, extraparams=None, nthreads=1):
+def misfit(vel, grd, coordsrc, coordrec, pickobs, stdobs, extraparams=None, nthreads=1, precision="fp64"):
     L = lib()
     g = grid_struct(grd)
-    p = params_struct(extraparams, nthreads)
+    p = params_struct(extraparams, nthreads, precision)
     v = _vel(vel, grd)
     cs, off, packed = _pack(coordsrc, coordrec, grd.ndim)
     po = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.float64).ravel() for x in pickobs]))
@@ -182,11 +190,11 @@ def misfit(vel, grd, coordsrc, coordrec, pickobs, stdobs, extraparams=None, nthr
     return float(m[0])
 
 
-def fmm_record(vel, grd, src, extraparams=None):
+def fmm_record(vel, grd, src, extraparams=None, precision="fp64"):
     """Single-source FMM: (tt grid, acceptance order, codes (N, ndim), n source points, n accepted)."""
     L = lib()
     g = grid_struct(grd)
-    p = params_struct(extraparams)
+    p = params_struct(extraparams, 1, precision)
     v = _vel(vel, grd)
     N = grd.npts
     tt = np.zeros(N)

@@ -1,7 +1,8 @@
 // Host check of the stencil header: the axis replay (axis_terms + replay_axes) returns the same
 // value, pop key and derivative codes, bit for bit, as the per-candidate replay (replay_plain) on
 // random neighbourhoods: discrete values with many ties, smooth fronts, unreached and off-grid
-// neighbours, fixed 2-away nodes, Cartesian and per-axis spacings, FP64 and FP32.  Neighbourhoods
+// neighbours, fixed 2-away nodes, Cartesian and per-axis spacings, FP64, FP32 and the mixed
+// arithmetic (FP64 values, FP32 increment).  Neighbourhoods
 // with an "odd" axis (which the device code routes to replay_plain) are counted, not compared,
 // unless the first argument is "all" (then they must differ somewhere: the check is sensitive).
 #include "eik_stencil.h"
@@ -13,8 +14,9 @@
 using namespace eik;
 template <typename T> static bool same(T a, T b) { return memcmp(&a, &b, sizeof(T)) == 0; }
 static bool g_all = false;
-template <int ND, typename T>
+template <int ND, typename T, bool MIX = false>
 static long run(std::mt19937_64& rng, long n, long& nodd, long& nused, bool sph) {
+    using A = Arith<T, MIX>;
     std::uniform_real_distribution<double> U(0.0, 1.0);
     long bad = 0;
     for (long it = 0; it < n; it++) {
@@ -46,19 +48,19 @@ static long run(std::mt19937_64& rng, long n, long& nodd, long& nused, bool sph)
             if (!in2) v2[k] = (T)INFINITY;
             if (in1 && v1[k] < (T)INFINITY) cand |= 1u << k;
         }
-        Metric<T> m;
-        for (int ax = 0; ax < 3; ax++) m.dh[ax] = sph ? (T)(0.5 + U(rng)) : (T)1.0;
-        if (!sph) { T h = (T)(0.25 + U(rng)); for (int ax = 0; ax < 3; ax++) m.dh[ax] = h; }
+        Metric<A> m;
+        for (int ax = 0; ax < 3; ax++) m.dh[ax] = sph ? (A)(0.5 + U(rng)) : (A)1.0;
+        if (!sph) { A h = (A)(0.25 + U(rng)); for (int ax = 0; ax < 3; ax++) m.dh[ax] = h; }
         metric_coefs(m, ND);
-        const T s = (T)(0.15 + 0.6 * U(rng));
+        const A s = (A)(0.15 + 0.6 * U(rng));
         int c0[3] = {9, 9, 9}, c1[3] = {9, 9, 9};
         Key<T> k0, k1;
-        const T r0 = replay_plain<ND, T>(v1, v2, cand, false, 0u, f2, ok1, ok2, s, m, c0, &k0);
-        AxisTerm<T> at[ND];
-        const bool odd = axis_terms<ND, T>(v1, v2, cand, ok2, m, at);
+        const T r0 = replay_plain<ND, T, MIX>(v1, v2, cand, false, 0u, f2, ok1, ok2, s, m, c0, &k0);
+        AxisTerm<T, A> at[ND];
+        const bool odd = axis_terms<ND, T, MIX>(v1, v2, cand, ok2, m, at);
         if (odd) { nodd++; if (!g_all) continue; }
         nused++;
-        const T r1 = replay_axes<ND, T>(at, s, m, c1, &k1);
+        const T r1 = replay_axes<ND, T, MIX>(at, s, m, c1, &k1);
         bool ok = same(r0, r1) && same(k0.K, k1.K) && same(k0.S, k1.S) && k0.D == k1.D;
         for (int d = 0; d < ND; d++) ok = ok && c0[d] == c1[d];
         if (!ok) {
@@ -84,6 +86,8 @@ int main(int argc, char** argv) {
         o = u = 0; tot += run<3, double>(rng, n, o, u, sph); printf("3D f64 sph=%d used %ld odd %ld\n", sph, u, o);
         o = u = 0; tot += run<2, float>(rng, n, o, u, sph); printf("2D f32 sph=%d used %ld odd %ld\n", sph, u, o);
         o = u = 0; tot += run<3, float>(rng, n, o, u, sph); printf("3D f32 sph=%d used %ld odd %ld\n", sph, u, o);
+        o = u = 0; tot += run<2, double, true>(rng, n, o, u, sph); printf("2D mix sph=%d used %ld odd %ld\n", sph, u, o);
+        o = u = 0; tot += run<3, double, true>(rng, n, o, u, sph); printf("3D mix sph=%d used %ld odd %ld\n", sph, u, o);
     }
     printf("mismatches: %ld\n", tot);
     return g_all ? (tot == 0) : (tot != 0);

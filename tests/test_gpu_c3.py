@@ -3,13 +3,15 @@ smooth random field, 256 sources with continuous positions, 128 surface receiver
 sources run in shared launches.
 
 * FP64 :gradvelandsrcloc against the oracle (1e-7 relative L2, rounding level);
-* mixed precision (FP32 forward, FP64 adjoint) against the FP64 oracle within the stated 1e-3
-  relative L2 (:gradsrcloc passes; :gradvel measures 1.245e-3 and is marked xfail, reason below);
+* mixed precision (FP32 stencil increments on FP64 traveltimes, FP64 adjoint) against the FP64
+  oracle within the stated 1e-3 relative L2 (the oracle's restatement of it measures :gradvel
+  3.6e-5, :gradsrcloc 6.8e-6);
 * the FP64 and the mixed adjoint against central finite differences of the FP64 misfit on 20 random cells
-  (seed 30305, dh = 1e-3 km/s) and on the coordinates of 16 sources (dh = 1e-4 km), the steps the
+  (seed 30305, dh = 1e-3 km/s) and on all 512 source coordinates (dh = 1e-4 km), the steps the
   reference's own tests use (test_cartesian_2D.jl:226,323).  Finite differences of a heap FMM are
   noisy (acceptance order can change under the perturbation), so the comparison is in relative L2
-  over the sampled entries.
+  over the sampled entries; a source coordinate whose differences at dh and dh/2 disagree sits on a
+  jump of the misfit and is counted, not compared (28 of 512 here).
 
 Observed picks: FP64 forward on the model scaled by 1 + 0.02 * smooth field (seed 30306), plus
 N(0, 0.01^2) noise (seed 30307); std 0.01.
@@ -76,15 +78,17 @@ def test_c3_mixed_gradsrc_within_tolerance(c3, oracle_grad):
     assert mg == pytest.approx(mo, rel=1e-5)
 
 
-@pytest.mark.xfail(reason="mixed-precision :gradvel at C3 measures 1.245e-3 relative L2 against FP64 (tolerance 1e-3): "
-                          "the 8 sharp layer interfaces put many heap acceptance-order near-ties within FP32 "
-                          "resolution, where the FP32 forward takes the other branch and the adjoint inherits its "
-                          "derivative codes (see test_gpu_forward.py)", strict=False)
 def test_c3_mixed_gradvel_within_tolerance(c3, oracle_grad):
     grd, vel, src, recs, obs, std, model = c3
-    gv, gs, mg = model.gradient(src, recs, obs, std, "gradvel", extraparams=ExtraParams(precision="mixed"))
+    gv, mg = model.gradient(src, recs, obs, std, "gradvel", extraparams=ExtraParams(precision="mixed"))
     ov, os_, mo = oracle_grad
+    print(f"C3 mixed :gradvel rel L2 {_rel_l2(gv, ov):.3e}, misfit rel {abs(mg - mo) / mo:.3e}", flush=True)
+    assert np.all(np.isfinite(gv))
     assert _rel_l2(gv, ov) <= 1e-3
+
+
+FD_TOL_VEL = 2e-3  # measured 4.4e-4 (FP64 adjoint, 20 cells)
+FD_TOL_SRC = 5e-3  # measured 1.6e-3 over the 484 resolved source coordinates (oracle, FP64)
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
@@ -113,21 +117,41 @@ def test_c3_adjoint_matches_finite_differences(c3, precision):
     adj = np.array([gv[i, j] for (i, j) in cells])
     fd = np.array(fd)
     ev = _rel_l2(adj, fd)
-    # source coordinates of 16 sources (a perturbed source only changes its own misfit term)
-    sel = (uniform(30308, 16) * len(src)).astype(int)
-    dhs = 1e-4
-    fds, adjs = [], []
-    for s in sel:
-        for d in range(2):
-            sp, sm = src[s:s + 1].copy(), src[s:s + 1].copy()
-            sp[0, d] += dhs
-            sm[0, d] -= dhs
-            mp = model.misfit(sp, recs[s:s + 1], obs[s:s + 1], std[s:s + 1], extraparams=ep)
-            mm = model.misfit(sm, recs[s:s + 1], obs[s:s + 1], std[s:s + 1], extraparams=ep)
-            fds.append((mp - mm) / (2 * dhs))
-            adjs.append(gs[s, d])
-    es = _rel_l2(adjs, fds)
+    # every coordinate of every source (512): a perturbed source only changes its own misfit term,
+    # so the 2 x 512 perturbed sources run as one batched forward call and each term is summed here
+    ns = len(src)
+
+    def fd_sources(dhs):
+        pert, prec = [], []
+        for s in range(ns):
+            for d in range(2):
+                for sg in (1.0, -1.0):
+                    q = src[s].copy()
+                    q[d] += sg * dhs
+                    pert.append(q)
+                    prec.append(recs[s])
+        picks = model.traveltime(np.array(pert), prec, extraparams=ep)
+        k, out = 0, np.zeros((ns, 2))
+        for s in range(ns):
+            for d in range(2):
+                mp = 0.5 * float(np.sum((picks[k] - obs[s]) ** 2 / std[s] ** 2))
+                mm = 0.5 * float(np.sum((picks[k + 1] - obs[s]) ** 2 / std[s] ** 2))
+                out[s, d] = (mp - mm) / (2 * dhs)
+                k += 2
+        return out
+
+    # The heap FMM's misfit is only piecewise smooth in the source position: where the source sits
+    # within dh of a point at which the heap's order flips (e.g. halfway between two refined-grid
+    # nodes, where two source-box corners are equidistant) the misfit jumps and a central
+    # difference measures the jump, not a derivative.  A coordinate counts as resolved when the
+    # differences with dh = 1e-4 and 5e-5 agree to 1 %; the adjoint is compared on those.
+    fds, fds2 = fd_sources(1e-4), fd_sources(5e-5)
+    resolved = np.abs(fds - fds2) <= 1e-2 * np.maximum(np.abs(fds), np.abs(fds2))
+    es = _rel_l2(gs[resolved], fds[resolved])
     print(f"C3 FD check ({precision} adjoint): gradvel rel L2 over 20 cells {ev:.3e} (|g| ~ {np.abs(adj).mean():.3e}); "
-          f"gradsrc rel L2 over 32 coordinates {es:.3e}", flush=True)
-    assert ev <= 5e-2
-    assert es <= 5e-2
+          f"gradsrc rel L2 over the {int(resolved.sum())} of {2 * ns} source coordinates whose differences are "
+          f"resolved {es:.3e} (all {2 * ns}: {_rel_l2(gs, fds):.3e})", flush=True)
+    # a few times the measured errors: 4.4e-4 (cells, FP64) and 1.6e-3 over 484 resolved coordinates
+    assert ev <= FD_TOL_VEL
+    assert resolved.sum() >= 0.9 * resolved.size
+    assert es <= FD_TOL_SRC

@@ -8,12 +8,10 @@ no refinement) and the oracle restates those: parity unpinned against the refere
 against the oracle here.  Refinement and :gradsrcloc raise; :gradvel differentiates this
 forward and is checked against the oracle (below), whose adjoint finite differences pin.
 
-The radial model puts many heap acceptance-order near-ties within FP32 resolution.  Mixed
-precision treats traveltimes within 4 ulps as one heap batch (eik_stencil.h, tie_below); with
-exact FP32 comparisons the iteration had several fixed points there and could circulate for
-thousands of rounds (receivers then 1.6e-5 to 3e-5 from FP64).  Measured now: receivers 2.4e-6
-from the oracle; runs with different launch grids still differ on up to 0.04 % of the nodes (by
-at most 9e-5) and by 3e-7 at the receivers.
+Mixed precision keeps FP64 traveltimes and decides the heap order in FP64; only the quadratic's
+increment is FP32 (FP64 within 4 nodes of the source box).  Against FP64 at C4 the oracle's
+restatement of it measures grids 8.3e-5 relative at most, receivers 2.1e-6 and :gradvel 2.9e-4
+relative L2: the stated tolerances hold on every node.
 """
 import os
 
@@ -22,7 +20,7 @@ import pytest
 
 from cases import sphere3d
 from eikonalsolvers.grids import ExtraParams
-from eikonalsolvers.synthetic import c4_3d_sphere
+from eikonalsolvers.synthetic import c4_3d_sphere, gaussian_random_field, normal
 
 pytestmark = pytest.mark.gpu
 
@@ -54,10 +52,7 @@ def test_c4_sphere_forward(eik, c4, precision):
             assert rel.max() <= 1e-10
             np.testing.assert_allclose(pg[s], po[s], rtol=1e-12, atol=0)
         else:
-            # grid tolerance on 99.5 % of the nodes and in relative L2 (heap near-ties, see
-            # test_gpu_forward.py); receivers strictly
-            assert np.quantile(rel, 0.995) <= 1e-4
-            assert np.linalg.norm(tg[s] - to[s]) <= 1e-4 * np.linalg.norm(to[s])
+            assert rel.max() <= 1e-4
             assert relp.max() <= 1e-5
     print(f"C4 {precision}: worst grid max rel over 32 sources {worst:.3e}, receivers {worstp:.3e}", flush=True)
 
@@ -100,14 +95,27 @@ def test_sphere3d_gradvel_matches_oracle(eik, oracle, radius):
     print(f"sphere3d gradvel radius {radius}: FP64 vs oracle {e64:.2e}, mixed {e32:.2e}", flush=True)
     assert e64 <= 1e-7
     assert gm == pytest.approx(om, rel=1e-12)
-    if e32 > 1e-3:
-        pytest.xfail(f"mixed :gradvel {e32:.2e} from FP64 (tolerance 1e-3): FP32 heap near-ties change "
-                     "derivative codes on this radial-type model; FP64 is exact")
+    assert np.all(np.isfinite(gv32))
+    assert e32 <= 1e-3
 
 
 def test_c4_sphere_gradvel_mixed_vs_fp64(eik, c4):
+    """Observed picks as in the 3D Cartesian benchmark: the forward on a second model (the same
+    radial profile with another seeded 3 % perturbation, seed 40404) plus N(0, 0.01^2) noise, std
+    0.01.  The gradient scales with the residuals, so its relative error is about the pick error
+    over the residual: residuals of ~0.01 s alone on ~100 s picks would need picks to ~1e-7
+    relative for a 1e-3 gradient, beyond the stated 1e-5 receiver tolerance."""
     grd, vel, src, recs, po, to = c4
-    obs = [p + 0.01 * np.cos(np.arange(len(p))) for p in po]
+    r = grd.r[:, None, None]
+    velobs = (5.8 + 2.2 * (6371.0 - r) / 238.0) * (1.0 + 0.03 * gaussian_random_field(grd.grsize, 40404, 10.0, 1.0))
+    mo = eik.EikModel(velobs, grd)
+    pobs = mo.traveltime(src, recs, extraparams=ExtraParams(refinearoundsrc=False))
+    mo.close()
+    noise = normal(40405, sum(len(p) for p in pobs)) * 0.01
+    obs, k = [], 0
+    for p in pobs:
+        obs.append(p + noise[k:k + len(p)])
+        k += len(p)
     std = [np.full(len(p), 0.01) for p in po]
     m = eik.EikModel(vel, grd)
     g64, m64 = m.gradient(src, recs, obs, std, "gradvel", extraparams=ExtraParams(refinearoundsrc=False))
@@ -120,6 +128,4 @@ def test_c4_sphere_gradvel_mixed_vs_fp64(eik, c4):
     # residuals are ~0.01 s against picks of ~100 s, so the receivers' 1e-5 relative moves the
     # misfit by up to ~1e-3 relative
     assert m32 == pytest.approx(m64, rel=1e-3)
-    if rel > 1e-3:
-        pytest.xfail(f"mixed :gradvel {rel:.2e} from FP64 at C4 (tolerance 1e-3): the radial model's "
-                     "FP32 heap near-ties change derivative codes; FP64 is exact")
+    assert rel <= 1e-3

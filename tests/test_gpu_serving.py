@@ -1,7 +1,7 @@
 """Serving loop (eik_server): every request's result is the same alone and in the mix.
 
-A mix of 2D/3D traveltime and gradient requests (the serving workload's distribution, shrunk) is
-submitted at once to a 4-slot server, so requests are admitted while others run and their work
+A mix of 2D/3D traveltime and gradient requests (the serving workload's distribution: shrunk, and
+at full size with its largest request, 160^3 with 64 sources) is submitted at once to a 4-slot server, so requests are admitted while others run and their work
 overlaps on the device; each result is compared with the same request run alone through the
 plain API.  FP64: identical picks, gradients to 1e-12 relative.  Mixed precision: receivers 1e-5
 relative, gradients 1e-3 relative L2 (the stated tolerances).
@@ -11,8 +11,8 @@ import time
 import numpy as np
 import pytest
 
-from eikonalsolvers.grids import ExtraParams
-from eikonalsolvers.synthetic import c5_request
+from eikonalsolvers.grids import ExtraParams, Grid3DCart
+from eikonalsolvers.synthetic import c5_request, smooth_velocity, uniform
 
 pytestmark = pytest.mark.gpu
 
@@ -24,6 +24,75 @@ def _alone(req, model, ep):
         return model.traveltime(req["src"], req["recs"], extraparams=ep)
     return model.gradient(req["src"], req["recs"], req["pickobs"], req["stdobs"], "gradvelandsrcloc",
                           extraparams=ep)
+
+
+def _largest_gradient_request():
+    """The largest request the serving mix draws: 3D 160^3 (h = 1), 64 sources, 128 top-face
+    receivers, a gradient; drawn like the mix's requests (seeds 50999 / 51000 / 51001)."""
+    n, nsrc, nrec = 160, 64, 128
+    grd = Grid3DCart(hgrid=1.0, cooinit=(0.0, 0.0, 0.0), grsize=(n, n, n))
+    L = np.full(3, (n - 1) * 1.0)
+    src = (0.05 + 0.9 * uniform(51000, nsrc * 3).reshape(nsrc, 3)) * L
+    rec = uniform(51001, nrec * 3).reshape(nrec, 3) * L
+    rec[:, -1] = 0.0
+    return {"id": -1, "kind": "gradient", "grd": grd, "vel": smooth_velocity(grd.grsize, 50999), "src": src,
+            "recs": [rec.copy() for _ in range(nsrc)],
+            "pickobs": [np.sqrt(((rec - s) ** 2).sum(1)) / 3.5 for s in src],
+            "stdobs": [np.full(nrec, 0.01) for _ in range(nsrc)]}
+
+
+def _mix(reqs, models, ep, eik, nslots=4):
+    with eik.Server(nslots=nslots) as srv:
+        t_sub, handles = [], []
+        for r, m in zip(reqs, models):
+            t_sub.append(time.perf_counter())
+            if r["kind"] == "traveltime":
+                handles.append(srv.submit_traveltime(m, r["src"], r["recs"], extraparams=ep))
+            else:
+                handles.append(srv.submit_gradient(m, r["src"], r["recs"], r["pickobs"], r["stdobs"],
+                                                   "gradvelandsrcloc", extraparams=ep))
+        out = [h.result() for h in handles]
+        st = srv.stats()
+    spans = [(t + h.queue_ms / 1e3, t + (h.queue_ms + h.run_ms) / 1e3) for t, h in zip(t_sub, handles)]
+    n = len(reqs)
+    overlaps = sum(1 for i in range(n) for j in range(i + 1, n) if spans[i][0] < spans[j][1] and spans[j][0] < spans[i][1])
+    return out, handles, overlaps, st
+
+
+def _same(reqs, alone, mixed, precision):
+    for r, a, b in zip(reqs, alone, mixed):
+        if r["kind"] == "traveltime":
+            for pa, pb in zip(a, b):
+                if precision == "fp64":
+                    np.testing.assert_array_equal(pa, pb)
+                else:
+                    np.testing.assert_allclose(pb, pa, rtol=1e-5, atol=0)
+        else:
+            gva, gsa, ma = a
+            gvb, gsb, mb = b
+            tol = 1e-12 if precision == "fp64" else 1e-3
+            assert np.linalg.norm(gvb - gva) <= tol * np.linalg.norm(gva)
+            assert np.linalg.norm(gsb - gsa) <= tol * max(np.linalg.norm(gsa), 1e-300)
+            assert mb == pytest.approx(ma, rel=1e-12 if precision == "fp64" else 1e-5)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_full_size_mix_same_alone_and_in_the_mix(eik, precision):
+    """Full-size serving requests, including the largest the mix can draw (160^3, 64 sources, gradient),
+    run concurrently with six full-size mix requests; each equals its run alone."""
+    ep = ExtraParams(precision=precision)
+    reqs = [_largest_gradient_request()] + [c5_request(i) for i in range(6)]
+    print(f"precision={precision}: " + ", ".join(f"{r['kind'][0]}{r['grd'].grsize}x{len(r['src'])}" for r in reqs),
+          flush=True)
+    models = [eik.EikModel(r["vel"], r["grd"]) for r in reqs]
+    alone = [_alone(r, m, ep) for r, m in zip(reqs, models)]
+    mixed, handles, overlaps, st = _mix(reqs, models, ep, eik)
+    print(f"  latencies ms: {[round(h.latency_ms, 1) for h in handles]}; overlapping pairs: {overlaps}; {st}",
+          flush=True)
+    assert overlaps > 0, "requests never ran concurrently"
+    _same(reqs, alone, mixed, precision)
+    for m in models:
+        m.close()
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
