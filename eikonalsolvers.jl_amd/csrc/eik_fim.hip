@@ -414,12 +414,14 @@ int it = 0;
             const Key<T> kn{s_k[n], s_s[n], (int)s_d[n]};
             if (!(v == v0 && keq(kn, k0))) {
                 const bool quiet = tiny_change<T>(v0, k0, v, kn, settle);
+                // every field of the node is written (K = value, S = -inf for an ordinary node, and
+                // the key pools start as an unreached node's key): a reader racing this write sees each
+                // field from the node's old or new state, never a leftover key, so the wake-up rule
+                // below (the smaller of the old and new keys) covers whatever it read
                 tt[gl] = v;
                 dd[gl] = (uint8_t)kn.D;
-                if (kn.D) {
-                    kk[gl] = kn.K;
-                    ss[gl] = kn.S;
-                }
+                kk[gl] = kn.K;
+                ss[gl] = kn.S;
                 if (!quiet) {
                     const unsigned pv = __float_as_uint(key_down(k0.K < kn.K ? k0.K : kn.K));
 #pragma unroll
@@ -855,6 +857,14 @@ static int fim_run_t(eik_ctx* ctx, const InstDev* d_inst, int ninst, int64_t nti
     // are rounding ripples, and at heap near-ties within FP32 resolution they can otherwise
     // circulate for thousands of rounds before the settling stage ends them.
     const T base_settle = MIX ? T(mix_settle()) : (sizeof(T) == 8 ? (!exact ? settle_tol : T(0)) : T(f32_settle()));
+    // Verification: when no tile is left flagged, one more round relaxes every tile of every
+    // instance from scratch (every node re-evaluated).  A node can end a round below its fixed
+    // point after reading a neighbour that was being written in the same round; its too-low key
+    // then filters out the very changes that would correct it.  A verification round that changes
+    // nothing proves the state a fixed point of the replay (nothing is written, so no read races a
+    // write); otherwise the iteration goes on and verifies again.
+    bool verifying = false;
+    unsigned rounds_at_verify = 0u;
     for (int64_t batch = 0;; batch++) {
         if (batch >= maxbatches) return ctx->fail(EIK_ENOCONV, "traveltime solve did not converge");
         // the tolerance grows tenfold every 500 rounds while the cycle persists
@@ -888,7 +898,15 @@ static int fim_run_t(eik_ctx* ctx, const InstDev* d_inst, int ninst, int64_t nti
             CK(hipEventElapsedTime(&ms, ctx->kev[2 * r], ctx->kev[2 * r + 1]));
             stt.kernel_ms += ms;
         }
-        if (h_cnt[kCntFlagged] == 0u) break;
+        if (h_cnt[kCntFlagged] == 0u) {
+            if (verifying && h_cnt[kCntRounds] <= rounds_at_verify + 1u) break;  // verified
+            k_fill_u32<<<nblocks(ntiles, 256), 256, 0, st>>>(flags, ntiles, kFullBit);
+            k_fill_u32<<<nblocks(ntiles, 256), 256, 0, st>>>(pend, ntiles, 0u);
+            CK(hipGetLastError());
+            verifying = true;
+            rounds_at_verify = h_cnt[kCntRounds];
+            stt.verifications++;
+        }
     }
     const unsigned long long* h_c64 = (const unsigned long long*)(h_cnt + kCnt64);
     stt.rounds = (int64_t)h_cnt[kCntRounds];
@@ -1081,6 +1099,14 @@ int forward_solve(eik_ctx* ctx, const std::vector<FwdGroup>& G, int prec, const 
         dc.ensure((size_t)totN))
         return ctx->fail(EIK_ENOMEM, "out of device memory (traveltime grids)");
     CK(hipMemsetAsync(dc.p, 0, (size_t)totN, st));
+    // key pools start as an unreached node's key (+inf, -inf)
+    if (f32) {
+        k_fill<float><<<nblocks(totN, 256), 256, 0, st>>>((float*)kc.p, totN, EIK_INFF);
+        k_fill<float><<<nblocks(totN, 256), 256, 0, st>>>((float*)sc.p, totN, -EIK_INFF);
+    } else {
+        k_fill<double><<<nblocks(totN, 256), 256, 0, st>>>((double*)kc.p, totN, EIK_INF);
+        k_fill<double><<<nblocks(totN, 256), 256, 0, st>>>((double*)sc.p, totN, -EIK_INF);
+    }
     const TTPools cpools{ttc.p, kc.p, sc.p, (uint8_t*)dc.p};
     DevBuf& dci = ctx->buf("inst_coarse");
     if (dci.ensure(cinst.size() * sizeof(InstDev))) return ctx->fail(EIK_ENOMEM, "out of device memory");
@@ -1158,6 +1184,8 @@ int forward_solve(eik_ctx* ctx, const std::vector<FwdGroup>& G, int prec, const 
             df.ensure((size_t)nfine_pts))
             return ctx->fail(EIK_ENOMEM, "out of device memory (refined grids)");
         CK(hipMemsetAsync(df.p, 0, (size_t)nfine_pts, st));
+        k_fill<double><<<nblocks(nfine_pts, 256), 256, 0, st>>>((double*)kf.p, nfine_pts, EIK_INF);
+        k_fill<double><<<nblocks(nfine_pts, 256), 256, 0, st>>>((double*)sf.p, nfine_pts, -EIK_INF);
         const TTPools fpools{ttf.p, kf.p, sf.p, (uint8_t*)df.p};
         CK(hipMemcpyAsync(dfi.p, finst.data(), finst.size() * sizeof(InstDev), hipMemcpyHostToDevice, st));
         CK(hipMemcpyAsync(dfd.p, fine.data(), fine.size() * sizeof(FineDev), hipMemcpyHostToDevice, st));
@@ -1249,6 +1277,7 @@ int forward_solve(eik_ctx* ctx, const std::vector<FwdGroup>& G, int prec, const 
     ctx->stats[9] = cst.changed;
     ctx->stats[10] = cst.keychanged;
     ctx->stats[11] = cst.settled;
+    ctx->stats[12] = cst.verifications;
     return EIK_OK;
 }
 

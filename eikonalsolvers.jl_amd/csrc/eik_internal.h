@@ -359,7 +359,7 @@ struct eik_ctx {
     void* h_small = nullptr;  // pinned host mirror of the counters
     hipEvent_t ev[2] = {nullptr, nullptr};   // phase timer
     hipEvent_t kev[2 * kRoundBatch] = {};    // round-kernel timers (start, end) of one round batch
-    double stats[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double stats[13] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     int xcap = 1;  // capacity per source of the transferred-point list
     std::mutex mu;
     // arena mode (the no-allocation entry point): buffers are carved from this block instead
@@ -460,6 +460,7 @@ struct FimStats {
     double changed = 0;        // node updates that changed the node
     double keychanged = 0;     // of those, the ones that changed only the pop key
     int settled = 0;           // settling stage reached (0: exact; k: tolerance 10^(k-1) x base)
+    int verifications = 0;     // verification rounds (every tile relaxed once more at convergence)
     double kernel_ms = 0;      // summed duration of the round launches
 };
 // traveltime pool plus its pop-key pools (K, S in the pool's precision, D as bytes)

@@ -139,7 +139,9 @@ int eik_mem_info(eik_ctx* ctx, int64_t* free_bytes, int64_t* total_bytes);
  * [8] = node updates (stencil replays) of the coarse solve, [9] = those that changed the node,
  * [10] = of those, key-only changes, [11] = settling stage of the coarse solve: 0 = exact; k > 0
  * = after 2000 rounds, changes below 10^(k-1) x 1e-12 relative (FP64; 4e-6 in FP32) stopped
- * propagating (the tolerance grows tenfold every 500 rounds).  Converging solves stay at 0. */
+ * propagating (the tolerance grows tenfold every 500 rounds).  Converging solves stay at 0.
+ * [12] = verification rounds of the coarse solve (at convergence every tile is relaxed once more;
+ * one round that changes nothing ends the solve). */
 int eik_last_stats(const eik_ctx* ctx, double* out, int32_t nout);
 
 /* tracerays (src/EikSolv/eikutils.jl:201-328): for every source's receivers, the ray descending
