@@ -17,6 +17,9 @@ FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     # FP64 stencils follow the reference's operation order; no implicit FMA contraction
     "-ffp-contract=off",
+    # atomics are issued as written: the kernels reduce across lanes themselves where that pays,
+    # and the compiler's per-wave rewrite of an atomic waits for its return value on the spot
+    "-mllvm", "-amdgpu-atomic-optimizer-strategy=None",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
     # the stencil combines comparisons with & and | on purpose (no short-circuit branches)
     "-Wno-bitwise-instead-of-logical",

@@ -460,7 +460,7 @@ int eik_mem_info(eik_ctx* ctx, int64_t* free_bytes, int64_t* total_bytes) {
 
 int eik_last_stats(const eik_ctx* ctx, double* out, int32_t nout) {
     if (!ctx || !out) return EIK_EINVAL;
-    for (int i = 0; i < nout && i < 13; i++) out[i] = ctx->stats[i];
+    for (int i = 0; i < nout && i < 17; i++) out[i] = ctx->stats[i];
     return EIK_OK;
 }
 

@@ -359,7 +359,7 @@ struct eik_ctx {
     void* h_small = nullptr;  // pinned host mirror of the counters
     hipEvent_t ev[2] = {nullptr, nullptr};   // phase timer
     hipEvent_t kev[2 * kRoundBatch] = {};    // round-kernel timers (start, end) of one round batch
-    double stats[13] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double stats[17] = {};
     int xcap = 1;  // capacity per source of the transferred-point list
     std::mutex mu;
     // arena mode (the no-allocation entry point): buffers are carved from this block instead
@@ -381,6 +381,21 @@ struct eik_ctx {
         return code;
     }
 };
+
+namespace eik {
+// device bytes a context's reusable pools hold (they grow to the largest call seen and are kept)
+inline int64_t ctx_pool_bytes(const eik_ctx* ctx) {
+    int64_t b = 0;
+    for (const auto& kv : ctx->bufs) b += (int64_t)kv.second.cap;
+    return b;
+}
+// release a context's reusable pools once its stream is idle (the next call allocates again)
+inline void ctx_trim(eik_ctx* ctx) {
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto& kv : ctx->bufs) kv.second.release();
+}
+}  // namespace eik
 
 struct eik_model {
     eik_ctx* ctx = nullptr;
@@ -462,6 +477,7 @@ struct FimStats {
     int settled = 0;           // settling stage reached (0: exact; k: tolerance 10^(k-1) x base)
     int verifications = 0;     // verification rounds (every tile relaxed once more at convergence)
     double kernel_ms = 0;      // summed duration of the round launches
+    int64_t launches = 0;      // round-kernel launches (rounds that found no tile included)
 };
 // traveltime pool plus its pop-key pools (K, S in the pool's precision, D as bytes)
 struct TTPools {

@@ -28,7 +28,8 @@ EXPORTS = (
     "eik_model_create", "eik_model_update_vel", "eik_model_destroy",
     "eik_traveltime", "eik_gradient", "eik_misfit", "eik_last_stats",
     "eik_server_create", "eik_server_destroy", "eik_submit", "eik_wait", "eik_server_stats",
-    "eik_tracerays", "eik_traveltime_batch", "eik_poll", "eik_tracerays_max_steps",
+    "eik_tracerays", "eik_traveltime_batch", "eik_poll", "eik_tracerays_max_steps", "eik_wait_stats",
+    "eik_request_bytes",
     "eik_traveltime_noalloc", "eik_mem_info",
 )
 
@@ -111,6 +112,10 @@ def load():
         L.eik_submit.restype = ctypes.c_int
         L.eik_wait.argtypes = [vp, ctypes.c_int64, dp, ctypes.c_char_p, ctypes.c_int64]
         L.eik_wait.restype = ctypes.c_int
+        L.eik_wait_stats.argtypes = [vp, ctypes.c_int64, dp, dp, ctypes.c_int32, ctypes.c_char_p, ctypes.c_int64]
+        L.eik_wait_stats.restype = ctypes.c_int
+        L.eik_request_bytes.argtypes = [P(EikRequest), ip]
+        L.eik_request_bytes.restype = ctypes.c_int
         L.eik_server_stats.argtypes = [vp, ip, ctypes.c_int32]
         L.eik_server_stats.restype = ctypes.c_int
         L.eik_tracerays.argtypes = [vp, P(EikGrid), ctypes.c_int64, dp, dp, ip, dp, ctypes.c_double, ip, dp,

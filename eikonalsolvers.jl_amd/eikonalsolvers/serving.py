@@ -38,6 +38,7 @@ class Request:
         self._out = outputs
         self.whichgrad = whichgrad
         self.latency_ms = self.queue_ms = self.run_ms = None
+        self.stats = None  # solver statistics of the call that ran the request (see result())
         self._done = False
         self._result = None
 
@@ -54,9 +55,13 @@ class Request:
         if self._done:
             return self._result
         t = np.zeros(3)
+        st = np.zeros(14)
         err = ctypes.create_string_buffer(512)
-        rc = _lib.load().eik_wait(self._server.handle, self.id, _lib.ptr(t), err, 512)
+        rc = _lib.load().eik_wait_stats(self._server.handle, self.id, _lib.ptr(t), _lib.ptr(st), 14, err, 512)
         self.latency_ms, self.queue_ms, self.run_ms = (float(x) for x in t)
+        self.stats = {"rounds": int(st[5]), "settle_stage": int(st[11]), "verify_rounds": int(st[12]),
+                      "kernel_ms": float(st[6]), "tile_activations": float(st[4]), "node_updates": float(st[8]),
+                      "requests_in_call": int(st[13])}
         self._done = True
         _lib.check(rc, None, err.value.decode(errors="replace"))
         off = self._off
@@ -104,12 +109,27 @@ def traveltime_batch(items, *, extraparams=None, ctx=None):
     return [[p[o[s]:o[s + 1]].copy() for s in range(ns)] for p, o, ns in outs]
 
 
+def request_bytes(model, nsrc: int, kind: str = "traveltime", *, extraparams=None) -> int:
+    """The device working set (bytes) the server counts for a request of ``nsrc`` sources on
+    ``model`` (``kind`` "traveltime" or "gradient"; ``eik_request_bytes``, no device work)."""
+    req = _lib.EikRequest()
+    req.type = REQ_TRAVELTIME if kind == "traveltime" else REQ_GRADIENT
+    req.model = model.handle
+    req.nsrc = int(nsrc)
+    req.params = _params_struct(extraparams)
+    out = ctypes.c_int64()
+    _lib.check(_lib.load().eik_request_bytes(ctypes.byref(req), ctypes.byref(out)), None, "eik_request_bytes failed")
+    return int(out.value)
+
+
 class Server:
     """Continuous-batching request loop (``eik_server``) on one device."""
 
     def __init__(self, nslots: int = 4, mem_budget: int | None = None, device: int = 0):
+        """``mem_budget``: device bytes the slots may hold; None (or 0): the device's free memory,
+        read at every admission, less a reserve of max(2 GiB, 1/32 of the device)."""
         if mem_budget is None:
-            mem_budget = 64 << 30
+            mem_budget = 0
         h = ctypes.c_void_p()
         rc = _lib.load().eik_server_create(int(device), int(nslots), int(mem_budget), ctypes.byref(h))
         _lib.check(rc, None, "eik_server_create failed")
@@ -174,10 +194,11 @@ class Server:
         return r
 
     def stats(self) -> dict:
-        out = np.zeros(6, dtype=np.int64)
-        _lib.load().eik_server_stats(self.handle, _lib.ptr(out, ctypes.c_int64), 6)
+        out = np.zeros(10, dtype=np.int64)
+        _lib.load().eik_server_stats(self.handle, _lib.ptr(out, ctypes.c_int64), 10)
         return {"queued": int(out[0]), "running": int(out[1]), "admitted_bytes": int(out[2]), "budget": int(out[3]),
-                "shared_batches": int(out[4]), "requests_in_shared_batches": int(out[5])}
+                "shared_batches": int(out[4]), "requests_in_shared_batches": int(out[5]), "pool_bytes": int(out[6]),
+                "rejected_over_budget": int(out[7]), "pool_releases": int(out[8]), "oom_retries": int(out[9])}
 
     def close(self):
         """Wait for every submitted request, then release the slots."""

@@ -141,7 +141,10 @@ int eik_mem_info(eik_ctx* ctx, int64_t* free_bytes, int64_t* total_bytes);
  * = after 2000 rounds, changes below 10^(k-1) x 1e-12 relative (FP64; 4e-6 in FP32) stopped
  * propagating (the tolerance grows tenfold every 500 rounds).  Converging solves stay at 0.
  * [12] = verification rounds of the coarse solve (at convergence every tile is relaxed once more;
- * one round that changes nothing ends the solve). */
+ * one round that changes nothing ends the solve), [13] = round-kernel launches of the coarse solve
+ * (rounds that found no tile included), [14] = summed duration of the near-source (fine grid)
+ * solve's round launches (ms), [15] = those launches, [16] = that solve's tile activations.
+ * Values a call did not produce are 0. */
 int eik_last_stats(const eik_ctx* ctx, double* out, int32_t nout);
 
 /* tracerays (src/EikSolv/eikutils.jl:201-328): for every source's receivers, the ray descending
@@ -168,8 +171,14 @@ int eik_tracerays_max_steps(const eik_grid* grid, double steplen, int64_t* out);
  * No reference counterpart: the reference evaluates one call at a time.  A server runs
  * traveltime and gradient requests with continuous batching: nslots slots (each a context with
  * its own stream and reusable device pools) take queued requests as soon as a slot is free and
- * the request's device working set fits in what mem_budget (bytes) has left; admitted requests
- * run concurrently and each batches all of its sources into shared launches.  The arrays a
+ * the request's device working set fits in what the memory budget has left; admitted requests
+ * run concurrently and each batches all of its sources into shared launches.  What the server
+ * holds on the device is, per slot, the larger of the pools the slot keeps from its largest call
+ * and the working set of the request it runs; a request is admitted when that sum stays within
+ * the budget (idle slots' pools are released when that makes the room).  mem_budget > 0 fixes the
+ * budget in bytes; mem_budget = 0 takes the device's free memory (read at every admission) plus
+ * what the slots hold, less a reserve of max(2 GiB, 1/32 of the device).  A request whose working
+ * set exceeds the whole budget fails on its own with EIK_ENOMEM (eik_wait returns it).  The arrays a
  * request points to (inputs and outputs) must stay valid until eik_wait returns for it, and its
  * model must not be updated or destroyed meanwhile. */
 #define EIK_REQ_TRAVELTIME 1 /* eiktraveltime: fills ttpicks */
@@ -197,15 +206,26 @@ int eik_server_create(int32_t device, int32_t nslots, int64_t mem_budget, eik_se
 /* waits for queued and running requests, then releases the slots */
 void eik_server_destroy(eik_server* server);
 int eik_submit(eik_server* server, const eik_request* request, int64_t* id);
+/* The device working set the server counts for a request (host-only, no device work): reads only
+ * type, model, nsrc and params. */
+int eik_request_bytes(const eik_request* request, int64_t* out);
 /* Blocks until request id has finished; returns its status (EIK_OK or the call's error code).
  * times_ms (optional, 3 doubles): latency from submission, queueing delay, run time; errbuf
  * (optional) receives the error message. */
 int eik_wait(eik_server* server, int64_t id, double* times_ms, char* errbuf, int64_t errlen);
+/* eik_wait, and stats (optional, up to 14 doubles): the eik_last_stats values of the call that ran
+ * the request ([0..12]), then the number of requests that call held ([13]; > 1 for a shared
+ * traveltime launch). */
+int eik_wait_stats(eik_server* server, int64_t id, double* times_ms, double* stats, int32_t nstats, char* errbuf,
+                   int64_t errlen);
 /* Non-blocking: *done = 1 when request id has finished (eik_wait then returns at once), else 0.
  * EIK_EINVAL for an unknown id (never submitted, or already waited for). */
 int eik_poll(eik_server* server, int64_t id, int32_t* done);
-/* out[0] queued requests, [1] running, [2] admitted working-set bytes, [3] budget,
- * [4] shared launches run for two or more traveltime requests, [5] requests run in them */
+/* out[0] queued requests, [1] running, [2] admitted working-set bytes, [3] budget (the last
+ * admission's), [4] shared launches run for two or more traveltime requests, [5] requests run in
+ * them, [6] bytes the slots' pools hold, [7] requests failed as larger than the budget, [8] idle
+ * slots whose pools were released to admit a request, [9] calls retried after running out of
+ * device memory */
 int eik_server_stats(eik_server* server, int64_t* out, int32_t nout);
 
 /* Several traveltime requests solved in shared launches (the server's continuous batching; no

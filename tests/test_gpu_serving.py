@@ -208,3 +208,91 @@ def test_server_batches_queued_traveltime_requests(eik):
             np.testing.assert_array_equal(pb, pa)
     for m in models:
         m.close()
+
+
+def _same_gradient_requests(eik, count):
+    """`count` copies of one gradient request of the shrunk serving mix, on one model (equal
+    working sets), and that request's result alone."""
+    r = next(q for q in (c5_request(i, small=True) for i in range(40)) if q["kind"] == "gradient")
+    m = eik.EikModel(r["vel"], r["grd"])
+    return r, m, [r] * count
+
+
+def test_budget_binds_requests_wait_for_memory(eik):
+    """A budget of 1.5 working sets on a two-slot server: two requests never run at once (the
+    second waits, or the idle slot's pools are released for it), the pools the slots hold stay
+    within the budget, and every result equals the request alone (FP64: gradients to 1e-12
+    relative, the order of the atomic reductions aside)."""
+    from eikonalsolvers.serving import request_bytes
+
+    ep = ExtraParams()
+    r, m, reqs = _same_gradient_requests(eik, 4)
+    b = request_bytes(m, len(r["src"]), "gradient", extraparams=ep)
+    budget = int(1.5 * b)
+    alone = _alone(r, m, ep)
+    with eik.Server(nslots=2, mem_budget=budget) as srv:
+        t_sub, hs = [], []
+        for q in reqs:
+            t_sub.append(time.perf_counter())
+            hs.append(srv.submit_gradient(m, q["src"], q["recs"], q["pickobs"], q["stdobs"], "gradvelandsrcloc",
+                                          extraparams=ep))
+        out = [h.result() for h in hs]
+        st = srv.stats()
+    spans = sorted((t + h.queue_ms / 1e3, t + (h.queue_ms + h.run_ms) / 1e3) for t, h in zip(t_sub, hs))
+    print(f"working set {b} B, budget {budget} B, stats {st}, spans {spans}")
+    assert st["budget"] == budget
+    assert all(spans[k + 1][0] >= spans[k][1] - 1e-3 for k in range(len(spans) - 1)), "requests overlapped"
+    assert 0 < st["pool_bytes"] <= budget
+    _same(reqs, [alone] * len(reqs), out, "fp64")  # gradients to 1e-12 (atomic reductions), misfit 1e-12
+    m.close()
+
+
+def test_request_larger_than_budget_fails_alone(eik):
+    """A request whose working set exceeds the whole budget fails with EIK_ENOMEM (MemoryError)
+    without running; a smaller request submitted with it runs."""
+    from eikonalsolvers.serving import request_bytes
+
+    ep = ExtraParams()
+    r, m, _ = _same_gradient_requests(eik, 1)
+    small = next(q for q in (c5_request(i, small=True) for i in range(40))
+                 if q["kind"] == "traveltime" and q["grd"].ndim == 2)
+    ms = eik.EikModel(small["vel"], small["grd"])
+    b_big = request_bytes(m, len(r["src"]), "gradient", extraparams=ep)
+    b_small = request_bytes(ms, len(small["src"]), "traveltime", extraparams=ep)
+    assert b_small < b_big // 2
+    with eik.Server(nslots=2, mem_budget=b_big // 2) as srv:
+        h_big = srv.submit_gradient(m, r["src"], r["recs"], r["pickobs"], r["stdobs"], "gradvel", extraparams=ep)
+        h_small = srv.submit_traveltime(ms, small["src"], small["recs"], extraparams=ep)
+        with pytest.raises(MemoryError, match="budget"):
+            h_big.result()
+        got = h_small.result()
+        st = srv.stats()
+    assert st["rejected_over_budget"] == 1
+    for pa, pb in zip(ms.traveltime(small["src"], small["recs"], extraparams=ep), got):
+        np.testing.assert_array_equal(pb, pa)
+    m.close()
+    ms.close()
+
+
+def test_default_budget_follows_free_device_memory(eik):
+    """With no budget given the server admits against the device's free memory less the reserve
+    (max(2 GiB, 1/32 of the device)), read at every admission."""
+    import ctypes
+
+    from eikonalsolvers import _lib
+
+    free, total = ctypes.c_int64(), ctypes.c_int64()
+    assert _lib.load().eik_mem_info(eik.default_context().handle, ctypes.byref(free), ctypes.byref(total)) == 0
+    reserve = max(2 << 30, total.value // 32)
+    r, m, reqs = _same_gradient_requests(eik, 2)
+    with eik.Server(nslots=2) as srv:
+        b0 = srv.stats()["budget"]
+        hs = [srv.submit_gradient(m, q["src"], q["recs"], q["pickobs"], q["stdobs"], "gradvel") for q in reqs]
+        for h in hs:
+            h.result()
+            assert h.stats["rounds"] > 0 and h.stats["requests_in_call"] == 1
+        st = srv.stats()
+    print(f"free {free.value}, total {total.value}, budget at creation {b0}, after {st}")
+    assert 0 < b0 <= free.value - reserve + (64 << 20)
+    assert 0 < st["budget"] <= total.value - reserve
+    m.close()
