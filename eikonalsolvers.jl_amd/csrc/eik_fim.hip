@@ -71,13 +71,6 @@ __device__ __forceinline__ unsigned wave_min_u32(unsigned v) {
     return min(min(a, b), min(c, d));
 }
 
-// An empty asm that takes x and hands it back: the compiler must have x at this point and can
-// assume nothing about the result (no load is moved past it or turned conditional).
-template <typename T>
-__device__ __forceinline__ void pin_value(T& x) {
-    asm volatile("" : "+v"(x));
-}
-
 template <typename T>
 __device__ __forceinline__ bool is_fixed(T raw) {
     return signbit(raw);
@@ -168,8 +161,10 @@ struct TileLds {
 // (local tile, instance, pending value, face bits) from a cursor, claiming the next entry while
 // the current tile runs.  SPH: spherical grids (per-node metric); Cartesian grids share one
 // metric, held in scalar registers.  MIX: the mixed precision (FP64 values, keys and comparisons,
-// the quadratic's increment in FP32; eik_stencil.h).
-template <int ND, typename T, bool SPH, bool MIX>
+// the quadratic's increment in FP32; eik_stencil.h).  NEAR: some instance has an FP64 region (an
+// unrefined mixed solve) whose nodes take the FP64 stencil; compiled only where it can occur, since
+// that second replay costs registers.
+template <int ND, typename T, bool SPH, bool MIX, bool NEAR>
 __global__ void __attribute__((amdgpu_flat_work_group_size(kTileThreads, kTileThreads),
                                amdgpu_waves_per_eu(RoundOcc<ND, T>::value))) k_fim_round(
     const uint4* __restrict__ list, unsigned* __restrict__ count, const InstDev* __restrict__ inst,
@@ -250,19 +245,22 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(kTileThreads, kTileTh
         const int nx = I.n[0], ny = I.n[1], nz = ND == 3 ? I.n[2] : 1;
         const int64_t nxy = (int64_t)nx * ny;
 
-        // Stage the core and the face halos (this lane's own node first).  Every load of the tile
-        // image is issued before any of them is used, from addresses clamped into the grid (slots
-        // outside it are replaced by an unreached node afterwards), so the image costs one memory
-        // round trip.  The key pools hold a cascade node's key; an ordinary node's key is its value.
+        // Stage the core and the face halos (this lane's own node first).  The values and depth
+        // bytes of the whole tile image are loaded before any is used, from addresses clamped into
+        // the grid (slots outside it are replaced by an unreached node afterwards): one memory round
+        // trip.  The key pools are read only for cascade nodes (depth byte set), which are rare: a
+        // wave with none makes no second trip and reads no key bytes.  An ordinary node's key is
+        // its value.
         constexpr int NC = NL / NN - 1;  // halo chunks of NN slots
         const int g[3] = {ox + cx, oy + cy, oz + cz};
         const bool inside = g[0] < nx && g[1] < ny && g[2] < nz;
         const int64_t gl = g[0] + (int64_t)nx * g[1] + nxy * g[2];
         const int64_t glc = min(g[0], nx - 1) + (int64_t)nx * min(g[1], ny - 1) + nxy * min(g[2], nz - 1);
-        T v0 = tt[glc], sl = slow[glc], k0r = kk[glc], s0r = ss[glc];
+        T v0 = tt[glc], sl = slow[glc];
         int d0 = dd[glc];
-        T hv[NC], hk[NC], hs[NC];
+        T hv[NC];
         int hd[NC];
+        int64_t hl[NC];
         bool hin[NC];
 #pragma unroll
         for (int h = 0; h < NC; h++) {
@@ -270,36 +268,41 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(kTileThreads, kTileTh
             L::halo_pos((h + 1) * NN + n, p);
             const int hx = ox + p[0], hy = oy + p[1], hz = oz + p[2];
             hin[h] = hx >= 0 && hx < nx && hy >= 0 && hy < ny && hz >= 0 && hz < nz;
-            const int64_t l = min(max(hx, 0), nx - 1) + (int64_t)nx * min(max(hy, 0), ny - 1) +
-                              nxy * min(max(hz, 0), nz - 1);
-            hv[h] = tt[l];
-            hd[h] = dd[l];
-            hk[h] = kk[l];
-            hs[h] = ss[l];
-        }
-        // (an empty asm on the key values keeps the compiler from turning the selects below into
-        // branches around the key loads, which would cost a second round trip)
-        pin_value(k0r);
-        pin_value(s0r);
-#pragma unroll
-        for (int h = 0; h < NC; h++) {
-            pin_value(hk[h]);
-            pin_value(hs[h]);
+            hl[h] = min(max(hx, 0), nx - 1) + (int64_t)nx * min(max(hy, 0), ny - 1) + nxy * min(max(hz, 0), nz - 1);
+            hv[h] = tt[hl[h]];
+            hd[h] = dd[hl[h]];
         }
         if (!inside) {
             v0 = tinf<T>();
             sl = T(0);
             d0 = 0;
         }
-        const Key<T> k0{d0 ? k0r : fabs(v0), d0 ? s0r : -tinf<T>(), d0};
+        Key<T> k0{fabs(v0), -tinf<T>(), d0};
+        if (d0) {
+            k0.K = kk[glc];
+            k0.S = ss[glc];
+        }
+        T hk[NC], hs[NC];
+#pragma unroll
+        for (int h = 0; h < NC; h++) {
+            if (!hin[h]) {
+                hv[h] = tinf<T>();
+                hd[h] = 0;
+            }
+            hk[h] = fabs(hv[h]);
+            hs[h] = -tinf<T>();
+            if (hd[h]) {
+                hk[h] = kk[hl[h]];
+                hs[h] = ss[hl[h]];
+            }
+        }
 #pragma unroll
         for (int h = 0; h < NC; h++) {
             const int c = (h + 1) * NN + n;
-            const int dv = hin[h] ? hd[h] : 0;
-            s_t[c] = hin[h] ? hv[h] : tinf<T>();
-            s_k[c] = hin[h] ? (dv ? hk[h] : fabs(hv[h])) : tinf<T>();
-            s_s[c] = dv ? hs[h] : -tinf<T>();
-            s_d[c] = (uint8_t)dv;
+            s_t[c] = hv[h];
+            s_k[c] = hk[h];
+            s_s[c] = hs[h];
+            s_d[c] = (uint8_t)hd[h];
         }
         s_t[n] = v0;
         s_k[n] = k0.K;
@@ -399,7 +402,7 @@ int it = 0;
                 Key<T> nk;
                 T nv = tinf<T>();
                 bool near64 = false;  // mixed precision near the source: the FP64 stencil
-                if constexpr (MIX) {
+                if constexpr (MIX && NEAR) {
                     near64 = in_fp64_region(I, g);
                     if (near64) {
                         Metric<double> m64;
@@ -900,7 +903,7 @@ static double mix_settle() {
 // the per-instance minimum, selects the tiles within the band, and runs the round kernel on a
 // fixed grid of resident blocks that take tiles from the device-side list.  After a batch the
 // host reads how many tiles are still flagged; none means converged.
-template <int ND, typename T, bool SPH, bool MIX>
+template <int ND, typename T, bool SPH, bool MIX, bool NEAR>
 static int fim_run_t(eik_ctx* ctx, const InstDev* d_inst, int ninst, int64_t ntiles, const TTPools& P,
                      const T* slowpool, bool exact, FimStats* fs) {
     DevBuf& list = ctx->buf("fim_tile_list");
@@ -923,7 +926,7 @@ static int fim_run_t(eik_ctx* ctx, const InstDev* d_inst, int ninst, int64_t nti
     static const int grid = [ctx] {
         int ncu = 0, occ = 0;
         if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess) ncu = 1;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_fim_round<ND, T, SPH, MIX>, kTileThreads, 0) != hipSuccess)
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_fim_round<ND, T, SPH, MIX, NEAR>, kTileThreads, 0) != hipSuccess)
             occ = 1;
         constexpr int kMaxPerCu = ND == 3 ? 8 : 5;
         int gr = std::max(1, ncu * std::min(kMaxPerCu, std::max(1, occ)));
@@ -973,7 +976,7 @@ static int fim_run_t(eik_ctx* ctx, const InstDev* d_inst, int ninst, int64_t nti
                                                                (const unsigned*)wm.p, (uint4*)list.p,
                                                                d_cnt + kCntList);
             CK(hipEventRecord(ctx->kev[2 * r], st));
-            k_fim_round<ND, T, SPH, MIX><<<g, kTileThreads, 0, st>>>((const uint4*)list.p, d_cnt, d_inst, (const unsigned*)wm.p,
+            k_fim_round<ND, T, SPH, MIX, NEAR><<<g, kTileThreads, 0, st>>>((const uint4*)list.p, d_cnt, d_inst, (const unsigned*)wm.p,
                                                           (T*)P.tt, (T*)P.k, (T*)P.s, P.d, slowpool, flags, pend,
                                                           d_c64, settle, claim_chunk());
             CK(hipGetLastError());
@@ -1015,25 +1018,32 @@ static int fim_run_t(eik_ctx* ctx, const InstDev* d_inst, int ninst, int64_t nti
 }
 
 template <int ND, typename T, bool MIX>
-static int fim_run_k(eik_ctx* ctx, bool sph, const InstDev* d_inst, int ninst, int64_t ntiles, const TTPools& P,
-                     const void* slowpool, bool exact, FimStats* fs) {
-    return sph ? fim_run_t<ND, T, true, MIX>(ctx, d_inst, ninst, ntiles, P, (const T*)slowpool, exact, fs)
-               : fim_run_t<ND, T, false, MIX>(ctx, d_inst, ninst, ntiles, P, (const T*)slowpool, exact, fs);
+static int fim_run_k(eik_ctx* ctx, bool sph, bool near, const InstDev* d_inst, int ninst, int64_t ntiles,
+                     const TTPools& P, const void* slowpool, bool exact, FimStats* fs) {
+    if constexpr (MIX) {
+        if (near)
+            return sph ? fim_run_t<ND, T, true, true, true>(ctx, d_inst, ninst, ntiles, P, (const T*)slowpool, exact, fs)
+                       : fim_run_t<ND, T, false, true, true>(ctx, d_inst, ninst, ntiles, P, (const T*)slowpool, exact, fs);
+    }
+    return sph ? fim_run_t<ND, T, true, MIX, false>(ctx, d_inst, ninst, ntiles, P, (const T*)slowpool, exact, fs)
+               : fim_run_t<ND, T, false, MIX, false>(ctx, d_inst, ninst, ntiles, P, (const T*)slowpool, exact, fs);
 }
 
 template <int ND>
-static int fim_run_nd(eik_ctx* ctx, int prec, bool sph, const InstDev* d_inst, int ninst, int64_t ntiles,
+static int fim_run_nd(eik_ctx* ctx, int prec, bool sph, bool near, const InstDev* d_inst, int ninst, int64_t ntiles,
                       const TTPools& P, const void* slowpool, bool exact, FimStats* fs) {
-    if (prec == EIK_PREC_FP32) return fim_run_k<ND, float, false>(ctx, sph, d_inst, ninst, ntiles, P, slowpool, exact, fs);
-    if (prec == EIK_PREC_MIXED) return fim_run_k<ND, double, true>(ctx, sph, d_inst, ninst, ntiles, P, slowpool, exact, fs);
-    return fim_run_k<ND, double, false>(ctx, sph, d_inst, ninst, ntiles, P, slowpool, exact, fs);
+    if (prec == EIK_PREC_FP32)
+        return fim_run_k<ND, float, false>(ctx, sph, false, d_inst, ninst, ntiles, P, slowpool, exact, fs);
+    if (prec == EIK_PREC_MIXED)
+        return fim_run_k<ND, double, true>(ctx, sph, near, d_inst, ninst, ntiles, P, slowpool, exact, fs);
+    return fim_run_k<ND, double, false>(ctx, sph, false, d_inst, ninst, ntiles, P, slowpool, exact, fs);
 }
 
 int fim_run(eik_ctx* ctx, int nd, int prec, int kind, const InstDev* d_inst, int ninst, int64_t ntiles,
-            const TTPools& pools, const void* slowpool, bool exact, FimStats* fs) {
+            const TTPools& pools, const void* slowpool, bool exact, bool near, FimStats* fs) {
     const bool sph = kind == EIK_GRID2D_SPHERE || kind == EIK_GRID3D_SPHERE;
-    return nd == 2 ? fim_run_nd<2>(ctx, prec, sph, d_inst, ninst, ntiles, pools, slowpool, exact, fs)
-                   : fim_run_nd<3>(ctx, prec, sph, d_inst, ninst, ntiles, pools, slowpool, exact, fs);
+    return nd == 2 ? fim_run_nd<2>(ctx, prec, sph, near, d_inst, ninst, ntiles, pools, slowpool, exact, fs)
+                   : fim_run_nd<3>(ctx, prec, sph, near, d_inst, ninst, ntiles, pools, slowpool, exact, fs);
 }
 
 // ----------------------------------------------------------------- forward pass of one call
@@ -1308,7 +1318,7 @@ int forward_solve(eik_ctx* ctx, const std::vector<FwdGroup>& G, int prec, const 
         tm.mark();
         FimStats fst{};
         int rc = fim_run(ctx, nd, EIK_PREC_FP64, kind, (const InstDev*)dfi.p, (int)nsrc, ntiles_f, fpools, slf.p,
-                         P->fast == 0, &fst);
+                         P->fast == 0, false, &fst);
         if (rc) return rc;
         ctx->stats[1] = tm.mark();
         ctx->stats[14] = fst.kernel_ms;
@@ -1363,8 +1373,10 @@ int forward_solve(eik_ctx* ctx, const std::vector<FwdGroup>& G, int prec, const 
         tm.mark();
     }
     FimStats cst{};
+    // unrefined mixed solves give their instances an FP64 region around the source (above)
+    const bool near = prec == EIK_PREC_MIXED && !P->refinearoundsrc;
     int rc = fim_run(ctx, nd, prec, kind, (const InstDev*)dci.p, (int)nsrc, ntiles_c, cpools, slow, P->fast == 0,
-                     &cst);
+                     near, &cst);
     if (rc) return rc;
     ctx->stats[2] = tm.mark();
     ctx->stats[4] = cst.activations;

@@ -486,8 +486,9 @@ struct TTPools {
     void* s;
     uint8_t* d;
 };
+// near: some instance has an FP64 region (unrefined mixed solve; InstDev::xlo/xhi)
 int fim_run(eik_ctx* ctx, int nd, int prec, int kind, const InstDev* d_inst, int ninst, int64_t ntiles,
-            const TTPools& pools, const void* slowpool, bool exact, FimStats* fs);
+            const TTPools& pools, const void* slowpool, bool exact, bool near, FimStats* fs);
 
 // no-allocation scratch (eik_noalloc.hip): resolve this device's copy once (context creation)
 void scratch_setup(int device);
