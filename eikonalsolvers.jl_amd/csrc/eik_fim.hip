@@ -167,9 +167,10 @@ struct TileLds {
 template <int ND, typename T, bool SPH, bool MIX, bool NEAR>
 __global__ void __attribute__((amdgpu_flat_work_group_size(kTileThreads, kTileThreads),
                                amdgpu_waves_per_eu(RoundOcc<ND, T>::value))) k_fim_round(
-    const uint4* __restrict__ list, unsigned* __restrict__ count, const InstDev* __restrict__ inst,
-    const unsigned* __restrict__ wmin, T* __restrict__ ttpool, T* __restrict__ kpool, T* __restrict__ spool,
-    uint8_t* __restrict__ dpool, const T* __restrict__ slowpool, unsigned* __restrict__ flags,
+    const uint4* __restrict__ list, unsigned* __restrict__ count, unsigned* __restrict__ slot,
+    unsigned* __restrict__ slot_reset, const InstDev* __restrict__ inst, int ninst, const unsigned* __restrict__ wmin,
+    unsigned* __restrict__ wnext, unsigned* __restrict__ wreset, T* __restrict__ ttpool, T* __restrict__ kpool,
+    T* __restrict__ spool, uint8_t* __restrict__ dpool, const T* __restrict__ slowpool, unsigned* __restrict__ flags,
     unsigned* __restrict__ pend, unsigned long long* __restrict__ counters, T settle, unsigned chunk) {
     using S = TileShape<ND>;
     using L = TileLds<ND>;
@@ -189,9 +190,17 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(kTileThreads, kTileTh
     unsigned fmask = (unsigned)(cx < 2) | (unsigned)(cx >= E - 2) << 1 | (unsigned)(cy < 2) << 2 |
                      (unsigned)(cy >= E - 2) << 3;
     if (ND == 3) fmask |= (unsigned)(cz < 2) << 4 | (unsigned)(cz >= E - 2) << 5;
-    const unsigned nlist = count[kCntList];
+    const unsigned nlist = slot[kSlotList];
     unsigned tot_upd = 0u, tot_vchg = 0u, tot_kchg = 0u, tot_act = 0u, tot_sweeps = 0u;
-    if (blockIdx.x == 0 && n == 0 && nlist) atomicAdd(&count[kCntRounds], 1u);
+    if (blockIdx.x == 0) {
+        if (n == 0 && nlist) atomicAdd(&count[kCntRounds], 1u);
+        // the buffers of the round after next (last used by the previous round, which has ended)
+        for (int i = n; i < ninst; i += kTileThreads) wreset[i] = 0x7f800000u;
+        if (n == 0) {
+            slot_reset[kSlotList] = 0u;
+            slot_reset[kSlotCursor] = 0u;
+        }
+    }
     if (n == 0) {
         s_chg[0][NN] = EIK_INFF;
         s_chg[1][NN] = EIK_INFF;
@@ -210,7 +219,7 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(kTileThreads, kTileTh
         return r < 1u ? 1u : (r > chunk ? chunk : r);
     };
     unsigned claim = 0u, want = run_len(0u);
-    if (n == 0) claim = atomicAdd(&count[kCntCursor], want);
+    if (n == 0) claim = atomicAdd(&slot[kSlotCursor], want);
     unsigned rbase = (unsigned)__builtin_amdgcn_readfirstlane((int)claim);
     unsigned rlen = rbase < nlist ? min(want, nlist - rbase) : 0u, rdone = 0u;
     if ((unsigned)n < rlen) s_run[n] = list[rbase + n];
@@ -330,7 +339,7 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(kTileThreads, kTileTh
         // also wait for it: memory returns are counted in order)
         if (last) {
             want = run_len(rbase + rlen);
-            if (n == 0) claim = atomicAdd(&count[kCntCursor], want);
+            if (n == 0) claim = atomicAdd(&slot[kSlotCursor], want);
         }
         bool act = inside && !is_fixed(v0);
         float ptr = EIK_INFF;  // pending trigger key
@@ -482,9 +491,11 @@ int it = 0;
             unsigned* fl = flags + I.tile_base;
             unsigned* pd = pend + I.tile_base;
             const unsigned t0 = (unsigned)I.t[0], t01 = (unsigned)I.t[0] * (unsigned)I.t[1];
+            unsigned amin = 0x7f800000u;  // smallest pending value this tile hands out
             auto act_tile = [&](unsigned t, unsigned bit, unsigned v) {
                 atomicOr(&fl[t], bit);
                 atomicMin(&pd[t], v);
+                amin = min(amin, v);
             };
             // the x- neighbour sees this tile across its x+ face, and so on
             if (fmin[0] != 0x7f800000u && tix > 0) act_tile(tl - 1, 1u << 1, fmin[0]);
@@ -499,6 +510,9 @@ int it = 0;
                 act_tile(tl, kFullBit, fmin[6] == 0x7f800000u ? 0u : fmin[6]);
             else if (fmin[7] != 0x7f800000u)
                 act_tile(tl, kKeyBit, fmin[7]);
+            // the next round's lowest pending value of the instance (with the tiles the band
+            // selection left flagged, which k_band_select adds)
+            if (amin != 0x7f800000u) atomicMin(&wnext[e.y], amin);
         }
         tot_sweeps += (unsigned)it;
         tot_act++;
@@ -526,142 +540,119 @@ int it = 0;
     }
 }
 
-// Band scheduling over the active tiles.  A block scans kScanTiles consecutive tiles (kScanRows
-// rows of kScanThreads, so loads stay coalesced and every thread has all of its loads in flight at
-// once); such a range almost always lies in one instance, and the block then publishes its minimum
-// and flagged count with one atomic each (atomics on a few hot addresses from many blocks
-// serialise).  Blocks that straddle instances reduce per wave and row.
-constexpr int kScanThreads = 256;
-constexpr int kScanRows = 16;
-constexpr int64_t kScanTiles = (int64_t)kScanThreads * kScanRows;
+// Band scheduling over the active tiles, one tile per thread.  A block of kScanThreads
+// consecutive tiles almost always lies in one instance: its minimum and flagged count are then
+// reduced in LDS and published with one atomic each (atomics on a few hot addresses from every
+// wave serialise); blocks that straddle instances reduce per wave.
+constexpr int kScanThreads = 1024;
 
-// Pass 1: lowest pending value per instance, and the number of flagged tiles.
+// The lowest pending value per instance (into wmin, unless null) and the number of flagged tiles.
 __global__ void __launch_bounds__(kScanThreads) k_band_min(const unsigned* __restrict__ flags,
                                                          const unsigned* __restrict__ pend, int64_t ntiles,
                                                          const InstDev* __restrict__ inst, int ninst,
                                                          unsigned* __restrict__ wmin, unsigned* __restrict__ nflagged) {
     __shared__ unsigned s_min[kScanThreads / 64], s_cnt[kScanThreads / 64];
-    const int64_t b0 = (int64_t)blockIdx.x * kScanTiles;
+    const int64_t b0 = (int64_t)blockIdx.x * kScanThreads;
+    const int64_t t = b0 + threadIdx.x;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int ilo = find_inst(inst, ninst, b0);
-    const int ihi = find_inst(inst, ninst, (b0 + kScanTiles < ntiles ? b0 + kScanTiles : ntiles) - 1);
-    unsigned f[kScanRows], v[kScanRows];
-#pragma unroll
-    for (int r = 0; r < kScanRows; r++) {
-        const int64_t t = b0 + (int64_t)r * kScanThreads + threadIdx.x;
-        f[r] = t < ntiles ? flags[t] : 0u;
-        v[r] = t < ntiles ? pend[t] : 0x7f800000u;
+    const int ihi = find_inst(inst, ninst, (b0 + kScanThreads < ntiles ? b0 + kScanThreads : ntiles) - 1);
+    const bool on = t < ntiles && flags[t] != 0u;
+    unsigned long long m = __ballot(on);
+    const unsigned v = on ? pend[t] : 0x7f800000u;
+    const int i = on ? (ilo == ihi ? ilo : find_inst(inst, ninst, t)) : -1;
+    const unsigned wm = wmin && ilo == ihi ? wave_min_u32(v) : 0x7f800000u;
+    if (lane == 0) {
+        s_cnt[wave] = (unsigned)__popcll(m);
+        s_min[wave] = wm;
     }
-    unsigned cnt = 0u;
-    if (ilo == ihi) {
-        unsigned mn = 0x7f800000u;
-#pragma unroll
-        for (int r = 0; r < kScanRows; r++) {
-            cnt += f[r] ? 1u : 0u;
-            mn = f[r] ? min(mn, v[r]) : mn;
-        }
-        mn = wave_min_u32(mn);
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) cnt += (unsigned)__shfl_xor((int)cnt, o);
-        if (lane == 0) {
-            s_min[wave] = mn;
-            s_cnt[wave] = cnt;
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            unsigned m = s_min[0], c = s_cnt[0];
-            for (int w = 1; w < kScanThreads / 64; w++) {
-                m = min(m, s_min[w]);
-                c += s_cnt[w];
-            }
-            if (c) atomicAdd(nflagged, c);
-            if (m != 0x7f800000u) atomicMin(&wmin[ilo], m);
-        }
-        return;
-    }
-#pragma unroll 1
-    for (int r = 0; r < kScanRows; r++) {
-        const int64_t t = b0 + (int64_t)r * kScanThreads + threadIdx.x;
-        const bool on = f[r] != 0u;
-        unsigned long long m = __ballot(on);
-        cnt += on ? 1u : 0u;
-        const int i = on ? find_inst(inst, ninst, t) : -1;
+    if (wmin && ilo != ihi) {
         while (m) {
             const int leader = __ffsll((long long)m) - 1;
             const int li = __shfl(i, leader);
-            unsigned x = (i == li) ? v[r] : 0x7f800000u;
+            unsigned r = (i == li) ? v : 0x7f800000u;
 #pragma unroll
-            for (int o = 32; o > 0; o >>= 1) x = min(x, (unsigned)__shfl_xor((int)x, o));
-            if (lane == leader) atomicMin(&wmin[li], x);
+            for (int o = 32; o > 0; o >>= 1) r = min(r, (unsigned)__shfl_xor((int)r, o));
+            if (lane == leader) atomicMin(&wmin[li], r);
             m &= ~__ballot(i == li);
         }
     }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) cnt += (unsigned)__shfl_xor((int)cnt, o);
-    if (lane == 0 && cnt) atomicAdd(nflagged, cnt);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned c = 0u, mn = 0x7f800000u;
+        for (int w = 0; w < kScanThreads / 64; w++) {
+            c += s_cnt[w];
+            mn = min(mn, s_min[w]);
+        }
+        if (c) atomicAdd(nflagged, c);
+        if (mn != 0x7f800000u) atomicMin(&wmin[ilo], mn);
+    }
 }
 
-// Pass 2: list the active tiles whose pending value is within the instance's band of its lowest,
+// List the active tiles whose pending value is within the instance's band of its lowest (wmin),
 // in tile order, as (tile within its instance, instance, pending value, face bits), and clear
-// them; the others wait for a later round.  One list reservation per block.
+// them; the others wait for a later round, and their lowest pending value per instance goes to
+// the next round's minimum (wnext; the round kernel adds the tiles it activates).  One list
+// reservation per block.
 __global__ void __launch_bounds__(kScanThreads) k_band_select(unsigned* __restrict__ flags, unsigned* __restrict__ pend,
                                                             int64_t ntiles, const InstDev* __restrict__ inst,
                                                             int ninst, const unsigned* __restrict__ wmin,
-                                                            uint4* __restrict__ list, unsigned* __restrict__ count) {
-    constexpr int NW = kScanThreads / 64;
-    __shared__ unsigned s_wc[kScanRows * NW];
+                                                            unsigned* __restrict__ wnext, uint4* __restrict__ list,
+                                                            unsigned* __restrict__ slot) {
+    __shared__ unsigned s_wc[kScanThreads / 64], s_left[kScanThreads / 64];
     __shared__ unsigned s_base;
-    const int64_t b0 = (int64_t)blockIdx.x * kScanTiles;
+    const int64_t b0 = (int64_t)blockIdx.x * kScanThreads;
+    const int64_t t = b0 + threadIdx.x;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int ilo = find_inst(inst, ninst, b0);
-    const int ihi = find_inst(inst, ninst, (b0 + kScanTiles < ntiles ? b0 + kScanTiles : ntiles) - 1);
-    const float lim0 = __uint_as_float(wmin[ilo]) + (float)inst[ilo].band;
-    unsigned f[kScanRows], v[kScanRows];
-#pragma unroll
-    for (int r = 0; r < kScanRows; r++) {
-        const int64_t t = b0 + (int64_t)r * kScanThreads + threadIdx.x;
-        f[r] = t < ntiles ? flags[t] : 0u;
-        v[r] = t < ntiles ? pend[t] : 0x7f800000u;
+    const int ihi = find_inst(inst, ninst, (b0 + kScanThreads < ntiles ? b0 + kScanThreads : ntiles) - 1);
+    const unsigned f = t < ntiles ? flags[t] : 0u;
+    bool sel = false;
+    unsigned pv = 0x7f800000u;
+    int i = ilo;
+    if (f) {
+        if (ilo != ihi) i = find_inst(inst, ninst, t);
+        const float lim = __uint_as_float(wmin[i]) + (float)inst[i].band;
+        pv = pend[t];
+        sel = __uint_as_float(pv) <= lim;
     }
-    unsigned selm = 0u;  // bit r: this thread's tile of row r is selected
+    const bool left = f && !sel;
+    const unsigned long long m = __ballot(sel);
+    const unsigned lm = ilo == ihi ? wave_min_u32(left ? pv : 0x7f800000u) : 0x7f800000u;
+    if (lane == 0) {
+        s_wc[wave] = (unsigned)__popcll(m);
+        s_left[wave] = lm;
+    }
+    if (ilo != ihi) {
+        unsigned long long q = __ballot(left);
+        while (q) {
+            const int leader = __ffsll((long long)q) - 1;
+            const int li = __shfl(i, leader);
+            unsigned r = (left && i == li) ? pv : 0x7f800000u;
 #pragma unroll
-    for (int r = 0; r < kScanRows; r++) {
-        bool sel = false;
-        if (f[r]) {
-            float lim = lim0;
-            if (ilo != ihi) {
-                const int i = find_inst(inst, ninst, b0 + (int64_t)r * kScanThreads + threadIdx.x);
-                lim = __uint_as_float(wmin[i]) + (float)inst[i].band;
-            }
-            sel = __uint_as_float(v[r]) <= lim;
+            for (int o = 32; o > 0; o >>= 1) r = min(r, (unsigned)__shfl_xor((int)r, o));
+            if (lane == leader) atomicMin(&wnext[li], r);
+            q &= ~__ballot(left && i == li);
         }
-        selm |= (unsigned)sel << r;
-        const unsigned long long m = __ballot(sel);
-        if (lane == 0) s_wc[r * NW + wave] = (unsigned)__popcll(m);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        unsigned tot = 0u;
-        for (int k = 0; k < kScanRows * NW; k++) {  // row-major: the list keeps tile order
-            const unsigned c = s_wc[k];
-            s_wc[k] = tot;
+        unsigned tot = 0u, mn = 0x7f800000u;
+        for (int w = 0; w < kScanThreads / 64; w++) {
+            const unsigned c = s_wc[w];
+            s_wc[w] = tot;
             tot += c;
+            mn = min(mn, s_left[w]);
         }
-        s_base = tot ? atomicAdd(count, tot) : 0u;
+        s_base = tot ? atomicAdd(&slot[kSlotList], tot) : 0u;
+        if (mn != 0x7f800000u) atomicMin(&wnext[ilo], mn);
     }
     __syncthreads();
-#pragma unroll
-    for (int r = 0; r < kScanRows; r++) {
-        const bool sel = (selm >> r) & 1u;
-        const unsigned long long m = __ballot(sel);
-        if (sel) {
-            const int64_t t = b0 + (int64_t)r * kScanThreads + threadIdx.x;
-            const int i = ilo == ihi ? ilo : find_inst(inst, ninst, t);
-            const unsigned pos = s_base + s_wc[r * NW + wave] + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
-            list[pos] = make_uint4((unsigned)(t - inst[i].tile_base), (unsigned)i, v[r], f[r]);
-            flags[t] = 0u;
-            pend[t] = 0x7f800000u;
-        }
+    if (sel) {
+        const unsigned pos = s_base + s_wc[wave] + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
+        list[pos] = make_uint4((unsigned)(t - inst[i].tile_base), (unsigned)i, pv, f);
+        flags[t] = 0u;
+        pend[t] = 0x7f800000u;
     }
 }
 
@@ -858,15 +849,6 @@ __global__ void k_export(const T* __restrict__ src, double* __restrict__ dst, in
 
 static unsigned nblocks(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
-// Per-round reset: lowest pending value per instance, list length, tile cursor, flagged count.
-__global__ void k_round_reset(unsigned* __restrict__ wmin, int ninst, unsigned* __restrict__ cnt) {
-    for (int i = threadIdx.x; i < ninst; i += blockDim.x) wmin[i] = 0x7f800000u;
-    if (threadIdx.x == 0) {
-        cnt[kCntList] = 0u;
-        cnt[kCntCursor] = 0u;
-        cnt[kCntFlagged] = 0u;
-    }
-}
 
 
 
@@ -899,16 +881,20 @@ static double mix_settle() {
     return v;
 }
 
-// Rounds are enqueued in batches of kRoundBatch without host synchronisation: each round resets
-// the per-instance minimum, selects the tiles within the band, and runs the round kernel on a
-// fixed grid of resident blocks that take tiles from the device-side list.  After a batch the
-// host reads how many tiles are still flagged; none means converged.
+// Rounds are enqueued in batches of kRoundBatch without host synchronisation, two launches each:
+// k_band_select lists the tiles within the band of their instance's lowest pending value, and the
+// round kernel runs them on a fixed grid of resident blocks that take tiles from the list.  The
+// lowest pending value per instance for the next round is gathered on the way (the tiles the
+// selection leaves flagged, and the tiles the round activates), in three buffers used round
+// robin: round r reads buffer r % 3, fills (r + 1) % 3 and clears (r + 2) % 3, which round r - 1
+// was the last to use; the list length and cursor rotate the same way.  After a batch the host
+// reads how many tiles are still flagged; none means converged.
 template <int ND, typename T, bool SPH, bool MIX, bool NEAR>
 static int fim_run_t(eik_ctx* ctx, const InstDev* d_inst, int ninst, int64_t ntiles, const TTPools& P,
                      const T* slowpool, bool exact, FimStats* fs) {
     DevBuf& list = ctx->buf("fim_tile_list");
     DevBuf& wm = ctx->buf("fim_wmin");
-    if (list.ensure((size_t)ntiles * sizeof(uint4)) || wm.ensure((size_t)ninst * sizeof(unsigned)))
+    if (list.ensure((size_t)ntiles * sizeof(uint4)) || wm.ensure((size_t)3 * ninst * sizeof(unsigned)))
         return ctx->fail(EIK_ENOMEM, "out of device memory (tile list)");
     unsigned* flags = (unsigned*)ctx->buf("fim_tflags").p;
     unsigned* pend = (unsigned*)ctx->buf("fim_pend").p;
@@ -959,6 +945,14 @@ static int fim_run_t(eik_ctx* ctx, const InstDev* d_inst, int ninst, int64_t nti
     // write); otherwise the iteration goes on and verifies again.
     bool verifying = false;
     unsigned rounds_at_verify = 0u;
+    unsigned* W = (unsigned*)wm.p;      // 3 x ninst lowest pending values, by round mod 3
+    unsigned* SL = d_cnt + kCntSlots;   // 3 round slots (list length, cursor), by round mod 3
+    // round 0's lowest pending values from the seeded tiles; the other two buffers start empty
+    k_fill_u32<<<nblocks(3 * (int64_t)ninst, 256), 256, 0, st>>>(W, 3 * (int64_t)ninst, 0x7f800000u);
+    k_band_min<<<nblocks(ntiles, kScanThreads), kScanThreads, 0, st>>>(flags, pend, ntiles, d_inst, ninst, W,
+                                                                      d_cnt + kCntFlagged);
+    CK(hipGetLastError());
+    int64_t round = 0;
     for (int64_t batch = 0;; batch++) {
         if (batch >= maxbatches) return ctx->fail(EIK_ENOCONV, "traveltime solve did not converge");
         // the tolerance grows tenfold every 500 rounds while the cycle persists
@@ -968,23 +962,23 @@ static int fim_run_t(eik_ctx* ctx, const InstDev* d_inst, int ninst, int64_t nti
             for (int64_t b = settle_batch + 500 / kRoundBatch; b <= batch; b += 500 / kRoundBatch) settle *= T(10);
             stt.settled = std::max(stt.settled, (int)(1 + (batch - settle_batch) / (500 / kRoundBatch)));
         }
-        for (int r = 0; r < kRoundBatch; r++) {
-            k_round_reset<<<1, 256, 0, st>>>((unsigned*)wm.p, ninst, d_cnt);
-            k_band_min<<<nblocks(ntiles, (int)kScanTiles), kScanThreads, 0, st>>>(flags, pend, ntiles, d_inst, ninst,
-                                                                              (unsigned*)wm.p, d_cnt + kCntFlagged);
-            k_band_select<<<nblocks(ntiles, (int)kScanTiles), kScanThreads, 0, st>>>(flags, pend, ntiles, d_inst, ninst,
-                                                               (const unsigned*)wm.p, (uint4*)list.p,
-                                                               d_cnt + kCntList);
+        for (int r = 0; r < kRoundBatch; r++, round++) {
+            const int c = (int)(round % 3), nx = (int)((round + 1) % 3), rs = (int)((round + 2) % 3);
+            k_band_select<<<nblocks(ntiles, kScanThreads), kScanThreads, 0, st>>>(
+                flags, pend, ntiles, d_inst, ninst, W + (size_t)c * ninst, W + (size_t)nx * ninst, (uint4*)list.p,
+                SL + c * kSlotWords);
             CK(hipEventRecord(ctx->kev[2 * r], st));
-            k_fim_round<ND, T, SPH, MIX, NEAR><<<g, kTileThreads, 0, st>>>((const uint4*)list.p, d_cnt, d_inst, (const unsigned*)wm.p,
-                                                          (T*)P.tt, (T*)P.k, (T*)P.s, P.d, slowpool, flags, pend,
-                                                          d_c64, settle, claim_chunk());
+            k_fim_round<ND, T, SPH, MIX, NEAR><<<g, kTileThreads, 0, st>>>(
+                (const uint4*)list.p, d_cnt, SL + c * kSlotWords, SL + rs * kSlotWords, d_inst, ninst,
+                W + (size_t)c * ninst, W + (size_t)nx * ninst, W + (size_t)rs * ninst, (T*)P.tt, (T*)P.k, (T*)P.s, P.d,
+                slowpool, flags, pend, d_c64, settle, claim_chunk());
             CK(hipGetLastError());
             CK(hipEventRecord(ctx->kev[2 * r + 1], st));
         }
-        k_round_reset<<<1, 256, 0, st>>>((unsigned*)wm.p, ninst, d_cnt);
-        k_band_min<<<nblocks(ntiles, (int)kScanTiles), kScanThreads, 0, st>>>(flags, pend, ntiles, d_inst, ninst,
-                                                                          (unsigned*)wm.p, d_cnt + kCntFlagged);
+        // the tiles still flagged (the next round's lowest pending values are already gathered)
+        CK(hipMemsetAsync(d_cnt + kCntFlagged, 0, sizeof(unsigned), st));
+        k_band_min<<<nblocks(ntiles, kScanThreads), kScanThreads, 0, st>>>(flags, pend, ntiles, d_inst, ninst, nullptr,
+                                                                          d_cnt + kCntFlagged);
         CK(hipMemcpyAsync(h_cnt, d_cnt, kSmallBytes, hipMemcpyDeviceToHost, st));
         CK(hipStreamSynchronize(st));
         for (int r = 0; r < kRoundBatch; r++) {
@@ -1000,6 +994,8 @@ static int fim_run_t(eik_ctx* ctx, const InstDev* d_inst, int ninst, int64_t nti
             if (verifying && h_cnt[kCntRounds] <= rounds_at_verify + 1u) break;  // verified
             k_fill_u32<<<nblocks(ntiles, 256), 256, 0, st>>>(flags, ntiles, kFullBit);
             k_fill_u32<<<nblocks(ntiles, 256), 256, 0, st>>>(pend, ntiles, 0u);
+            // every tile now pends at 0: the next round's lowest pending value
+            k_fill_u32<<<nblocks(ninst, 256), 256, 0, st>>>(W + (size_t)(round % 3) * ninst, ninst, 0u);
             CK(hipGetLastError());
             verifying = true;
             rounds_at_verify = h_cnt[kCntRounds];

@@ -345,10 +345,12 @@ struct DevBuf {
 constexpr int kRoundBatch = 8;
 // rounds after which a traveltime solve starts settling (see fim_run_t)
 constexpr int kSettleRound = 2000;
-// device counters (ctx->d_small, 32-bit words): list length, list cursor, flagged tiles, sweeps,
-// non-empty rounds; from word kCnt64 on, 64-bit node updates, changed updates, tile activations
+// device counters (ctx->d_small, 32-bit words): flagged tiles, sweeps, non-empty rounds; from word
+// kCntSlots on, three round slots of kSlotWords words (list length, list cursor; a round uses slot
+// round mod 3); from word kCnt64 on, 64-bit node updates, changed updates, tile activations
 constexpr size_t kSmallBytes = 256;
-constexpr int kCntList = 0, kCntCursor = 1, kCntFlagged = 2, kCntSweeps = 3, kCntRounds = 4, kCnt64 = 8;
+constexpr int kCntFlagged = 2, kCntSweeps = 3, kCntRounds = 4, kCntSlots = 8, kCnt64 = 16;
+constexpr int kSlotList = 0, kSlotCursor = 1, kSlotWords = 2;
 
 struct eik_ctx {
     int device = 0;

@@ -66,16 +66,16 @@ size_t forward_bytes(const eik_model* M, const SourceSet& S, const eik_params& P
         b += 4 * al((size_t)nfine * sizeof(double)) + al((size_t)nfine);
         if (M->g.kind == EIK_GRID3D_SPHERE) b += al((size_t)sinth * sizeof(double));
         b += al((size_t)corn * sizeof(CornerDev)) + al((size_t)nsrc * sizeof(int)) + al((size_t)nsrc * cap * sizeof(SrcPt));
-        b += al((size_t)ntf * sizeof(uint4)) + al((size_t)nsrc * sizeof(unsigned));
+        b += al((size_t)ntf * sizeof(uint4)) + al((size_t)3 * nsrc * sizeof(unsigned));
         maxt = std::max(maxt, ntf);
         maxat = std::max(maxat, natf);
     } else {
         b += al((size_t)nsrc * (1 << nd) * sizeof(CornerDev));
     }
     // adjoint-tile flags (cleared by every forward call), forward tile flags and pending values,
-    // the coarse solve's tile list and per-instance minimum
+    // the coarse solve's tile list and per-instance minima (three, used round robin)
     b += al((size_t)maxat) + al((size_t)maxat * sizeof(unsigned)) + 2 * al((size_t)maxt * sizeof(unsigned));
-    b += al((size_t)nsrc * tiles * sizeof(uint4)) + al((size_t)nsrc * sizeof(unsigned));
+    b += al((size_t)nsrc * tiles * sizeof(uint4)) + al((size_t)3 * nsrc * sizeof(unsigned));
     const int64_t nrec = S.nrec();
     if (nrec > 0) b += al((size_t)nrec * nd * sizeof(double)) + al((size_t)nrec * sizeof(int)) + al((size_t)nrec * sizeof(double));
     if (ttgrid) b += al((size_t)N * sizeof(double));
