@@ -32,8 +32,8 @@ TILE_NODES_3D = 64  # forward tiles are 4x4x4 (one wavefront each)
 # algorithmic bytes per node-visit: slowness + traveltime read + traveltime write (FP64 pools for
 # fp64 and mixed, FP32 pools for fp32)
 BYTES_PER_VISIT = {"fp64": 24, "mixed": 24, "fp32": 12}
-KERNELS = {"fp64": "k_fim_round<3,double,false,false>", "mixed": "k_fim_round<3,double,false,true>",
-           "fp32": "k_fim_round<3,float,false,false>"}
+KERNELS = {"fp64": "k_fim_round<3,double,false,false,false>", "mixed": "k_fim_round<3,double,false,true,false>",
+           "fp32": "k_fim_round<3,float,false,false,false>"}
 
 
 def run_steps(model, src, recs, ep, steps, warmup):
