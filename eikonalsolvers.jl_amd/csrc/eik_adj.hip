@@ -70,8 +70,10 @@ __device__ __forceinline__ bool before_stop(const InstDev& I, const Key<T>& k) {
 }
 
 // Codes and G_d of every node: replay of the final stencil evaluation (eikmainfwd.jl:794-809)
-// and 2*(D_d t)_i accumulated over the stencil in acceptance order (far node first).
-template <int ND, typename T, bool MIX>
+// and 2*(D_d t)_i accumulated over the stencil in acceptance order (far node first).  NEAR: the
+// instances may have an FP64 region (an unrefined mixed solve), whose nodes replay the FP64
+// stencil; compiled only where it can occur (the second replay costs registers).
+template <int ND, typename T, bool MIX, bool NEAR>
 __global__ void __launch_bounds__(kAB) k_adj_prep(const InstDev* __restrict__ inst, TTPools pools,
                                                   const T* __restrict__ slowpool, uint16_t* __restrict__ code_out,
                                                   double* __restrict__ G, int64_t gstride) {
@@ -133,7 +135,7 @@ __global__ void __launch_bounds__(kAB) k_adj_prep(const InstDev* __restrict__ in
             int code[3] = {0, 0, 0};
             Key<T> kz;
             bool near64 = false;  // the forward's FP64 near-source stencil
-            if constexpr (MIX) {
+            if constexpr (MIX && NEAR) {
                 near64 = in_fp64_region(I, g);
                 if (near64) {
                     Metric<double> m64;
@@ -492,16 +494,29 @@ __global__ void k_adj_rhs(const double* __restrict__ rec, const int* __restrict_
 
 // per-source misfit sum in receiver order, then the total in source order (calcgradsomesrc,
 // eikmaingrad.jl:237: totmisf += 0.5*misfit1src)
-__global__ void k_misfit(const double* __restrict__ mterm, const int64_t* __restrict__ rec_off, int64_t nsrc,
-                         double* __restrict__ out) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+// One block: each source's receiver terms are summed in receiver order by one thread, and the
+// sources' halves are added in source order by thread 0, chunk by chunk: the summation order of
+// a serial loop over sources and receivers, so the same result bit for bit.
+constexpr int kMisfitThreads = 256;
+__global__ void __launch_bounds__(kMisfitThreads) k_misfit(const double* __restrict__ mterm,
+                                                            const int64_t* __restrict__ rec_off, int64_t nsrc,
+                                                            double* __restrict__ out) {
+    __shared__ double s_m[kMisfitThreads];
     double tot = 0.0;
-    for (int64_t s = 0; s < nsrc; s++) {
+    for (int64_t base = 0; base < nsrc; base += kMisfitThreads) {
+        const int64_t s = base + threadIdx.x;
         double m = 0.0;
-        for (int64_t r = rec_off[s]; r < rec_off[s + 1]; r++) m += mterm[r];
-        tot += 0.5 * m;
+        if (s < nsrc)
+            for (int64_t r = rec_off[s]; r < rec_off[s + 1]; r++) m += mterm[r];
+        s_m[threadIdx.x] = m;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const int64_t cnt = nsrc - base < kMisfitThreads ? nsrc - base : kMisfitThreads;
+            for (int64_t k = 0; k < cnt; k++) tot += 0.5 * s_m[k];
+        }
+        __syncthreads();
     }
-    out[0] = tot;
+    if (threadIdx.x == 0) out[0] = tot;
 }
 
 // source points of every instance: dpsiup = sum_i A^S_iq lambda_i, dpsidus = (P_A^T fact)_q
@@ -883,14 +898,14 @@ static int adj_run(eik_ctx* ctx, int nd, int prec, const InstDev* d_inst, int ni
                    : adj_run_t<3, float>(ctx, d_inst, ninst, ntiles, code, G, gstride, b, lam, (const float*)ttpool, fs);
 }
 
-template <typename T, bool MIX>
+template <typename T, bool MIX, bool NEAR = false>
 static void launch_prep(int nd, hipStream_t st, const InstDev* inst, int ninst, const TTPools& pools,
                         const void* slow, uint16_t* code, double* G, int64_t gstride) {
     dim3 grid(256, (unsigned)ninst);
     if (nd == 2)
-        k_adj_prep<2, T, MIX><<<grid, kAB, 0, st>>>(inst, pools, (const T*)slow, code, G, gstride);
+        k_adj_prep<2, T, MIX, NEAR><<<grid, kAB, 0, st>>>(inst, pools, (const T*)slow, code, G, gstride);
     else
-        k_adj_prep<3, T, MIX><<<grid, kAB, 0, st>>>(inst, pools, (const T*)slow, code, G, gstride);
+        k_adj_prep<3, T, MIX, NEAR><<<grid, kAB, 0, st>>>(inst, pools, (const T*)slow, code, G, gstride);
 }
 
 static void taper_box(const eik_grid& g, const double* src, int R, TaperDev& T) {
@@ -948,6 +963,8 @@ int gradient_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, const eik_par
     const TTPools cpools{ttc, ctx->buf("key_K").p, ctx->buf("key_S").p, (uint8_t*)ctx->buf("key_D").p};
     if (prec == EIK_PREC_FP64)
         launch_prep<double, false>(nd, st, dci, (int)nsrc, cpools, slow, (uint16_t*)code_c.p, (double*)G_c.p, P_c);
+    else if (prec == EIK_PREC_MIXED && !refine)  // unrefined mixed: FP64 regions around the sources
+        launch_prep<double, true, true>(nd, st, dci, (int)nsrc, cpools, slow, (uint16_t*)code_c.p, (double*)G_c.p, P_c);
     else if (prec == EIK_PREC_MIXED)
         launch_prep<double, true>(nd, st, dci, (int)nsrc, cpools, slow, (uint16_t*)code_c.p, (double*)G_c.p, P_c);
     else
@@ -987,7 +1004,7 @@ int gradient_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, const eik_par
                                                             (const double*)dso.p, (double*)b_c.p, (double*)dmt.p, flags, apend);
         CK(hipGetLastError());
     }
-    k_misfit<<<1, 1, 0, st>>>((const double*)dmt.p, (const int64_t*)droff.p, nsrc, (double*)dmis.p);
+    k_misfit<<<1, kMisfitThreads, 0, st>>>((const double*)dmt.p, (const int64_t*)droff.p, nsrc, (double*)dmis.p);
     CK(hipMemcpyAsync(misf, dmis.p, sizeof(double), hipMemcpyDeviceToHost, st));
 
     FimStats ast{};
