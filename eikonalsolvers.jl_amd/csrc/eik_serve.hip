@@ -45,6 +45,8 @@ struct Req {
     double stats[kReqStats] = {};  // the solver statistics of the call that ran it
 };
 
+// longest a blocked request lets smaller ones pass it (EIK_SERVER_BYPASS_MS overrides, read when
+// the server is created: a test hook)
 constexpr double kMaxBypassMs = 2000.0;
 constexpr int kMaxBatch = 16;
 constexpr double kBatchSpread = 2.0;
@@ -65,6 +67,7 @@ struct eik_server {
     int64_t budget = 0;      // fixed budget in bytes; 0: free device memory less the reserve
     int64_t budget_now = 0;  // the budget the last admission used
     int64_t reserve = 0;
+    double bypass_ms = kMaxBypassMs;
     std::vector<eik_ctx*> slots;
     std::vector<int64_t> held;  // per slot: device bytes its pools keep
     std::vector<int64_t> run;   // per slot: working set of the request it runs (0: idle)
@@ -210,7 +213,7 @@ static void worker(eik_server* S, int slot) {
                     S->queue.erase(it);
                     return true;
                 }
-                if (waited > kMaxBypassMs) break;  // the oldest request goes next
+                if (waited > S->bypass_ms) break;  // the oldest request goes next
                 ++it;
             }
             return false;
@@ -290,6 +293,7 @@ int eik_server_create(int32_t device, int32_t nslots, int64_t mem_budget, eik_se
     S->budget = mem_budget;
     S->held.assign((size_t)nslots, 0);
     S->run.assign((size_t)nslots, 0);
+    if (const char* e = getenv("EIK_SERVER_BYPASS_MS")) S->bypass_ms = std::max(0.0, atof(e));
     for (int i = 0; i < nslots; i++) {
         eik_ctx* c = nullptr;
         const int rc = eik_ctx_create(device, &c);

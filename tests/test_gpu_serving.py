@@ -296,3 +296,82 @@ def test_default_budget_follows_free_device_memory(eik):
     assert 0 < b0 <= free.value - reserve + (64 << 20)
     assert 0 < st["budget"] <= total.value - reserve
     m.close()
+
+
+def _tt_request_2d(n, nsrc, seed):
+    """A 2D n x n traveltime request with nsrc sources and 16 top receivers each."""
+    from eikonalsolvers.grids import Grid2DCart
+    from eikonalsolvers.synthetic import uniform
+
+    grd = Grid2DCart(hgrid=1.0, cooinit=(0.0, 0.0), grsize=(n, n))
+    L = (n - 1) * 1.0
+    src = (0.1 + 0.8 * uniform(seed, 2 * nsrc).reshape(nsrc, 2)) * L
+    rec = np.column_stack([np.linspace(0.0, L, 16), np.zeros(16)])
+    return grd, src, [rec.copy() for _ in range(nsrc)]
+
+
+def test_shared_launch_takes_only_requests_within_twice_the_working_set(eik):
+    """With the only slot busy, three traveltime requests on one model queue: the first is
+    admitted with the one whose working set is within 2x of its own (one shared call), not the
+    one 4x larger, which runs in a call of its own; results equal separate calls."""
+    from eikonalsolvers.serving import request_bytes
+
+    ep = ExtraParams()
+    grd, src, recs = _tt_request_2d(300, 24, 61001)
+    model = eik.EikModel(smooth_velocity(grd.grsize, 61000), grd)
+    sizes = {"a": 4, "b": 7, "c": 24}
+    b = {k: request_bytes(model, v, "traveltime", extraparams=ep) for k, v in sizes.items()}
+    assert 1.0 < b["b"] / b["a"] <= 2.0 and b["c"] / b["a"] > 2.0 and b["c"] / b["b"] > 2.0
+    r, m_blk, _ = _same_gradient_requests(eik, 1)
+    with eik.Server(nslots=1) as srv:
+        blk = srv.submit_gradient(m_blk, r["src"], r["recs"], r["pickobs"], r["stdobs"], "gradvel")
+        hs = {k: srv.submit_traveltime(model, src[:v], recs[:v], extraparams=ep) for k, v in sizes.items()}
+        out = {k: h.result() for k, h in hs.items()}
+        blk.result()
+    calls = {k: h.stats["requests_in_call"] for k, h in hs.items()}
+    print(f"working sets {b}, requests per call {calls}")
+    assert calls == {"a": 2, "b": 2, "c": 1}
+    for k, v in sizes.items():
+        for pa, pb in zip(model.traveltime(src[:v], recs[:v], extraparams=ep), out[k]):
+            np.testing.assert_array_equal(pb, pa)
+    model.close()
+    m_blk.close()
+
+
+@pytest.mark.parametrize("bypass_ms", [None, 0])
+def test_blocked_request_is_passed_only_within_the_bypass_time(eik, monkeypatch, bypass_ms):
+    """Two slots, a budget of 1.5 gradient working sets: while one gradient runs, a second waits
+    for memory and a small request that fits is admitted ahead of it (default: a blocked request
+    may be passed for 2 s).  With the bypass time set to 0 (EIK_SERVER_BYPASS_MS, read when the
+    server is created) the small request waits until the blocked one has started."""
+    from eikonalsolvers.serving import request_bytes
+
+    if bypass_ms is not None:
+        monkeypatch.setenv("EIK_SERVER_BYPASS_MS", str(bypass_ms))
+    ep = ExtraParams()
+    r, m, _ = _same_gradient_requests(eik, 1)
+    w = request_bytes(m, len(r["src"]), "gradient", extraparams=ep)
+    grd, src, recs = _tt_request_2d(120, 2, 62001)
+    ms = eik.EikModel(smooth_velocity(grd.grsize, 62000), grd)
+    assert request_bytes(ms, 2, "traveltime", extraparams=ep) <= 0.4 * w
+    with eik.Server(nslots=2, mem_budget=int(1.5 * w)) as srv:
+        t_sub = []
+        hs = []
+        for kind in ("g1", "g2", "small"):
+            t_sub.append(time.perf_counter())
+            if kind == "small":
+                hs.append(srv.submit_traveltime(ms, src, recs, extraparams=ep))
+            else:
+                hs.append(srv.submit_gradient(m, r["src"], r["recs"], r["pickobs"], r["stdobs"], "gradvel",
+                                              extraparams=ep))
+        for h in hs:
+            h.result()
+    start = [t + h.queue_ms / 1e3 for t, h in zip(t_sub, hs)]
+    print(f"bypass {bypass_ms}: starts after the first submission (ms) "
+          f"{[round(1e3 * (x - t_sub[0]), 1) for x in start]}")
+    if bypass_ms is None:
+        assert start[2] < start[1], "the small request should pass the blocked gradient"
+    else:
+        assert start[2] >= start[1] - 1e-4, "the small request passed the blocked gradient"
+    m.close()
+    ms.close()
