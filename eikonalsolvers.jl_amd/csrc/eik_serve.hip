@@ -170,7 +170,8 @@ static void reject(eik_server* S, Req& q, int64_t B) {
 
 // Whether a request of b bytes can run on idle slot `slot` within budget B; releases the pools
 // of idle slots when that makes the room.  Called with the server lock held (an idle slot's
-// context is then in no call).
+// context is then in no call).  A release waits for the device (hipFree synchronises), so the
+// other slots' running calls finish first; it happens only when the budget binds.
 static bool admit(eik_server* S, int slot, int64_t b, int64_t B) {
     if (footprint_with(S, slot, b) <= B) return true;
     int64_t floor = b;  // the footprint with every idle slot's pools released
