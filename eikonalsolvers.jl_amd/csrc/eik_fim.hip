@@ -545,6 +545,40 @@ int it = 0;
 // reduced in LDS and published with one atomic each (atomics on a few hot addresses from every
 // wave serialise); blocks that straddle instances reduce per wave.
 constexpr int kScanThreads = 1024;
+// Instances whose first tile ids a scan block stages in LDS (one coalesced read instead of a
+// binary search through global memory, one dependent load per step); beyond it the search reads
+// the instance table itself.
+constexpr int kScanLdsInst = 1024;
+
+__device__ __forceinline__ int find_tile_base(const int64_t* __restrict__ tb, int ninst, int64_t gid) {
+    int lo = 0, hi = ninst - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tb[mid] <= gid)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// The instance a tile belongs to: from the LDS copy of the first tile ids when it holds them all.
+struct InstFinder {
+    const InstDev* inst;
+    const int64_t* tb;  // LDS, or null
+    int ninst;
+    __device__ int operator()(int64_t gid) const {
+        return tb ? find_tile_base(tb, ninst, gid) : find_inst(inst, ninst, gid);
+    }
+};
+
+__device__ __forceinline__ InstFinder stage_tile_bases(const InstDev* __restrict__ inst, int ninst, int64_t* s_tb) {
+    const bool lds = ninst <= kScanLdsInst;
+    if (lds)
+        for (int k = threadIdx.x; k < ninst; k += blockDim.x) s_tb[k] = inst[k].tile_base;
+    __syncthreads();
+    return InstFinder{inst, lds ? s_tb : nullptr, ninst};
+}
 
 // The lowest pending value per instance (into wmin, unless null) and the number of flagged tiles.
 __global__ void __launch_bounds__(kScanThreads) k_band_min(const unsigned* __restrict__ flags,
@@ -552,15 +586,20 @@ __global__ void __launch_bounds__(kScanThreads) k_band_min(const unsigned* __res
                                                          const InstDev* __restrict__ inst, int ninst,
                                                          unsigned* __restrict__ wmin, unsigned* __restrict__ nflagged) {
     __shared__ unsigned s_min[kScanThreads / 64], s_cnt[kScanThreads / 64];
+    __shared__ int64_t s_tb[kScanLdsInst];
     const int64_t b0 = (int64_t)blockIdx.x * kScanThreads;
     const int64_t t = b0 + threadIdx.x;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int ilo = find_inst(inst, ninst, b0);
-    const int ihi = find_inst(inst, ninst, (b0 + kScanThreads < ntiles ? b0 + kScanThreads : ntiles) - 1);
-    const bool on = t < ntiles && flags[t] != 0u;
+    // the tile's words are read before the instance table is searched (no dependence)
+    const unsigned fl = t < ntiles ? flags[t] : 0u;
+    const unsigned pd = t < ntiles ? pend[t] : 0x7f800000u;
+    const InstFinder find = stage_tile_bases(inst, ninst, s_tb);
+    const int ilo = find(b0);
+    const int ihi = find((b0 + kScanThreads < ntiles ? b0 + kScanThreads : ntiles) - 1);
+    const bool on = fl != 0u;
     unsigned long long m = __ballot(on);
-    const unsigned v = on ? pend[t] : 0x7f800000u;
-    const int i = on ? (ilo == ihi ? ilo : find_inst(inst, ninst, t)) : -1;
+    const unsigned v = on ? pd : 0x7f800000u;
+    const int i = on ? (ilo == ihi ? ilo : find(t)) : -1;
     const unsigned wm = wmin && ilo == ihi ? wave_min_u32(v) : 0x7f800000u;
     if (lane == 0) {
         s_cnt[wave] = (unsigned)__popcll(m);
@@ -601,19 +640,27 @@ __global__ void __launch_bounds__(kScanThreads) k_band_select(unsigned* __restri
                                                             unsigned* __restrict__ slot) {
     __shared__ unsigned s_wc[kScanThreads / 64], s_left[kScanThreads / 64];
     __shared__ unsigned s_base;
+    __shared__ int64_t s_tb[kScanLdsInst];
     const int64_t b0 = (int64_t)blockIdx.x * kScanThreads;
     const int64_t t = b0 + threadIdx.x;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int ilo = find_inst(inst, ninst, b0);
-    const int ihi = find_inst(inst, ninst, (b0 + kScanThreads < ntiles ? b0 + kScanThreads : ntiles) - 1);
+    // the tile's words are read before the instance table is searched (no dependence)
     const unsigned f = t < ntiles ? flags[t] : 0u;
+    const unsigned pd = t < ntiles ? pend[t] : 0x7f800000u;
+    const InstFinder find = stage_tile_bases(inst, ninst, s_tb);
+    const int ilo = find(b0);
+    const int ihi = find((b0 + kScanThreads < ntiles ? b0 + kScanThreads : ntiles) - 1);
+    const float lim0 = __uint_as_float(wmin[ilo]) + (float)inst[ilo].band;
     bool sel = false;
     unsigned pv = 0x7f800000u;
     int i = ilo;
     if (f) {
-        if (ilo != ihi) i = find_inst(inst, ninst, t);
-        const float lim = __uint_as_float(wmin[i]) + (float)inst[i].band;
-        pv = pend[t];
+        float lim = lim0;
+        if (ilo != ihi) {
+            i = find(t);
+            lim = __uint_as_float(wmin[i]) + (float)inst[i].band;
+        }
+        pv = pd;
         sel = __uint_as_float(pv) <= lim;
     }
     const bool left = f && !sel;
