@@ -215,13 +215,20 @@ static_assert(AdjTile<2>::TX == adj_tile_edge(2) && AdjTile<3>::TX == adj_tile_e
 
 // one adjoint round over the listed tiles; a neighbour woken by a changed row gets that row's
 // traveltime (rounded up) as its pending value
+// Adjoint round counters (ctx->d_small words, while the adjoint runs): two list-length slots used
+// by round parity, then sweeps, tile activations and non-empty rounds.
+constexpr int kAdjSlots = 0, kAdjSweeps = 2, kAdjActs = 3, kAdjRounds = 4, kAdjWords = 5;
+constexpr int kAdjBatch = 8;  // adjoint rounds enqueued between two host checks
+
 template <int ND, typename T>
-__global__ void __launch_bounds__(kAB) k_adj_round(const int64_t* __restrict__ list, const InstDev* __restrict__ inst,
-                                                   int ninst, const uint16_t* __restrict__ codep,
+__global__ void __launch_bounds__(kAB) k_adj_round(const int64_t* __restrict__ list, const unsigned* __restrict__ slot,
+                                                   unsigned* __restrict__ slot_reset, unsigned* __restrict__ wreset,
+                                                   const InstDev* __restrict__ inst, int ninst,
+                                                   const uint16_t* __restrict__ codep,
                                                    const double* __restrict__ Gp, int64_t gstride,
                                                    const double* __restrict__ bp, double* __restrict__ lamp,
                                                    const T* __restrict__ ttpool, uint8_t* __restrict__ flags,
-                                                   unsigned* __restrict__ apend, unsigned* __restrict__ nsweeps) {
+                                                   unsigned* __restrict__ apend, unsigned* __restrict__ stats) {
 #pragma clang fp contract(off)
     using S = AdjTile<ND>;
     constexpr int NL = S::LX * S::LY * S::LZ;
@@ -230,161 +237,177 @@ __global__ void __launch_bounds__(kAB) k_adj_round(const int64_t* __restrict__ l
     __shared__ uint16_t s_code[NL];
     __shared__ unsigned s_fmax[7];  // per face (and whole tile): largest traveltime of a changed row
     const int tid = threadIdx.x;
-    const int64_t gid = list[blockIdx.x];
-    int lo = 0, hi = ninst - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (inst[mid].atile_base <= gid)
-            lo = mid;
-        else
-            hi = mid - 1;
-    }
-    const InstDev I = inst[lo];
-    const int64_t tl = gid - I.atile_base;
-    const int tix = (int)(tl % I.at[0]), tiy = (int)((tl / I.at[0]) % I.at[1]);
-    const int tiz = ND == 3 ? (int)(tl / ((int64_t)I.at[0] * I.at[1])) : 0;
-    const int ox = tix * S::TX, oy = tiy * S::TY, oz = tiz * S::TZ;
-    const int64_t nx = I.n[0], nxy = (int64_t)I.n[0] * I.n[1];
-    const uint16_t* code = codep + I.tt_off;
-    const double* b = bp + I.tt_off;
-    double* lam = lamp + I.tt_off;
-    if (tid < 7) s_fmax[tid] = 0u;
-    for (int c = tid; c < NL; c += kAB) {
-        const int lx = c % S::LX, ly = (c / S::LX) % S::LY, lz = ND == 3 ? c / (S::LX * S::LY) : 0;
-        int nout = (lx < S::H || lx >= S::H + S::TX) + (ly < S::H || ly >= S::H + S::TY);
-        if (ND == 3) nout += (lz < S::H || lz >= S::H + S::TZ);
-        double lv = 0.0;
-        uint16_t cv = 0x1B6;  // all codes zero ((0+2) in every 3-bit field)
-        double gv[3] = {0.0, 0.0, 0.0};
-        if (nout <= 1) {
-            const int gx = ox + lx - S::H, gy = oy + ly - S::H, gz = ND == 3 ? oz + lz - S::H : 0;
-            if (gx >= 0 && gx < I.n[0] && gy >= 0 && gy < I.n[1] && (ND == 2 || (gz >= 0 && gz < I.n[2]))) {
-                const int64_t l = gx + nx * gy + nxy * gz;
-                lv = lam[l];
-                cv = code[l];
-#pragma unroll
-                for (int d = 0; d < ND; d++) gv[d] = Gp[d * gstride + I.tt_off + l];
+    const unsigned nlist = *slot;
+    if (blockIdx.x == 0) {
+        // the next round's list length and per-instance maxima (last used by the previous round)
+        for (int i = tid; i < ninst; i += kAB) wreset[i] = 0u;
+        if (tid == 0) {
+            *slot_reset = 0u;
+            if (nlist) {
+                atomicAdd(&stats[kAdjRounds], 1u);
+                atomicAdd(&stats[kAdjActs], nlist);
             }
         }
-        s_lam[c] = lv;
-        s_code[c] = cv;
-#pragma unroll
-        for (int d = 0; d < ND; d++) s_G[d][c] = gv[d];
     }
-    __syncthreads();
-
-    constexpr int NS = 4 * ND;  // candidate rows: axis d, offset +-1, +-2
-    double w[S::NP][NS];
-    int off[NS];
-    double diag[S::NP], bj[S::NP], l0[S::NP];
-    bool act[S::NP];
-    int lidx[S::NP], gxyz[S::NP][3];
-    const int strides[3] = {1, S::LX, S::LX * S::LY};
-#pragma unroll
-    for (int k = 0; k < NS; k++) {
-        const int d = k / 4, o = (k % 4 < 2) ? ((k % 2) ? -1 : 1) : ((k % 2) ? -2 : 2);
-        off[k] = o * strides[d];
-    }
-#pragma unroll
-    for (int p = 0; p < S::NP; p++) {
-        const int lx = tid % S::TX, ly = (tid / S::TX) % S::TY;
-        const int lz = ND == 3 ? (tid / (S::TX * S::TY)) + p * (S::TZ / S::NP) : 0;
-        const int g[3] = {ox + lx, oy + ly, oz + lz};
-        gxyz[p][0] = g[0];
-        gxyz[p][1] = g[1];
-        gxyz[p][2] = g[2];
-        lidx[p] = (lx + S::H) + S::LX * ((ly + S::H) + S::LY * (ND == 3 ? lz + S::H : 0));
-        const bool inside = g[0] < I.n[0] && g[1] < I.n[1] && (ND == 2 || g[2] < I.n[2]);
-        const uint16_t cj = s_code[lidx[p]];
-        bool hasrow = false;
-#pragma unroll
-        for (int d = 0; d < ND; d++) hasrow = hasrow || unpack_code(cj, d) != 0;
-        act[p] = inside && hasrow;
-        diag[p] = 1.0;
-        bj[p] = 0.0;
-#pragma unroll
-        for (int k = 0; k < NS; k++) w[p][k] = 0.0;
-        if (act[p]) {
-            double Gj[3] = {0, 0, 0};
-#pragma unroll
-            for (int d = 0; d < ND; d++) Gj[d] = s_G[d][lidx[p]];
-            diag[p] = adj_diag<ND>(I, g, cj, Gj);
-            bj[p] = b[g[0] + nx * g[1] + nxy * g[2]];
-#pragma unroll
-            for (int k = 0; k < NS; k++) {
-                const int d = k / 4, o = (k % 4 < 2) ? ((k % 2) ? -1 : 1) : ((k % 2) ? -2 : 2);
-                int gi[3] = {g[0], g[1], g[2]};
-                gi[d] += o;
-                if (gi[d] < 0 || gi[d] > I.n[d] - 1) continue;
-                const int li = lidx[p] + off[k];
-                w[p][k] = contrib(I, gi, d, o, s_code[li], s_G[d][li]);
-            }
+    // a fixed grid strides over the listed tiles, one tile per block at a time
+    for (unsigned e = blockIdx.x; e < nlist; e += gridDim.x) {
+        const int64_t gid = list[e];
+        int lo = 0, hi = ninst - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (inst[mid].atile_base <= gid)
+                lo = mid;
+            else
+                hi = mid - 1;
         }
-        l0[p] = s_lam[lidx[p]];
-    }
-    __syncthreads();
+        const InstDev I = inst[lo];
+        const int64_t tl = gid - I.atile_base;
+        const int tix = (int)(tl % I.at[0]), tiy = (int)((tl / I.at[0]) % I.at[1]);
+        const int tiz = ND == 3 ? (int)(tl / ((int64_t)I.at[0] * I.at[1])) : 0;
+        const int ox = tix * S::TX, oy = tiy * S::TY, oz = tiz * S::TZ;
+        const int64_t nx = I.n[0], nxy = (int64_t)I.n[0] * I.n[1];
+        const uint16_t* code = codep + I.tt_off;
+        const double* b = bp + I.tt_off;
+        double* lam = lamp + I.tt_off;
+        if (tid < 7) s_fmax[tid] = 0u;
+        for (int c = tid; c < NL; c += kAB) {
+            const int lx = c % S::LX, ly = (c / S::LX) % S::LY, lz = ND == 3 ? c / (S::LX * S::LY) : 0;
+            int nout = (lx < S::H || lx >= S::H + S::TX) + (ly < S::H || ly >= S::H + S::TY);
+            if (ND == 3) nout += (lz < S::H || lz >= S::H + S::TZ);
+            double lv = 0.0;
+            uint16_t cv = 0x1B6;  // all codes zero ((0+2) in every 3-bit field)
+            double gv[3] = {0.0, 0.0, 0.0};
+            if (nout <= 1) {
+                const int gx = ox + lx - S::H, gy = oy + ly - S::H, gz = ND == 3 ? oz + lz - S::H : 0;
+                if (gx >= 0 && gx < I.n[0] && gy >= 0 && gy < I.n[1] && (ND == 2 || (gz >= 0 && gz < I.n[2]))) {
+                    const int64_t l = gx + nx * gy + nxy * gz;
+                    lv = lam[l];
+                    cv = code[l];
+    #pragma unroll
+                    for (int d = 0; d < ND; d++) gv[d] = Gp[d * gstride + I.tt_off + l];
+                }
+            }
+            s_lam[c] = lv;
+            s_code[c] = cv;
+    #pragma unroll
+            for (int d = 0; d < ND; d++) s_G[d][c] = gv[d];
+        }
+        __syncthreads();
 
-    int it = 0;
-    for (; it < S::MAXIT; it++) {
-        int any = 0;
-#pragma unroll
+        constexpr int NS = 4 * ND;  // candidate rows: axis d, offset +-1, +-2
+        double w[S::NP][NS];
+        int off[NS];
+        double diag[S::NP], bj[S::NP], l0[S::NP];
+        bool act[S::NP];
+        int lidx[S::NP], gxyz[S::NP][3];
+        const int strides[3] = {1, S::LX, S::LX * S::LY};
+    #pragma unroll
+        for (int k = 0; k < NS; k++) {
+            const int d = k / 4, o = (k % 4 < 2) ? ((k % 2) ? -1 : 1) : ((k % 2) ? -2 : 2);
+            off[k] = o * strides[d];
+        }
+    #pragma unroll
+        for (int p = 0; p < S::NP; p++) {
+            const int lx = tid % S::TX, ly = (tid / S::TX) % S::TY;
+            const int lz = ND == 3 ? (tid / (S::TX * S::TY)) + p * (S::TZ / S::NP) : 0;
+            const int g[3] = {ox + lx, oy + ly, oz + lz};
+            gxyz[p][0] = g[0];
+            gxyz[p][1] = g[1];
+            gxyz[p][2] = g[2];
+            lidx[p] = (lx + S::H) + S::LX * ((ly + S::H) + S::LY * (ND == 3 ? lz + S::H : 0));
+            const bool inside = g[0] < I.n[0] && g[1] < I.n[1] && (ND == 2 || g[2] < I.n[2]);
+            const uint16_t cj = s_code[lidx[p]];
+            bool hasrow = false;
+    #pragma unroll
+            for (int d = 0; d < ND; d++) hasrow = hasrow || unpack_code(cj, d) != 0;
+            act[p] = inside && hasrow;
+            diag[p] = 1.0;
+            bj[p] = 0.0;
+    #pragma unroll
+            for (int k = 0; k < NS; k++) w[p][k] = 0.0;
+            if (act[p]) {
+                double Gj[3] = {0, 0, 0};
+    #pragma unroll
+                for (int d = 0; d < ND; d++) Gj[d] = s_G[d][lidx[p]];
+                diag[p] = adj_diag<ND>(I, g, cj, Gj);
+                bj[p] = b[g[0] + nx * g[1] + nxy * g[2]];
+    #pragma unroll
+                for (int k = 0; k < NS; k++) {
+                    const int d = k / 4, o = (k % 4 < 2) ? ((k % 2) ? -1 : 1) : ((k % 2) ? -2 : 2);
+                    int gi[3] = {g[0], g[1], g[2]};
+                    gi[d] += o;
+                    if (gi[d] < 0 || gi[d] > I.n[d] - 1) continue;
+                    const int li = lidx[p] + off[k];
+                    w[p][k] = contrib(I, gi, d, o, s_code[li], s_G[d][li]);
+                }
+            }
+            l0[p] = s_lam[lidx[p]];
+        }
+        __syncthreads();
+
+        int it = 0;
+        for (; it < S::MAXIT; it++) {
+            int any = 0;
+    #pragma unroll
+            for (int p = 0; p < S::NP; p++) {
+                if (!act[p]) continue;
+                double x = bj[p];
+    #pragma unroll
+                for (int k = 0; k < NS; k++)
+                    if (w[p][k] != 0.0) x -= s_lam[lidx[p] + off[k]] * w[p][k];
+                x = x / diag[p];
+                if (x != s_lam[lidx[p]]) {
+                    s_lam[lidx[p]] = x;
+                    any = 1;
+                }
+            }
+            if (!__syncthreads_or(any)) break;
+        }
+    #pragma unroll
         for (int p = 0; p < S::NP; p++) {
             if (!act[p]) continue;
-            double x = bj[p];
-#pragma unroll
-            for (int k = 0; k < NS; k++)
-                if (w[p][k] != 0.0) x -= s_lam[lidx[p] + off[k]] * w[p][k];
-            x = x / diag[p];
-            if (x != s_lam[lidx[p]]) {
-                s_lam[lidx[p]] = x;
-                any = 1;
+            const double v = s_lam[lidx[p]];
+            if (v != l0[p]) {
+                const int gx = gxyz[p][0], gy = gxyz[p][1], gz = gxyz[p][2];
+                const int64_t l = gx + nx * gy + nxy * gz;
+                lam[l] = v;
+                const unsigned tv = __float_as_uint(__double2float_ru(fabs((double)ttpool[I.tt_off + l])));
+                atomicMax(&s_fmax[6], tv);
+                // a changed row only feeds the nodes its own stencil reads (code c along an axis: the
+                // |c| nodes on the side opposite to the sign of c), so a neighbour tile is woken only
+                // when that stencil crosses into it
+                const int lc[3] = {gx - ox, gy - oy, gz - oz};
+                const int te[3] = {S::TX, S::TY, S::TZ};
+                const uint16_t cr = s_code[lidx[p]];
+    #pragma unroll
+                for (int d = 0; d < ND; d++) {
+                    const int c = unpack_code(cr, d);
+                    if (c > 0 && lc[d] < c) atomicMax(&s_fmax[2 * d], tv);
+                    if (c < 0 && lc[d] - c >= te[d]) atomicMax(&s_fmax[2 * d + 1], tv);
+                }
             }
         }
-        if (!__syncthreads_or(any)) break;
-    }
-#pragma unroll
-    for (int p = 0; p < S::NP; p++) {
-        if (!act[p]) continue;
-        const double v = s_lam[lidx[p]];
-        if (v != l0[p]) {
-            const int gx = gxyz[p][0], gy = gxyz[p][1], gz = gxyz[p][2];
-            const int64_t l = gx + nx * gy + nxy * gz;
-            lam[l] = v;
-            const unsigned tv = __float_as_uint(__double2float_ru(fabs((double)ttpool[I.tt_off + l])));
-            atomicMax(&s_fmax[6], tv);
-            // a changed row only feeds the nodes its own stencil reads (code c along an axis: the
-            // |c| nodes on the side opposite to the sign of c), so a neighbour tile is woken only
-            // when that stencil crosses into it
-            const int lc[3] = {gx - ox, gy - oy, gz - oz};
-            const int te[3] = {S::TX, S::TY, S::TZ};
-            const uint16_t cr = s_code[lidx[p]];
-#pragma unroll
-            for (int d = 0; d < ND; d++) {
-                const int c = unpack_code(cr, d);
-                if (c > 0 && lc[d] < c) atomicMax(&s_fmax[2 * d], tv);
-                if (c < 0 && lc[d] - c >= te[d]) atomicMax(&s_fmax[2 * d + 1], tv);
+        __syncthreads();
+        if (tid == 0) {
+            uint8_t* fl = flags + I.atile_base;
+            unsigned* pd = apend + I.atile_base;
+            const int64_t t0 = I.at[0], t01 = (int64_t)I.at[0] * I.at[1];
+            auto wake = [&](int64_t t, unsigned v) {
+                fl[t] = 1;
+                atomicMax(&pd[t], v);
+            };
+            if (s_fmax[0] && tix > 0) wake(tl - 1, s_fmax[0]);
+            if (s_fmax[1] && tix < I.at[0] - 1) wake(tl + 1, s_fmax[1]);
+            if (s_fmax[2] && tiy > 0) wake(tl - t0, s_fmax[2]);
+            if (s_fmax[3] && tiy < I.at[1] - 1) wake(tl + t0, s_fmax[3]);
+            if (ND == 3) {
+                if (s_fmax[4] && tiz > 0) wake(tl - t01, s_fmax[4]);
+                if (s_fmax[5] && tiz < I.at[2] - 1) wake(tl + t01, s_fmax[5]);
             }
+            if (it >= S::MAXIT) wake(tl, s_fmax[6]);
+            atomicAdd(&stats[kAdjSweeps], (unsigned)it);
         }
-    }
-    __syncthreads();
-    if (tid == 0) {
-        uint8_t* fl = flags + I.atile_base;
-        unsigned* pd = apend + I.atile_base;
-        const int64_t t0 = I.at[0], t01 = (int64_t)I.at[0] * I.at[1];
-        auto wake = [&](int64_t t, unsigned v) {
-            fl[t] = 1;
-            atomicMax(&pd[t], v);
-        };
-        if (s_fmax[0] && tix > 0) wake(tl - 1, s_fmax[0]);
-        if (s_fmax[1] && tix < I.at[0] - 1) wake(tl + 1, s_fmax[1]);
-        if (s_fmax[2] && tiy > 0) wake(tl - t0, s_fmax[2]);
-        if (s_fmax[3] && tiy < I.at[1] - 1) wake(tl + t0, s_fmax[3]);
-        if (ND == 3) {
-            if (s_fmax[4] && tiz > 0) wake(tl - t01, s_fmax[4]);
-            if (s_fmax[5] && tiz < I.at[2] - 1) wake(tl + t01, s_fmax[5]);
-        }
-        if (it >= S::MAXIT) wake(tl, s_fmax[6]);
-        atomicAdd(nsweeps, (unsigned)it);
+        __syncthreads();  // LDS is reused by the next tile
     }
 }
 
@@ -853,34 +876,51 @@ static int adj_run_t(eik_ctx* ctx, const InstDev* d_inst, int ninst, int64_t nti
                      const double* G, int64_t gstride, const double* b, double* lam, const T* ttpool, FimStats* fs) {
     DevBuf& list = ctx->buf("fim_list");
     DevBuf& wm = ctx->buf("adj_wmax");
-    if (list.ensure((size_t)ntiles * sizeof(int64_t)) || wm.ensure((size_t)ninst * sizeof(unsigned)))
+    if (list.ensure((size_t)ntiles * sizeof(int64_t)) || wm.ensure((size_t)2 * ninst * sizeof(unsigned)))
         return ctx->fail(EIK_ENOMEM, "out of device memory (tile list)");
     uint8_t* flags = (uint8_t*)ctx->buf("fim_flags").p;
     unsigned* apend = (unsigned*)ctx->buf("adj_pend").p;
     unsigned* d_cnt = (unsigned*)ctx->d_small;
     unsigned* h_cnt = (unsigned*)ctx->h_small;
-    FimStats st{};
-    CK(hipMemsetAsync(d_cnt + 1, 0, sizeof(unsigned), ctx->stream));
+    unsigned* W = (unsigned*)wm.p;  // 2 x ninst highest pending values, by round parity
+    hipStream_t st = ctx->stream;
+    // resident blocks that stride over each round's list (EIK_GRID does not apply here)
+    static const int grid = [ctx] {
+        int ncu = 0, occ = 0;
+        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess) ncu = 1;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_adj_round<ND, T>, kAB, 0) != hipSuccess) occ = 1;
+        return std::max(1, ncu * std::max(1, occ));
+    }();
+    // Rounds are enqueued kAdjBatch at a time with no host synchronisation: the round kernel reads
+    // its list length on the device and clears the next round's length and maxima (round parity;
+    // both were last used by the previous round, which has ended).  After a batch the host reads
+    // the last round's list length; none listed means converged (a round with nothing listed wakes
+    // nothing, so every later round is empty too).
+    CK(hipMemsetAsync(d_cnt, 0, kAdjWords * sizeof(unsigned), st));
+    k_fill_u32<<<nb(2 * (int64_t)ninst, 256), 256, 0, st>>>(W, 2 * (int64_t)ninst, 0u);
+    FimStats stt{};
+    int64_t round = 0;
     for (;;) {
-        CK(hipMemsetAsync(d_cnt, 0, sizeof(unsigned), ctx->stream));
-        k_fill_u32<<<nb(ninst, 256), 256, 0, ctx->stream>>>((unsigned*)wm.p, ninst, 0u);
-        k_adj_band_max<<<nb(ntiles, 256), 256, 0, ctx->stream>>>(flags, apend, ntiles, d_inst, ninst, (unsigned*)wm.p);
-        k_adj_band_select<<<nb(ntiles, 256), 256, 0, ctx->stream>>>(flags, apend, ntiles, d_inst, ninst,
-                                                                   (const unsigned*)wm.p, (int64_t*)list.p, d_cnt);
-        CK(hipMemcpyAsync(h_cnt, d_cnt, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
-        CK(hipStreamSynchronize(ctx->stream));
-        const unsigned nact = h_cnt[0];
-        if (nact == 0) {
-            st.sweeps = (double)h_cnt[1];
-            break;
+        for (int r = 0; r < kAdjBatch; r++, round++) {
+            const int p = (int)(round & 1);
+            k_adj_band_max<<<nb(ntiles, 256), 256, 0, st>>>(flags, apend, ntiles, d_inst, ninst, W + (size_t)p * ninst);
+            k_adj_band_select<<<nb(ntiles, 256), 256, 0, st>>>(flags, apend, ntiles, d_inst, ninst,
+                                                               (const unsigned*)W + (size_t)p * ninst,
+                                                               (int64_t*)list.p, d_cnt + kAdjSlots + p);
+            k_adj_round<ND, T><<<grid, kAB, 0, st>>>((const int64_t*)list.p, d_cnt + kAdjSlots + p,
+                                                      d_cnt + kAdjSlots + (p ^ 1), W + (size_t)(p ^ 1) * ninst, d_inst,
+                                                      ninst, code, G, gstride, b, lam, ttpool, flags, apend, d_cnt);
+            CK(hipGetLastError());
         }
-        if (++st.rounds > 50000) return ctx->fail(EIK_ENOCONV, "adjoint solve did not converge");
-        st.activations += nact;
-        k_adj_round<ND, T><<<nact, kAB, 0, ctx->stream>>>((const int64_t*)list.p, d_inst, ninst, code, G, gstride, b,
-                                                           lam, ttpool, flags, apend, d_cnt + 1);
-        CK(hipGetLastError());
+        CK(hipMemcpyAsync(h_cnt, d_cnt, kAdjWords * sizeof(unsigned), hipMemcpyDeviceToHost, st));
+        CK(hipStreamSynchronize(st));
+        if (h_cnt[kAdjSlots + (int)((round - 1) & 1)] == 0u) break;
+        if (round > 50000) return ctx->fail(EIK_ENOCONV, "adjoint solve did not converge");
     }
-    if (fs) *fs = st;
+    stt.rounds = (int64_t)h_cnt[kAdjRounds];
+    stt.activations = (double)h_cnt[kAdjActs];
+    stt.sweeps = (double)h_cnt[kAdjSweeps];
+    if (fs) *fs = stt;
     return EIK_OK;
 }
 
