@@ -97,10 +97,11 @@ __device__ __forceinline__ float key_down(float k) { return k; }
 __device__ __forceinline__ float key_up(double k) { return __double2float_ru(k); }
 __device__ __forceinline__ float key_up(float k) { return k; }
 
-// Compact LDS image of a tile: the NODES core nodes (x fastest), then for each face k = 2*axis + l
-// (l = 0: + side, l = 1: - side; the neighbour-slot numbering of eik_stencil.h) its two halo
-// layers of FA nodes each, nearer layer first.  The edges and corners of the halo are never read
-// by the stencil and get no slot, which halves the image of a 3D tile (256 slots instead of 8^3).
+// LDS image of a tile.  Compact layout (the pop keys K and S, which only the rare cascade path
+// reads around a node): the NODES core nodes (x fastest), then for each face k = 2*axis + l (l = 0:
+// + side, l = 1: - side; the neighbour-slot numbering of eik_stencil.h) its two halo layers of FA
+// nodes each, nearer layer first; the edges and corners of the halo are never read by the stencil
+// and get no slot (256 slots instead of 8^3 for a 3D tile).  The staging loads go by compact slot.
 template <int ND>
 struct TileLds {
     static constexpr int E = TileShape<ND>::TX;  // tiles are squares / cubes
@@ -123,13 +124,16 @@ struct TileLds {
         const int q = l ? d - 1 - c : c + d - E;
         return in ? n + (l ? -d : d) * stride(ax) : NN + (2 * ax + l) * 2 * FA + q * FA + face_index(n, ax);
     }
-    // the same, or NN when that node lies outside the core (halo nodes never change in a round)
-    __device__ static int core_nbr(int n, int k, int d) {
-        const int ax = k >> 1, l = k & 1;
-        const int c = coord(n, ax);
-        const bool in = l ? c - d >= 0 : c + d < E;
-        return in ? n + (l ? -d : d) * stride(ax) : NN;
-    }
+    // The values, depth bytes and change keys sit in a padded box instead: the core plus two
+    // layers on every side (edges and corners unused), so every stencil neighbour of a core node
+    // lies at a fixed offset from its slot and a lane reads all of them from one address register
+    // with immediate offsets.
+    static constexpr int P = E + 4;
+    static constexpr int PV = ND == 3 ? P * P * P : P * P;
+    static constexpr int pstride(int ax) { return ax == 0 ? 1 : (ax == 1 ? P : P * P); }
+    __device__ static int pslot(int x, int y, int z) { return (x + 2) + P * (y + 2) + (ND == 3 ? P * P * (z + 2) : 0); }
+    // padded-box offset of the node d steps across face k
+    static constexpr int poff(int k, int d) { return ((k & 1) ? -d : d) * pstride(k >> 1); }
     // tile-local position of halo slot c (NN <= c < NL); coordinates may be -2, -1, E, E + 1
     __device__ static void halo_pos(int c, int* p) {
         const int r = c - NN;
@@ -178,14 +182,26 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(kTileThreads, kTileTh
     constexpr int NN = S::NODES, NL = L::NL, E = L::E;
     static_assert(NN == kTileThreads, "one node per lane");
     static_assert(NL % NN == 0, "staging loop");
-    __shared__ T s_t[NL];               // value, sign bit = init node
-    __shared__ T s_k[NL];               // pop key K (the value itself outside cascades)
-    __shared__ T s_s[NL];               // pop key S (cascade nodes)
-    __shared__ uint8_t s_d[NL];         // pop key D
-    __shared__ float s_chg[2][NN + 1];  // smaller of old/new key of a node changed in a sweep; [NN]: +inf
-    __shared__ uint4 s_run[32];         // list entries of the claimed run
+    constexpr int PV = L::PV;
+    // The tile image.  The key arrays come first: the padded arrays then start at least two
+    // layers' worth of bytes into LDS, so each neighbour's offset from a lane's slot, plus the
+    // array's own address, is a non-negative immediate.
+    struct Image {
+        T k[NL];              // pop key K (the value itself outside cascades), compact layout
+        T s[NL];              // pop key S (cascade nodes), compact layout
+        T t[PV];              // value, sign bit = init node
+        float chg[2][PV];     // smaller of old/new key of a core node changed in a sweep; +inf elsewhere
+        uint8_t d[PV];        // pop key D
+    };
+    __shared__ Image img;
+    __shared__ uint4 s_run[32];  // list entries of the claimed run
+    T* const s_k = img.k;
+    T* const s_s = img.s;
+    T* const s_t = img.t;
+    uint8_t* const s_d = img.d;
     const int n = threadIdx.x;
     const int cx = L::coord(n, 0), cy = L::coord(n, 1), cz = ND == 3 ? L::coord(n, 2) : 0;
+    const int ps = L::pslot(cx, cy, cz);  // this lane's node in the padded arrays
     // faces this node lies within two of: bit 2a = - side of axis a, bit 2a + 1 = + side
     unsigned fmask = (unsigned)(cx < 2) | (unsigned)(cx >= E - 2) << 1 | (unsigned)(cy < 2) << 2 |
                      (unsigned)(cy >= E - 2) << 3;
@@ -201,10 +217,8 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(kTileThreads, kTileTh
             slot_reset[kSlotCursor] = 0u;
         }
     }
-    if (n == 0) {
-        s_chg[0][NN] = EIK_INFF;
-        s_chg[1][NN] = EIK_INFF;
-    }
+    // change keys outside the core stay +inf (core slots are reset per tile)
+    for (int i = n; i < 2 * PV; i += kTileThreads) (&img.chg[0][0])[i] = EIK_INFF;
     // Entries are claimed in runs of consecutive list entries, which are neighbouring tiles (the
     // list is built in tile order): a block relaxing a tile right after its neighbour sees that
     // neighbour's new values, which saves work.  Runs shrink towards the end of the list (at most
@@ -308,17 +322,20 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(kTileThreads, kTileTh
 #pragma unroll
         for (int h = 0; h < NC; h++) {
             const int c = (h + 1) * NN + n;
-            s_t[c] = hv[h];
+            int p[3];
+            L::halo_pos(c, p);
+            const int q = L::pslot(p[0], p[1], p[2]);
+            s_t[q] = hv[h];
             s_k[c] = hk[h];
             s_s[c] = hs[h];
-            s_d[c] = (uint8_t)hd[h];
+            s_d[q] = (uint8_t)hd[h];
         }
-        s_t[n] = v0;
+        s_t[ps] = v0;
         s_k[n] = k0.K;
         s_s[n] = k0.S;
-        s_d[n] = (uint8_t)k0.D;
-        s_chg[0][n] = EIK_INFF;
-        s_chg[1][n] = EIK_INFF;
+        s_d[ps] = (uint8_t)k0.D;
+        img.chg[0][ps] = EIK_INFF;
+        img.chg[1][ps] = EIK_INFF;
         // stencil neighbours inside the grid (ok1) and usable 2-away nodes (ok2), per slot k
         unsigned ok1 = 0u, ok2 = 0u;
 #pragma unroll
@@ -349,7 +366,7 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(kTileThreads, kTileTh
             } else {
                 bool reached = false;
 #pragma unroll
-                for (int k = 0; k < 2 * ND; k++) reached = reached | (fabs(s_t[L::nbr(n, k, 1)]) < tinf<T>());
+                for (int k = 0; k < 2 * ND; k++) reached = reached | (fabs(s_t[ps + L::poff(k, 1)]) < tinf<T>());
                 const bool cand = (fmask & fbits) != 0u || ((fbits & kKeyBit) != 0u && reached);
                 if (cand && (T)pend0 <= k0.K) ptr = pend0;
             }
@@ -371,7 +388,7 @@ int it = 0;
                 for (int k = 0; k < 2 * ND; k++) {
 #pragma unroll
                     for (int d = 1; d <= 2; d++) {
-                        const float c = s_chg[pb][L::core_nbr(n, k, d)];
+                        const float c = img.chg[pb][ps + L::poff(k, d)];
                         tr = (c <= kx && c < tr) ? c : tr;
                     }
                 }
@@ -380,7 +397,7 @@ int it = 0;
             const bool need = act & (ptr < EIK_INFF) & (ptr <= lim);
             ptr = need ? EIK_INFF : ptr;
             __syncthreads();  // every lane has read the previous sweep's changes
-            s_chg[pb][n] = EIK_INFF;
+            img.chg[pb][ps] = EIK_INFF;
             if (__ballot(need) == 0ull) break;
             // 2. update
             if (need) {
@@ -390,7 +407,7 @@ int it = 0;
                 // neighbour below is masked by ok1 / ok2, so the reads need no bounds select.
 #pragma unroll
                 for (int k = 0; k < 2 * ND; k++) {
-                    const int o1 = L::nbr(n, k, 1), o2 = L::nbr(n, k, 2);
+                    const int o1 = ps + L::poff(k, 1), o2 = ps + L::poff(k, 2);
                     const T a = s_t[o1];
                     const T b = s_t[o2];
                     const int da = (int)s_d[o1], db = (int)s_d[o2];
@@ -405,7 +422,7 @@ int it = 0;
                 // pop keys straight from LDS (only the cascade path reads them)
                 auto keys = [&](int k, bool two) {
                     const int o = L::nbr(n, k, two ? 2 : 1);
-                    return Key<T>{s_k[o], s_s[o], (int)s_d[o]};
+                    return Key<T>{s_k[o], s_s[o], (int)s_d[ps + L::poff(k, two ? 2 : 1)]};
                 };
                 int code[3];
                 Key<T> nk;
@@ -431,21 +448,21 @@ int it = 0;
                     nv = replay<ND, T, MIX>(v1, v2, keys, d1, d2, dcap, f1, f2, ok1, ok2, sa, mcart, code, &nk);
                 }
                 tot_upd++;
-                const T cur = s_t[n];
+                const T cur = s_t[ps];
                 const T kold = s_k[n];
                 const T sold = s_s[n];
-                const int dold = (int)s_d[n];
+                const int dold = (int)s_d[ps];
                 const bool vchg = nv == nv && nv != cur && !(nv >= tinf<T>() && cur >= tinf<T>());
                 const bool kchg = nv < tinf<T>() && (nk.K != kold || nk.S != sold || nk.D != dold);
                 if (vchg || kchg) {
                     tot_vchg += vchg ? 1u : 0u;
                     tot_kchg += vchg ? 0u : 1u;
                     const bool quiet = tiny_change<T>(cur, Key<T>{kold, sold, dold}, nv, nk, settle);
-                    s_t[n] = nv;
+                    s_t[ps] = nv;
                     s_k[n] = nk.K;
                     s_s[n] = nk.S;
-                    s_d[n] = (uint8_t)nk.D;
-                    if (!quiet) s_chg[cb][n] = key_down(kold < nk.K ? kold : nk.K);
+                    s_d[ps] = (uint8_t)nk.D;
+                    if (!quiet) img.chg[cb][ps] = key_down(kold < nk.K ? kold : nk.K);
                 }
             }
             __syncthreads();
@@ -464,8 +481,8 @@ int it = 0;
         // write back a changed node; its smaller key (old or new) wakes the neighbour tiles
         unsigned pvl = 0x7f800000u;
         if (act) {
-            const T v = s_t[n];
-            const Key<T> kn{s_k[n], s_s[n], (int)s_d[n]};
+            const T v = s_t[ps];
+            const Key<T> kn{s_k[n], s_s[n], (int)s_d[ps]};
             if (!(v == v0 && keq(kn, k0))) {
                 const bool quiet = tiny_change<T>(v0, k0, v, kn, settle);
                 // every field of the node is written (K = value, S = -inf for an ordinary node, and
