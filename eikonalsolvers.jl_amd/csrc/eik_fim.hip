@@ -242,10 +242,7 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(kTileThreads, kTileTh
         // the run's next tile: lowest pending value among those not yet relaxed (lowest lane on ties)
         const bool open = (unsigned)n < rlen && !((rdone >> (n & 31)) & 1u);
         const unsigned mine = open ? s_run[n].z : 0xffffffffu;
-        unsigned best = mine;
-#pragma unroll
-        for (int o = 16; o > 0; o >>= 1) best = min(best, (unsigned)__shfl_xor((int)best, o));
-        best = (unsigned)__builtin_amdgcn_readfirstlane((int)best);  // lanes 0-31 hold the run
+        const unsigned best = wave_min_u32(mine);  // lanes 0-31 hold the run
         const unsigned long long cand = __ballot(open && mine == best);
         const int bk = cand ? __ffsll((long long)cand) - 1 : 0;
         rdone |= 1u << bk;
@@ -382,17 +379,23 @@ int it = 0;
             const int cb = it & 1, pb = cb ^ 1;
             // 1. collect the triggers of the previous sweep; a node within the band is updated
             if (act && it > 0) {
+                // the smallest change key not above the node's own key (a tree of minima: the keys
+                // are never NaN, and +inf stands for none)
                 const float kx = key_up(s_k[n]);
-                float tr = ptr;
+                float m[4 * ND];
 #pragma unroll
                 for (int k = 0; k < 2 * ND; k++) {
 #pragma unroll
                     for (int d = 1; d <= 2; d++) {
                         const float c = img.chg[pb][ps + L::poff(k, d)];
-                        tr = (c <= kx && c < tr) ? c : tr;
+                        m[2 * k + d - 1] = c <= kx ? c : EIK_INFF;
                     }
                 }
-                ptr = tr;
+#pragma unroll
+                for (int w = 1; w < 4 * ND; w *= 2)
+#pragma unroll
+                    for (int i = 0; i + w < 4 * ND; i += 2 * w) m[i] = fminf(m[i], m[i + w]);
+                ptr = fminf(ptr, m[0]);
             }
             const bool need = act & (ptr < EIK_INFF) & (ptr <= lim);
             ptr = need ? EIK_INFF : ptr;
