@@ -356,6 +356,13 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(kTileThreads, kTileTh
             if (n == 0) claim = atomicAdd(&slot[kSlotCursor], want);
         }
         bool act = inside && !is_fixed(v0);
+        // plain image: no init node and no cascade key anywhere in the core or the halo.  The
+        // sweeps then read values only (no sign bits to strip, no depth bytes); it ends with the
+        // first cascade key a sweep writes.
+        bool odd_node = is_fixed(v0) || d0 != 0;
+#pragma unroll
+        for (int h = 0; h < NC; h++) odd_node = odd_node || is_fixed(hv[h]) || hd[h] != 0;
+        bool plain = !(MIX && NEAR) && __ballot(odd_node) == 0ull;
         float ptr = EIK_INFF;  // pending trigger key
         if (act) {
             if (fbits & kFullBit) {
@@ -403,7 +410,44 @@ int it = 0;
             img.chg[pb][ps] = EIK_INFF;
             if (__ballot(need) == 0ull) break;
             // 2. update
-            if (need) {
+            bool newkey = false;  // this lane wrote a cascade key
+            if (need && plain) {
+                T v1[2 * ND], v2[2 * ND];
+#pragma unroll
+                for (int k = 0; k < 2 * ND; k++) {
+                    v1[k] = s_t[ps + L::poff(k, 1)];
+                    v2[k] = s_t[ps + L::poff(k, 2)];
+                }
+                auto nokeys = [](int, bool) { return Key<T>{T(0), T(0), 0}; };  // never called: no cascade key
+                int code[3];
+                Key<T> nk;
+                T nv;
+                if constexpr (SPH) {
+                    Metric<A> mt;
+                    node_spacing<A>(I, g, mt.dh);
+                    metric_coefs(mt, ND);
+                    nv = replay<ND, T, MIX>(v1, v2, nokeys, 0u, 0u, 0u, 0u, 0u, ok1, ok2, sa, mt, code, &nk);
+                } else {
+                    nv = replay<ND, T, MIX>(v1, v2, nokeys, 0u, 0u, 0u, 0u, 0u, ok1, ok2, sa, mcart, code, &nk);
+                }
+                tot_upd++;
+                const T cur = s_t[ps];
+                const T kold = s_k[n];
+                // an ordinary node's key is (value, -inf, 0)
+                const bool vchg = nv == nv && nv != cur && !(nv >= tinf<T>() && cur >= tinf<T>());
+                const bool kchg = nv < tinf<T>() && (nk.K != kold || nk.S != -tinf<T>() || nk.D != 0);
+                if (vchg || kchg) {
+                    tot_vchg += vchg ? 1u : 0u;
+                    tot_kchg += vchg ? 0u : 1u;
+                    const bool quiet = tiny_change<T>(cur, Key<T>{kold, -tinf<T>(), 0}, nv, nk, settle);
+                    s_t[ps] = nv;
+                    s_k[n] = nk.K;
+                    s_s[n] = nk.S;
+                    s_d[ps] = (uint8_t)nk.D;
+                    newkey = nk.D != 0;
+                    if (!quiet) img.chg[cb][ps] = key_down(kold < nk.K ? kold : nk.K);
+                }
+            } else if (need) {
                 unsigned f1 = 0u, f2 = 0u, d1 = 0u, d2 = 0u, dcap = 0u;
                 T v1[2 * ND], v2[2 * ND];
                 // Slots outside the grid are staged as +inf with depth 0, and every use of a
@@ -468,6 +512,7 @@ int it = 0;
                     if (!quiet) img.chg[cb][ps] = key_down(kold < nk.K ? kold : nk.K);
                 }
             }
+            plain = plain && __ballot(newkey) == 0ull;
             __syncthreads();
         }
         const bool converged = it < S::MAXIT;
