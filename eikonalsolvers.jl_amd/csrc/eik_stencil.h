@@ -440,8 +440,22 @@ __device__ __forceinline__ bool axis_terms(const T* v1, const T* v2, unsigned ca
 }
 
 // eval64 / eval32 over the axes in `on` from their precomputed contributions (same operations in
-// the same order, so the same result).
+// the same order, so the same result).  FB: form the fallback for a negative discriminant (the
+// caller knows whether any lane needs it); disc_on is the discriminant alone.
 template <int ND>
+__device__ __forceinline__ double disc_on(const AxisTerm<double, double>* at, unsigned on, double s) {
+#pragma clang fp contract(off)
+    double alpha = 0.0, beta = 0.0, gamma = -(s * s);
+#pragma unroll
+    for (int ax = 0; ax < ND; ax++) {
+        const bool u = (on >> ax) & 1u;
+        alpha = u ? alpha + at[ax].ca : alpha;
+        beta = u ? beta + at[ax].bt : beta;
+        gamma = u ? gamma + at[ax].gt : gamma;
+    }
+    return beta * beta - 4.0 * alpha * gamma;
+}
+template <int ND, bool FB>
 __device__ __forceinline__ double eval_on(const AxisTerm<double, double>* at, unsigned on, double s,
                                           const Metric<double>& m, int* code) {
 #pragma clang fp contract(off)
@@ -457,7 +471,7 @@ __device__ __forceinline__ double eval_on(const AxisTerm<double, double>* at, un
     const double sqA = beta * beta - 4.0 * alpha * gamma;
     double a = alpha, b = beta, q = sqA, oned = 0.0;
     bool useoned = false;
-    if (__ballot(sqA < 0.0) != 0ull) {
+    if (FB) {
         double alpha1 = 0.0, beta1 = 0.0, gamma1 = -(s * s);
         double cmin = EIK_INF, hmin = m.dh[0];
         int idc = 0;
@@ -492,6 +506,18 @@ __device__ __forceinline__ double eval_on(const AxisTerm<double, double>* at, un
     return useoned ? oned : t;
 }
 template <int ND>
+__device__ __forceinline__ float disc_on(const AxisTerm<float, float>* at, unsigned on, float s) {
+    float A = 0.f, B = 0.f, C = -(s * s);
+#pragma unroll
+    for (int ax = 0; ax < ND; ax++) {
+        const bool u = (on >> ax) & 1u;
+        A = u ? A + at[ax].ca : A;
+        B = u ? B + at[ax].bt : B;
+        C = u ? C + at[ax].gt : C;
+    }
+    return B * B - A * C;
+}
+template <int ND, bool FB>
 __device__ __forceinline__ float eval_on(const AxisTerm<float, float>* at, unsigned on, float s, const Metric<float>& m,
                                          int* code) {
     float r = EIK_INFF;
@@ -509,7 +535,7 @@ __device__ __forceinline__ float eval_on(const AxisTerm<float, float>* at, unsig
     const float discA = B * B - A * C;
     float a = A, b = B, q = discA, oned = 0.f;
     bool useoned = false;
-    if (__ballot(discA < 0.f) != 0ull) {
+    if (FB) {
         float A1 = 0.f, B1 = 0.f, C1 = -(s * s);
         float cmin = EIK_INFF, hmin = m.dh[0];
         int idc = 0;
@@ -548,6 +574,18 @@ __device__ __forceinline__ float eval_on(const AxisTerm<float, float>* at, unsig
 // eval_mix over the axes in `on` from their precomputed contributions (the first axis in use is
 // the smallest candidate, so r is the same shift axis_terms used).
 template <int ND>
+__device__ __forceinline__ float disc_on(const AxisTerm<double, float>* at, unsigned on, float s) {
+    float A = 0.f, B = 0.f, C = -(s * s);
+#pragma unroll
+    for (int ax = 0; ax < ND; ax++) {
+        const bool u = (on >> ax) & 1u;
+        A = u ? A + at[ax].ca : A;
+        B = u ? B + at[ax].bt : B;
+        C = u ? C + at[ax].gt : C;
+    }
+    return B * B - A * C;
+}
+template <int ND, bool FB>
 __device__ __forceinline__ double eval_on(const AxisTerm<double, float>* at, unsigned on, float s,
                                           const Metric<float>& m, int* code) {
 #pragma clang fp contract(off)
@@ -567,7 +605,7 @@ __device__ __forceinline__ double eval_on(const AxisTerm<double, float>* at, uns
     float a = A, b = B, q = discA, hs = 0.f;
     double cmin = EIK_INF;
     bool useoned = false;
-    if (__ballot(discA < 0.f) != 0ull) {
+    if (FB) {
         float A1 = 0.f, B1 = 0.f, C1 = -(s * s);
         float hmin = m.dh[0];
         int idc = 0;
@@ -603,31 +641,59 @@ __device__ __forceinline__ double eval_on(const AxisTerm<double, float>* at, uns
     return useoned ? cmin + (double)hs : t;
 }
 
-// replay_plain's sequence, one step per axis (see above).
+// replay_plain's sequence, one step per axis (see above).  The steps' axis sets follow from the
+// sorted candidate values, so every step is evaluated at once, the evaluations independent of each
+// other (one wave's long square-root and divide chains interleave), and the heap's sequence then
+// keeps the last step it reaches; a step beyond it is evaluated and dropped.  The fallback of a
+// negative discriminant is formed for every step when some lane needs it for any.
 template <int ND, typename T, bool MIX = false>
 __device__ __forceinline__ T replay_axes(const AxisTerm<T, Arith<T, MIX>>* at, Arith<T, MIX> s,
                                          const Metric<Arith<T, MIX>>& m, int* code, Key<T>* kout) {
+    T c[ND];          // distinct candidate values, increasing (+inf: none)
+    unsigned on[ND];  // the axes in use once c[k] has popped
+    T prev = -tinf<T>();
+#pragma unroll
+    for (int k = 0; k < ND; k++) {
+        T ck = tinf<T>();
+#pragma unroll
+        for (int ax = 0; ax < ND; ax++) ck = (at[ax].cv1 > prev && at[ax].cv1 < ck) ? at[ax].cv1 : ck;
+        unsigned o = 0u;
+#pragma unroll
+        for (int ax = 0; ax < ND; ax++) o |= (unsigned)(at[ax].cv1 <= ck) << ax;
+        c[k] = ck;
+        on[k] = o;
+        prev = ck;
+    }
+    bool neg = false;
+#pragma unroll
+    for (int k = 0; k < ND; k++) neg = neg | ((c[k] < tinf<T>()) & (disc_on<ND>(at, on[k], s) < 0));
+    T e[ND];
+    int cd[ND][3];
+    if (__ballot(neg) != 0ull) {
+#pragma unroll
+        for (int k = 0; k < ND; k++) e[k] = eval_on<ND, true>(at, on[k], s, m, cd[k]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < ND; k++) e[k] = eval_on<ND, false>(at, on[k], s, m, cd[k]);
+    }
     T cur = tinf<T>();
     T kzK = tinf<T>(), kzS = -tinf<T>();
     int kzD = 0;
 #pragma unroll
     for (int d = 0; d < ND; d++) code[d] = 0;
-    T prev = -tinf<T>();
-    unsigned on = 0u;
-#pragma unroll 1
-    for (int step = 0; step < ND; step++) {
-        T c = tinf<T>();
+    bool go = true;
 #pragma unroll
-        for (int ax = 0; ax < ND; ax++) c = (at[ax].cv1 > prev && at[ax].cv1 < c) ? at[ax].cv1 : c;
-        if (!(c < tinf<T>()) || kzD != 0 || !(c < tie_below(kzK))) break;
-        prev = c;
+    for (int k = 0; k < ND; k++) {
+        go = go && (c[k] < tinf<T>()) && kzD == 0 && (c[k] < tie_below(kzK));
+        if (go) {
+            cur = e[k];
+            const bool ord = pops_after(cur, c[k]);
+            kzK = ord ? cur : c[k];
+            kzS = ord ? -tinf<T>() : cur;
+            kzD = ord ? 0 : 1;
 #pragma unroll
-        for (int ax = 0; ax < ND; ax++) on |= (unsigned)(at[ax].cv1 == c) << ax;
-        cur = eval_on<ND>(at, on, s, m, code);
-        const bool ord = pops_after(cur, c);
-        kzK = ord ? cur : c;
-        kzS = ord ? -tinf<T>() : cur;
-        kzD = ord ? 0 : 1;
+            for (int d = 0; d < ND; d++) code[d] = cd[k][d];
+        }
     }
     *kout = Key<T>{kzK, kzS, kzD};
     return cur;
