@@ -89,12 +89,20 @@ __device__ __forceinline__ bool tiny_change(T v0, const Key<T>& k0, T v1, const 
 }
 
 // A pop key K as a float no larger than it (changes and pending values are compared against node
-// keys conservatively, so rounding must never move them up).
-__device__ __forceinline__ float key_down(double k) { return __double2float_rd(k); }
+// keys conservatively, so rounding must never move them up).  Keys here are >= 0 or +inf: the
+// nearest float, stepped one unit towards zero when it lies above K, is K rounded down (+inf stays).
+__device__ __forceinline__ float key_down(double k) {
+    const float f = (float)k;
+    return (double)f > k ? __uint_as_float(__float_as_uint(f) - 1u) : f;
+}
 __device__ __forceinline__ float key_down(float k) { return k; }
 // ... and a node's own key as a float no smaller than it: a trigger c <= key_up(K) includes every
-// c <= K (at most a spurious re-evaluation, which leaves a node whose inputs did not change as it is)
-__device__ __forceinline__ float key_up(double k) { return __double2float_ru(k); }
+// c <= K (at most a spurious re-evaluation, which leaves a node whose inputs did not change as it
+// is).  For K >= 0: the nearest float, stepped one unit up when it lies below K.
+__device__ __forceinline__ float key_up(double k) {
+    const float f = (float)k;
+    return (double)f < k ? __uint_as_float(__float_as_uint(f) + 1u) : f;
+}
 __device__ __forceinline__ float key_up(float k) { return k; }
 
 // LDS image of a tile.  Compact layout (the pop keys K and S, which only the rare cascade path
@@ -381,7 +389,7 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(kTileThreads, kTileTh
         for (int ax = 0; ax < 3; ax++) mcart.dh[ax] = (A)I.h[0];
         metric_coefs(mcart, ND);
         const A sa = (A)sl;
-int it = 0;
+        int it = 0;
         for (; it < S::MAXIT; it++) {
             const int cb = it & 1, pb = cb ^ 1;
             // 1. collect the triggers of the previous sweep; a node within the band is updated
