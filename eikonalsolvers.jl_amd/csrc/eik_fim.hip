@@ -1166,10 +1166,11 @@ int fim_run(eik_ctx* ctx, int nd, int prec, int kind, const InstDev* d_inst, int
 // Band width of the tile scheduler, in node crossing times (spacing x mean slowness).  A round
 // runs only the active tiles whose pending traveltime lies within the band of the lowest pending
 // one, so tiles far ahead of the front are not relaxed against values that will still change.
-// Measured optima: FP64 about one tile edge in 3D (4: 3 is slower on 201x201x101, 6-8 slower on the
-// 120x150x150 spherical grid), 10-16 in 2D (600x400 x 256 sources: 12 is 17 % faster than 4; 8x8
-// tiles are twice as wide).  FP32 has fewer rounding-level ripples and prefers a wider band: 6 in
-// 3D (6-10 flat on 201x201x101, 6 % faster than 4; 3 % on the spherical grid), 16 in 2D.
+// A wider band means fewer rounds and more updates per node.  Measured optima, FP64: 6 in 3D
+// (201x201x101 x 16 sources: 6-8 within 0.3 % of each other and 4 % faster than 4, 10 slower; the
+// 120x150x150 spherical grid 1 % slower at 6 than at 4-5), 16 in 2D (600x400 x 256 sources: 16-20
+// 2.5 % faster than 12, which is 17 % faster than 4; 8x8 tiles are twice as wide).  FP32: 6 in 3D
+// (6-10 flat on 201x201x101, 6 % faster than 4; 3 % on the spherical grid), 16 in 2D.
 // EIK_BAND overrides the multiplier (0 or less: no band, every active tile runs).
 static double band_mult(int nd, int prec) {
     static const double env = [] {
@@ -1178,9 +1179,10 @@ static double band_mult(int nd, int prec) {
     }();
     if (env == env) return env;
     if (prec == EIK_PREC_FP32) return nd == 3 ? 6.0 : 16.0;
-    // mixed on 201x201x101 x 16 sources: 4 -> 6 crossings 4 % faster, 8 the same as 6
-    if (prec == EIK_PREC_MIXED) return nd == 3 ? 6.0 : 12.0;
-    return nd == 3 ? 4.0 : 12.0;
+    // mixed on 201x201x101 x 16 sources: 4 -> 6 crossings 4 % faster, 8 the same as 6; on 600x400
+    // x 256 sources 16 is 3 % faster than 12
+    if (prec == EIK_PREC_MIXED) return nd == 3 ? 6.0 : 16.0;
+    return nd == 3 ? 6.0 : 16.0;
 }
 
 // EIK_NODEBAND: the same for the nodes inside a tile (default 0: a listed tile relaxes all of
