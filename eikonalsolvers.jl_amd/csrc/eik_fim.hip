@@ -1169,20 +1169,19 @@ int fim_run(eik_ctx* ctx, int nd, int prec, int kind, const InstDev* d_inst, int
 // A wider band means fewer rounds and more updates per node.  Measured optima, FP64: 6 in 3D
 // (201x201x101 x 16 sources: 6-8 within 0.3 % of each other and 4 % faster than 4, 10 slower; the
 // 120x150x150 spherical grid 1 % slower at 6 than at 4-5), 16 in 2D (600x400 x 256 sources: 16-20
-// 2.5 % faster than 12, which is 17 % faster than 4; 8x8 tiles are twice as wide).  FP32: 6 in 3D
-// (6-10 flat on 201x201x101, 6 % faster than 4; 3 % on the spherical grid), 16 in 2D.
-// EIK_BAND overrides the multiplier (0 or less: no band, every active tile runs).
-static double band_mult(int nd, int prec) {
+// 2.5 % faster than 12, which is 17 % faster than 4; 8x8 tiles are twice as wide).  Mixed and
+// FP32: below.  EIK_BAND overrides the multiplier (0 or less: no band, every active tile runs).
+static double band_mult(int nd, int prec, bool sph) {
     static const double env = [] {
         const char* e = getenv("EIK_BAND");
         return e ? atof(e) : NAN;
     }();
     if (env == env) return env;
-    if (prec == EIK_PREC_FP32) return nd == 3 ? 6.0 : 16.0;
-    // mixed on 201x201x101 x 16 sources: 4 -> 6 crossings 4 % faster, 8 the same as 6; on 600x400
-    // x 256 sources 16 is 3 % faster than 12
-    if (prec == EIK_PREC_MIXED) return nd == 3 ? 6.0 : 16.0;
-    return nd == 3 ? 6.0 : 16.0;
+    if (nd == 2) return 16.0;  // mixed on 600x400 x 256 sources: 3 % faster than 12, like FP64
+    if (prec == EIK_PREC_FP64) return 6.0;
+    // mixed and FP32 on 201x201x101 x 16 sources: 8 crossings 2-3 % faster than 6, 10 another 1-2 %;
+    // on the 120x150x150 spherical grid 8 is 2 % slower than 6
+    return sph ? 6.0 : 8.0;
 }
 
 // EIK_NODEBAND: the same for the nodes inside a tile (default 0: a listed tile relaxes all of
@@ -1222,7 +1221,8 @@ static double node_spacing_mean(const InstDev& I, int nd) {
 }
 
 static void set_band(InstDev& I, int nd, int prec, double smean) {
-    const double m = band_mult(nd, prec), mn = nodeband_mult(), ma = adjband_mult();
+    const bool sph = I.kind == EIK_GRID2D_SPHERE || I.kind == EIK_GRID3D_SPHERE;
+    const double m = band_mult(nd, prec, sph), mn = nodeband_mult(), ma = adjband_mult();
     const double cross = node_spacing_mean(I, nd) * smean;
     I.band = m > 0.0 ? m * cross : INFINITY;
     I.nband = mn > 0.0 ? mn * cross : INFINITY;
