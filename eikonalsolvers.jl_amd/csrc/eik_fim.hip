@@ -36,8 +36,14 @@ struct TileShape<3> {
 static_assert(TileShape<2>::TX == fwd_tile_edge(2) && TileShape<3>::TX == fwd_tile_edge(3), "tile edge");
 constexpr int kTileThreads = 64;
 // A run of consecutive list entries claimed by one block is at most 1/kRunDiv of what is left per
-// block (and at most the EIK_CLAIM maximum).
-constexpr unsigned kRunDiv = 4;
+// block (and at most the EIK_CLAIM maximum).  Longer runs relax more tiles right after their
+// neighbour and claim less often; shorter ones let the blocks finish together.  Measured against
+// 4: 2 is 7 % faster on 201x201x101 x 16 sources (FP64; mixed 5 %, FP32 4 %) and 4.5 % on 600x400 x
+// 256, but 1-3 % slower on the 120x150x150 spherical grid, where 3 is 0.5-1 % faster than 4.
+template <bool SPH>
+struct RunDiv {
+    static constexpr unsigned value = SPH ? 3u : 2u;
+};
 // Waves per SIMD the register allocation targets.  The kernel is bound by VALU/SALU issue, not
 // by latency: forcing 3 waves (168 registers) on the FP32 3D kernel measured 8 % slower than 2.
 template <int ND, typename T>
@@ -230,14 +236,14 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(kTileThreads, kTileTh
     // Entries are claimed in runs of consecutive list entries, which are neighbouring tiles (the
     // list is built in tile order): a block relaxing a tile right after its neighbour sees that
     // neighbour's new values, which saves work.  Runs shrink towards the end of the list (at most
-    // `chunk`, at least 1, about a quarter of what is left per block) so the blocks finish
+    // `chunk`, at least 1, a half or a third of what is left per block) so the blocks finish
     // together.  Inside a run the tiles go in increasing pending value (upstream first).  The next
     // run is claimed while the last tile of the current one runs.  The run's entries (at most 32)
     // are read once into LDS, and choosing the next tile is a wave minimum over them.
     const unsigned gsz = gridDim.x;
     auto run_len = [&](unsigned at) {
         const unsigned left = at < nlist ? nlist - at : 0u;
-        const unsigned r = left / (kRunDiv * gsz);
+        const unsigned r = left / (RunDiv<SPH>::value * gsz);
         return r < 1u ? 1u : (r > chunk ? chunk : r);
     };
     unsigned claim = 0u, want = run_len(0u);
