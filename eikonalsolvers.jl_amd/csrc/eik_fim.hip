@@ -38,12 +38,10 @@ constexpr int kTileThreads = 64;
 // A run of consecutive list entries claimed by one block is at most 1/kRunDiv of what is left per
 // block (and at most the EIK_CLAIM maximum).  Longer runs relax more tiles right after their
 // neighbour and claim less often; shorter ones let the blocks finish together.  Measured against
-// 4: 2 is 7 % faster on 201x201x101 x 16 sources (FP64; mixed 5 %, FP32 4 %) and 4.5 % on 600x400 x
-// 256, but 1-3 % slower on the 120x150x150 spherical grid, where 3 is 0.5-1 % faster than 4.
-template <bool SPH>
-struct RunDiv {
-    static constexpr unsigned value = SPH ? 3u : 2u;
-};
+// 4: 2 is 7 % faster on 201x201x101 x 16 sources (FP64; mixed 5 %, FP32 4 %), 4.5 % on 600x400 x
+// 256 and, with the x neighbours' activations merged into the run (k_fim_round), 1 % on the
+// 120x150x150 spherical grid (against 3).
+constexpr unsigned kRunDiv = 2;
 // Waves per SIMD the register allocation targets.  The kernel is bound by VALU/SALU issue, not
 // by latency: forcing 3 waves (168 registers) on the FP32 3D kernel measured 8 % slower than 2.
 template <int ND, typename T>
@@ -236,14 +234,14 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(kTileThreads, kTileTh
     // Entries are claimed in runs of consecutive list entries, which are neighbouring tiles (the
     // list is built in tile order): a block relaxing a tile right after its neighbour sees that
     // neighbour's new values, which saves work.  Runs shrink towards the end of the list (at most
-    // `chunk`, at least 1, a half or a third of what is left per block) so the blocks finish
+    // `chunk`, at least 1, half of what is left per block) so the blocks finish
     // together.  Inside a run the tiles go in increasing pending value (upstream first).  The next
     // run is claimed while the last tile of the current one runs.  The run's entries (at most 32)
     // are read once into LDS, and choosing the next tile is a wave minimum over them.
     const unsigned gsz = gridDim.x;
     auto run_len = [&](unsigned at) {
         const unsigned left = at < nlist ? nlist - at : 0u;
-        const unsigned r = left / (RunDiv<SPH>::value * gsz);
+        const unsigned r = left / (kRunDiv * gsz);
         return r < 1u ? 1u : (r > chunk ? chunk : r);
     };
     unsigned claim = 0u, want = run_len(0u);
@@ -576,9 +574,24 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(kTileThreads, kTileTh
                 atomicMin(&pd[t], v);
                 amin = min(amin, v);
             };
+            // An x neighbour that is the adjacent entry of this run and still to run takes the
+            // activation there, in place of a flag for the next round: its entry merges the face
+            // bit and the pending value.  The block wrote this tile, so that relaxation reads the
+            // new values.  (Not for the run's last tile: s_run already holds the next run.
+            // Relaxing an entry that has already run once more, in the same run, made the runs
+            // longer than the rounds they saved: 12 % slower.)
+            auto run_merge = [&](int k, unsigned t, unsigned bit, unsigned v) {
+                if (last || k < 0 || k >= (int)rlen || ((rdone >> k) & 1u)) return false;
+                const uint4 q = s_run[k];
+                if (q.x != t || q.y != e.y) return false;
+                s_run[k] = make_uint4(q.x, q.y, min(q.z, v), q.w | bit);
+                return true;
+            };
             // the x- neighbour sees this tile across its x+ face, and so on
-            if (fmin[0] != 0x7f800000u && tix > 0) act_tile(tl - 1, 1u << 1, fmin[0]);
-            if (fmin[1] != 0x7f800000u && tix < (unsigned)I.t[0] - 1) act_tile(tl + 1, 1u << 0, fmin[1]);
+            if (fmin[0] != 0x7f800000u && tix > 0 && !run_merge(bk - 1, tl - 1, 1u << 1, fmin[0]))
+                act_tile(tl - 1, 1u << 1, fmin[0]);
+            if (fmin[1] != 0x7f800000u && tix < (unsigned)I.t[0] - 1 && !run_merge(bk + 1, tl + 1, 1u << 0, fmin[1]))
+                act_tile(tl + 1, 1u << 0, fmin[1]);
             if (fmin[2] != 0x7f800000u && tiy > 0) act_tile(tl - t0, 1u << 3, fmin[2]);
             if (fmin[3] != 0x7f800000u && tiy < (unsigned)I.t[1] - 1) act_tile(tl + t0, 1u << 2, fmin[3]);
             if (ND == 3) {
