@@ -38,10 +38,11 @@ constexpr int kTileThreads = 64;
 // A run of consecutive list entries claimed by one block is at most 1/kRunDiv of what is left per
 // block (and at most the EIK_CLAIM maximum).  Longer runs relax more tiles right after their
 // neighbour and claim less often; shorter ones let the blocks finish together.  Measured against
-// 4: 2 is 7 % faster on 201x201x101 x 16 sources (FP64; mixed 5 %, FP32 4 %), 4.5 % on 600x400 x
-// 256 and, with the x neighbours' activations merged into the run (k_fim_round), 1 % on the
-// 120x150x150 spherical grid (against 3).
-constexpr unsigned kRunDiv = 2;
+// 4: 2 is 7 % faster on 201x201x101 x 16 sources (FP64; mixed 5 %, FP32 4 %) and 4.5 % on 600x400 x
+// 256; with the x neighbours' activations merged into the run (k_fim_round), 1 is another 1-3 %
+// faster on 201x201x101 and 1.5 % on the 120x150x150 spherical grid (the maximum of 8 entries
+// binds; 12 is slower).
+constexpr unsigned kRunDiv = 1;
 // Waves per SIMD the register allocation targets.  The kernel is bound by VALU/SALU issue, not
 // by latency: forcing 3 waves (168 registers) on the FP32 3D kernel measured 8 % slower than 2.
 template <int ND, typename T>
@@ -234,7 +235,7 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(kTileThreads, kTileTh
     // Entries are claimed in runs of consecutive list entries, which are neighbouring tiles (the
     // list is built in tile order): a block relaxing a tile right after its neighbour sees that
     // neighbour's new values, which saves work.  Runs shrink towards the end of the list (at most
-    // `chunk`, at least 1, half of what is left per block) so the blocks finish
+    // `chunk`, at least 1, what is left per block) so the blocks finish
     // together.  Inside a run the tiles go in increasing pending value (upstream first).  The next
     // run is claimed while the last tile of the current one runs.  The run's entries (at most 32)
     // are read once into LDS, and choosing the next tile is a wave minimum over them.
