@@ -232,10 +232,24 @@ __global__ void __launch_bounds__(kAB) k_adj_round(const int64_t* __restrict__ l
 #pragma clang fp contract(off)
     using S = AdjTile<ND>;
     constexpr int NL = S::LX * S::LY * S::LZ;
+    // G along axis d is read only by rows along d, so s_G[d] holds the core extended by the halo
+    // on the two faces across d (L = T + 2H along d, T along the other axes)
+    constexpr int NG = S::LX * S::TY * (ND == 3 ? S::TZ : 1);
+    static_assert(S::TX == S::TY && (ND == 2 || S::TZ == S::TX), "cubic adjoint tiles");
     __shared__ double s_lam[NL];
-    __shared__ double s_G[ND][NL];
+    __shared__ double s_G[ND][NG];
     __shared__ uint16_t s_code[NL];
-    __shared__ unsigned s_fmax[7];  // per face (and whole tile): largest traveltime of a changed row
+    __shared__ unsigned s_fmax[7];
+    // slot of local node (halo coordinates lc) in s_G[d]: the other axes drop their halo
+    auto gslot = [](int d, const int lc[3]) {
+        int s = 0, m = 1;
+    #pragma unroll
+        for (int a = 0; a < ND; a++) {
+            s += (a == d ? lc[a] : lc[a] - S::H) * m;
+            m *= a == d ? S::LX : S::TX;
+        }
+        return s;
+    };  // per face (and whole tile): largest traveltime of a changed row
     const int tid = threadIdx.x;
     const unsigned nlist = *slot;
     if (blockIdx.x == 0) {
@@ -289,8 +303,15 @@ __global__ void __launch_bounds__(kAB) k_adj_round(const int64_t* __restrict__ l
             }
             s_lam[c] = lv;
             s_code[c] = cv;
+            const int lc[3] = {lx, ly, lz};
     #pragma unroll
-            for (int d = 0; d < ND; d++) s_G[d][c] = gv[d];
+            for (int d = 0; d < ND; d++) {
+                bool in_d = true;
+    #pragma unroll
+                for (int a = 0; a < ND; a++)
+                    if (a != d) in_d = in_d && lc[a] >= S::H && lc[a] < S::H + S::TX;
+                if (in_d) s_G[d][gslot(d, lc)] = gv[d];
+            }
         }
         __syncthreads();
 
@@ -301,6 +322,7 @@ __global__ void __launch_bounds__(kAB) k_adj_round(const int64_t* __restrict__ l
         bool act[S::NP];
         int lidx[S::NP], gxyz[S::NP][3];
         const int strides[3] = {1, S::LX, S::LX * S::LY};
+        const int gstrides[3] = {1, S::TX, S::TX * S::TY};  // along axis d within s_G[d]
     #pragma unroll
         for (int k = 0; k < NS; k++) {
             const int d = k / 4, o = (k % 4 < 2) ? ((k % 2) ? -1 : 1) : ((k % 2) ? -2 : 2);
@@ -315,6 +337,7 @@ __global__ void __launch_bounds__(kAB) k_adj_round(const int64_t* __restrict__ l
             gxyz[p][1] = g[1];
             gxyz[p][2] = g[2];
             lidx[p] = (lx + S::H) + S::LX * ((ly + S::H) + S::LY * (ND == 3 ? lz + S::H : 0));
+            const int lcore[3] = {lx + S::H, ly + S::H, ND == 3 ? lz + S::H : 0};
             const bool inside = g[0] < I.n[0] && g[1] < I.n[1] && (ND == 2 || g[2] < I.n[2]);
             const uint16_t cj = s_code[lidx[p]];
             bool hasrow = false;
@@ -328,7 +351,7 @@ __global__ void __launch_bounds__(kAB) k_adj_round(const int64_t* __restrict__ l
             if (act[p]) {
                 double Gj[3] = {0, 0, 0};
     #pragma unroll
-                for (int d = 0; d < ND; d++) Gj[d] = s_G[d][lidx[p]];
+                for (int d = 0; d < ND; d++) Gj[d] = s_G[d][gslot(d, lcore)];
                 diag[p] = adj_diag<ND>(I, g, cj, Gj);
                 bj[p] = b[g[0] + nx * g[1] + nxy * g[2]];
     #pragma unroll
@@ -338,7 +361,7 @@ __global__ void __launch_bounds__(kAB) k_adj_round(const int64_t* __restrict__ l
                     gi[d] += o;
                     if (gi[d] < 0 || gi[d] > I.n[d] - 1) continue;
                     const int li = lidx[p] + off[k];
-                    w[p][k] = contrib(I, gi, d, o, s_code[li], s_G[d][li]);
+                    w[p][k] = contrib(I, gi, d, o, s_code[li], s_G[d][gslot(d, lcore) + o * gstrides[d]]);
                 }
             }
             l0[p] = s_lam[lidx[p]];
