@@ -41,7 +41,7 @@ def run_steps(model, src, recs, ep, steps, warmup):
         model.traveltime(src, recs, extraparams=ep)
     tot = {"_kernel_ms": 0.0, "tile_activations": 0.0, "rounds": 0.0, "node_updates": 0.0, "changed_updates": 0.0,
            "tile_sweeps": 0.0, "launches": 0.0, "fine_kernel_ms": 0.0, "fine_launches": 0.0,
-           "fine_tile_activations": 0.0}
+           "fine_tile_activations": 0.0, "kernel_busy_ms": 0.0, "all_kernel_busy_ms": 0.0}
     t0 = time.perf_counter()
     for _ in range(steps):
         model.traveltime(src, recs, extraparams=ep)  # synchronous: returns after the device is idle
@@ -135,7 +135,12 @@ def _roofline(prec: str, f: dict) -> dict:
         "frac": round(f["achieved_gbs"] / HBM_PEAK_GBS, 5) if f["achieved_gbs"] else None,
         "traffic": None,
         "traffic_scope": "HBM bytes per launch of this kernel (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a plain "
-                         "copy of known size); achieved = algorithmic bytes per launch / average launch time",
+                         "copy of known size)",
+        "achieved_scope": "algorithmic bytes of the step's launches / time in which at least one of them ran (the "
+                          "sources are split over two streams whose launches overlap); achieved_per_launch = "
+                          "algorithmic bytes per launch / average launch duration",
+        "achieved_per_launch": round(f["per_launch_gbs"], 2) if f["per_launch_gbs"] else None,
+        "kernel_busy_ms_per_step": round(f["kernel_busy_ms_per_step"], 3),
         "bytes_per_node_visit": BYTES_PER_VISIT[prec],
         "algorithmic_bytes_per_launch": round(alg_per_launch),
         "launches_per_step": launches,
@@ -267,11 +272,12 @@ def main() -> int:
         nn = args.steps * nsrc * grd.npts
         # the round kernel as a profiler sees it: in FP64 the refined boxes around the sources are
         # solved by the same kernel instantiation as the coarse grid, so their launches count too
-        k_ms, k_visits, k_launches = kern_ms, visits, tot["launches"]
+        k_ms, k_visits, k_launches, k_busy = kern_ms, visits, tot["launches"], tot["kernel_busy_ms"]
         if prec == "fp64":
             k_ms += tot["fine_kernel_ms"]
             k_visits += tot["fine_tile_activations"] * TILE_NODES_3D
             k_launches += tot["fine_launches"]
+            k_busy = tot["all_kernel_busy_ms"]
         res[prec] = {
             "solves_per_s": nsrc * args.steps / dt,
             "ms_per_step": 1e3 * dt / args.steps,
@@ -280,12 +286,15 @@ def main() -> int:
             "launches_per_step": k_launches / args.steps,
             "kernel_visits_per_step": k_visits / args.steps,
             "kernel_all_ms_per_step": k_ms / args.steps,
+            "kernel_busy_ms_per_step": k_busy / args.steps,
             "node_visits_per_step": visits / args.steps,
             "visits_per_node": visits / nn,
             "updates_per_node": tot["node_updates"] / nn,
             "changes_per_node": tot["changed_updates"] / nn,
             "sweeps_per_activation": tot["tile_sweeps"] / max(tot["tile_activations"], 1.0),
-            "achieved_gbs": (k_visits * BYTES_PER_VISIT[prec.replace("_fast", "")] / 1e9) / (k_ms / 1e3)
+            "achieved_gbs": (k_visits * BYTES_PER_VISIT[prec.replace("_fast", "")] / 1e9) / (k_busy / 1e3)
+            if k_busy > 0 else None,
+            "per_launch_gbs": (k_visits * BYTES_PER_VISIT[prec.replace("_fast", "")] / 1e9) / (k_ms / 1e3)
             if k_ms > 0 else None,
         }
     if args.legs != "all":

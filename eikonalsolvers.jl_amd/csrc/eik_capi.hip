@@ -2,6 +2,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <thread>
 
 #include "eik_internal.h"
 
@@ -194,6 +195,19 @@ const char* eik_version(void) { return "libeik 0.1 (gfx950)"; }
 
 const char* eik_last_error(const eik_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
+// stream, counters and timers of a context (a peer shares its owner's tbase)
+static bool ctx_init(eik_ctx* ctx, int device, bool with_tbase) {
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc(&ctx->d_small, kSmallBytes) != hipSuccess ||
+        hipHostMalloc(&ctx->h_small, kSmallBytes, hipHostMallocDefault) != hipSuccess ||
+        hipEventCreate(&ctx->ev[0]) != hipSuccess || hipEventCreate(&ctx->ev[1]) != hipSuccess)
+        return false;
+    for (auto& e : ctx->kev)
+        if (hipEventCreate(&e) != hipSuccess) return false;
+    return !with_tbase || hipEventCreate(&ctx->tbase) == hipSuccess;
+}
+
 int eik_ctx_create(int32_t device, eik_ctx** out) {
     if (!out) return EIK_EINVAL;
     *out = nullptr;
@@ -201,19 +215,10 @@ int eik_ctx_create(int32_t device, eik_ctx** out) {
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return EIK_EDEVICE;
     if (device < 0 || device >= ndev) return EIK_EINVAL;
     eik_ctx* ctx = new eik_ctx();
-    ctx->device = device;
-    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipMalloc(&ctx->d_small, kSmallBytes) != hipSuccess ||
-        hipHostMalloc(&ctx->h_small, kSmallBytes, hipHostMallocDefault) != hipSuccess ||
-        hipEventCreate(&ctx->ev[0]) != hipSuccess || hipEventCreate(&ctx->ev[1]) != hipSuccess) {
+    if (!ctx_init(ctx, device, true)) {
         eik_ctx_destroy(ctx);
         return EIK_EDEVICE;
     }
-    for (auto& e : ctx->kev)
-        if (hipEventCreate(&e) != hipSuccess) {
-            eik_ctx_destroy(ctx);
-            return EIK_EDEVICE;
-        }
     // the no-allocation entry point's scratch block: this device's address, resolved once
     scratch_setup(device);
     *out = ctx;
@@ -222,6 +227,7 @@ int eik_ctx_create(int32_t device, eik_ctx** out) {
 
 void eik_ctx_destroy(eik_ctx* ctx) {
     if (!ctx) return;
+    if (ctx->peer) eik_ctx_destroy(ctx->peer);
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (auto& kv : ctx->bufs) kv.second.release();
@@ -231,6 +237,7 @@ void eik_ctx_destroy(eik_ctx* ctx) {
         if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
     for (auto& e : ctx->kev)
         if (e) (void)hipEventDestroy(e);
+    if (ctx->tbase && !ctx->owner) (void)hipEventDestroy(ctx->tbase);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -323,6 +330,58 @@ void eik_model_destroy(eik_model* M) {
     delete M;
 }
 
+// Streams a traveltime call's sources are split over (EIK_STREAMS = 1 keeps one).  Each half runs
+// its own round loop on its own stream, so while the last tiles of one half's round finish, the
+// other half's round kernel takes the CUs they leave idle (201x201x101 x 16 sources: +11 % FP64,
+// +14 % FP32; 600x400 x 256: +28 %; 120x150x150 spherical x 32: +20 %; four streams are slower
+// than two).  Calls below kSplitNodes source-nodes stay on one stream (300x220 x 4 sources: the
+// second thread and stream cost 6 %).  Results do not depend on the split: every source is solved
+// on its own.
+constexpr int64_t kSplitNodes = (int64_t)1 << 22;
+static int split_streams() {
+    static const int v = [] {
+        const char* e = getenv("EIK_STREAMS");
+        return e ? std::min(2, std::max(1, atoi(e))) : 2;
+    }();
+    return v;
+}
+
+// the context's peer, created on first use
+static eik_ctx* get_peer(eik_ctx* ctx) {
+    if (!ctx->peer) {
+        eik_ctx* p = new eik_ctx();
+        p->owner = ctx;
+        if (!ctx_init(p, ctx->device, false)) {
+            eik_ctx_destroy(p);
+            return nullptr;
+        }
+        p->tbase = ctx->tbase;
+        ctx->peer = p;
+    }
+    return ctx->peer;
+}
+
+// sources [a, b) of S, receivers re-based
+static SourceSet sub_sources(const SourceSet& S, int64_t a, int64_t b) {
+    SourceSet T;
+    T.nsrc = b - a;
+    T.nd = S.nd;
+    T.src.assign(S.src.begin() + a * S.nd, S.src.begin() + b * S.nd);
+    T.rec_off.resize((size_t)(b - a + 1));
+    for (int64_t i = 0; i <= b - a; i++) T.rec_off[i] = S.rec_off[a + i] - S.rec_off[a];
+    T.rec.assign(S.rec.begin() + S.rec_off[a] * S.nd, S.rec.begin() + S.rec_off[b] * S.nd);
+    return T;
+}
+
+// a peer's statistics folded into its owner's: counts and launch durations add, phase times and
+// the settling stage take the larger, [17] is the union of both streams' round launches
+static void merge_peer_stats(eik_ctx* ctx) {
+    const double* q = ctx->peer->stats;
+    double* s = ctx->stats;
+    for (int i : {1, 2, 11}) s[i] = std::max(s[i], q[i]);
+    for (int i : {4, 5, 6, 7, 8, 9, 10, 12, 13, 14, 15, 16}) s[i] += q[i];
+}
+
 int eik_traveltime(eik_ctx* ctx, eik_model* model, int64_t nsrc, const double* coordsrc, const int64_t* rec_off,
                    const double* coordrec, const eik_params* params_in, double* ttpicks, double* ttgrid) {
     if (!ctx || !model) return EIK_EINVAL;
@@ -335,13 +394,42 @@ int eik_traveltime(eik_ctx* ctx, eik_model* model, int64_t nsrc, const double* c
     const eik_params* params = &pe;
     if (S.nrec() > 0 && !ttpicks) return ctx->fail(EIK_EINVAL, "null picks buffer");
     if (nsrc == 0) return EIK_OK;
-    for (double& x : ctx->stats) x = 0.0;
+    stats_begin(ctx);
     Timer total(ctx);
-    rc = forward_solve(ctx, model, S, params->precision, params);
+    const bool split = ctx->split && nsrc >= 2 && nsrc * model->N >= kSplitNodes && split_streams() == 2;
+    eik_ctx* peer = split ? get_peer(ctx) : nullptr;
+    if (!peer) {
+        rc = forward_solve(ctx, model, S, params->precision, params);
+        if (rc) return rc;
+        rc = forward_picks(ctx, model, S, params->precision, ttpicks, ttgrid);
+        ctx->stats[0] = total.mark();
+        ctx->stats[17] = busy_ms(ctx, false, false);
+        ctx->stats[18] = busy_ms(ctx, false, true);
+        return rc;
+    }
+    // the second half of the sources on the peer's stream, from a thread of its own
+    const int64_t h = (nsrc + 1) / 2;
+    const SourceSet S0 = sub_sources(S, 0, h), S1 = sub_sources(S, h, nsrc);
+    double* picks1 = ttpicks ? ttpicks + S.rec_off[h] : nullptr;
+    double* grid1 = ttgrid ? ttgrid + h * model->N : nullptr;
+    stats_begin(peer);
+    CK(hipEventSynchronize(ctx->tbase));  // the peer's launch spans are timed from it
+    int rc1 = EIK_OK;
+    std::thread th([&] {
+        (void)hipSetDevice(peer->device);
+        rc1 = forward_solve(peer, model, S1, params->precision, params);
+        if (rc1 == EIK_OK) rc1 = forward_picks(peer, model, S1, params->precision, picks1, grid1);
+    });
+    rc = forward_solve(ctx, model, S0, params->precision, params);
+    if (rc == EIK_OK) rc = forward_picks(ctx, model, S0, params->precision, ttpicks, ttgrid);
+    th.join();
     if (rc) return rc;
-    rc = forward_picks(ctx, model, S, params->precision, ttpicks, ttgrid);
+    if (rc1) return ctx->fail(rc1, "%s", peer->err.c_str());
+    merge_peer_stats(ctx);
     ctx->stats[0] = total.mark();
-    return rc;
+    ctx->stats[17] = busy_ms(ctx, true, false);
+    ctx->stats[18] = busy_ms(ctx, true, true);
+    return EIK_OK;
 }
 
 int eik_traveltime_batch(eik_ctx* ctx, int32_t nreq, const eik_request* reqs) {
@@ -367,13 +455,15 @@ int eik_traveltime_batch(eik_ctx* ctx, int32_t nreq, const eik_request* reqs) {
     }
     if (G.empty()) return EIK_OK;
     const eik_params pe = effective_params(&reqs[0].params, reqs[0].model->g.kind);
-    for (double& x : ctx->stats) x = 0.0;
+    stats_begin(ctx);
     Timer total(ctx);
     int rc = forward_solve(ctx, G, pe.precision, &pe);
     if (rc) return rc;
     std::vector<double> picks((size_t)std::max<int64_t>(nrec, 1));
     rc = forward_picks(ctx, G, pe.precision, picks.data(), nullptr);
     ctx->stats[0] = total.mark();
+    ctx->stats[17] = busy_ms(ctx, false, false);
+    ctx->stats[18] = busy_ms(ctx, false, true);
     if (rc) return rc;
     int64_t off = 0;
     for (int32_t i = 0; i < nreq; i++) {
@@ -440,10 +530,12 @@ int eik_gradient(eik_ctx* ctx, eik_model* model, int64_t nsrc, const double* coo
         if (gradvel && whichgrad != EIK_GRADSRCLOC) memset(gradvel, 0, model->N * sizeof(double));
         return EIK_OK;
     }
-    for (double& x : ctx->stats) x = 0.0;
+    stats_begin(ctx);
     Timer total(ctx);
     rc = gradient_solve(ctx, model, S, params, whichgrad, pickobs, stdobs, gradvel, gradsrc, misf);
     ctx->stats[0] = total.mark();
+    ctx->stats[17] = busy_ms(ctx, false, false);
+    ctx->stats[18] = busy_ms(ctx, false, true);
     return rc;
 }
 
@@ -460,7 +552,7 @@ int eik_mem_info(eik_ctx* ctx, int64_t* free_bytes, int64_t* total_bytes) {
 
 int eik_last_stats(const eik_ctx* ctx, double* out, int32_t nout) {
     if (!ctx || !out) return EIK_EINVAL;
-    for (int i = 0; i < nout && i < 17; i++) out[i] = ctx->stats[i];
+    for (int i = 0; i < nout && i < kNumStats; i++) out[i] = ctx->stats[i];
     return EIK_OK;
 }
 

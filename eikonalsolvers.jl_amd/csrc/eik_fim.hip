@@ -1125,6 +1125,12 @@ static int fim_run_t(eik_ctx* ctx, const InstDev* d_inst, int ninst, int64_t nti
             float ms = 0.f;
             CK(hipEventElapsedTime(&ms, ctx->kev[2 * r], ctx->kev[2 * r + 1]));
             stt.kernel_ms += ms;
+            if (ctx->tbase) {
+                float a = 0.f, b = 0.f;
+                CK(hipEventElapsedTime(&a, ctx->tbase, ctx->kev[2 * r]));
+                CK(hipEventElapsedTime(&b, ctx->tbase, ctx->kev[2 * r + 1]));
+                ctx->kspans.push_back(KSpan{a, b, ctx->span_fine});
+            }
         }
         stt.launches += kRoundBatch;
         if (h_cnt[kCntFlagged] == 0u) {
@@ -1455,8 +1461,10 @@ int forward_solve(eik_ctx* ctx, const std::vector<FwdGroup>& G, int prec, const 
         CK(hipGetLastError());
         tm.mark();
         FimStats fst{};
+        ctx->span_fine = true;
         int rc = fim_run(ctx, nd, EIK_PREC_FP64, kind, (const InstDev*)dfi.p, (int)nsrc, ntiles_f, fpools, slf.p,
                          P->fast == 0, false, &fst);
+        ctx->span_fine = false;
         if (rc) return rc;
         ctx->stats[1] = tm.mark();
         ctx->stats[14] = fst.kernel_ms;

@@ -352,6 +352,14 @@ constexpr size_t kSmallBytes = 256;
 constexpr int kCntFlagged = 2, kCntSweeps = 3, kCntRounds = 4, kCntSlots = 8, kCnt64 = 16;
 constexpr int kSlotList = 0, kSlotCursor = 1, kSlotWords = 2;
 
+// eik_last_stats entries
+constexpr int kNumStats = 19;
+// one round launch: start and end (ms after the call's base event), near-source (fine) solve or not
+struct KSpan {
+    float a, b;
+    bool fine;
+};
+
 struct eik_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -361,7 +369,18 @@ struct eik_ctx {
     void* h_small = nullptr;  // pinned host mirror of the counters
     hipEvent_t ev[2] = {nullptr, nullptr};   // phase timer
     hipEvent_t kev[2 * kRoundBatch] = {};    // round-kernel timers (start, end) of one round batch
-    double stats[17] = {};
+    double stats[kNumStats] = {};
+    // Round-launch spans of the current call (ms after tbase, an event recorded when the call
+    // starts): with the sources split over this context and its peer, the launches of the two
+    // streams overlap, and the union of the spans is the time the round kernel held the GPU.
+    hipEvent_t tbase = nullptr;
+    std::vector<KSpan> kspans;
+    bool span_fine = false;  // the launches being timed belong to the near-source solve
+    // a second context on its own stream that solves part of a call's sources concurrently
+    // (created on first use, owned by this context, only used while this context's lock is held)
+    eik_ctx* peer = nullptr;
+    eik_ctx* owner = nullptr;  // set on a peer: the context it belongs to (whose tbase it shares)
+    bool split = true;         // large calls may split their sources with the peer
     int xcap = 1;  // capacity per source of the transferred-point list
     std::mutex mu;
     // arena mode (the no-allocation entry point): buffers are carved from this block instead
@@ -389,13 +408,42 @@ namespace eik {
 inline int64_t ctx_pool_bytes(const eik_ctx* ctx) {
     int64_t b = 0;
     for (const auto& kv : ctx->bufs) b += (int64_t)kv.second.cap;
-    return b;
+    return b + (ctx->peer ? ctx_pool_bytes(ctx->peer) : 0);
+}
+// a call's statistics start from zero and its round-launch spans are timed from now
+inline void stats_begin(eik_ctx* ctx) {
+    for (double& x : ctx->stats) x = 0.0;
+    ctx->kspans.clear();
+    if (ctx->tbase) (void)hipEventRecord(ctx->tbase, ctx->stream);
+}
+// length of the union of the round-launch spans of a call on this context and its peer (ms):
+// the coarse solve's launches, and the near-source solve's too if `fine`
+inline double busy_ms(const eik_ctx* ctx, bool with_peer, bool fine) {
+    std::vector<std::pair<float, float>> v;
+    for (const eik_ctx* c : {ctx, with_peer ? (const eik_ctx*)ctx->peer : nullptr})
+        if (c)
+            for (const KSpan& k : c->kspans)
+                if (fine || !k.fine) v.emplace_back(k.a, k.b);
+    std::sort(v.begin(), v.end());
+    double tot = 0.0, lo = 0.0, hi = -1.0;
+    for (const auto& x : v) {
+        if (x.first > hi) {
+            if (hi > lo) tot += hi - lo;
+            lo = x.first;
+            hi = x.second;
+        } else if (x.second > hi) {
+            hi = x.second;
+        }
+    }
+    if (hi > lo) tot += hi - lo;
+    return tot;
 }
 // release a context's reusable pools once its stream is idle (the next call allocates again)
 inline void ctx_trim(eik_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (auto& kv : ctx->bufs) kv.second.release();
+    if (ctx->peer) ctx_trim(ctx->peer);
 }
 }  // namespace eik
 

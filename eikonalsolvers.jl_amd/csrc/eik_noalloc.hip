@@ -128,11 +128,15 @@ extern "C" int eik_traveltime_noalloc(eik_ctx* ctx, eik_model* model, int64_t ns
     A.cap = kScratchBytes;
     ctx->abufs.clear();
     ctx->arena = &A;
-    for (double& x : ctx->stats) x = 0.0;
+    stats_begin(ctx);
     Timer total(ctx);
     rc = forward_solve(ctx, model, S, pe.precision, &pe);
     if (rc == EIK_OK) rc = forward_picks(ctx, model, S, pe.precision, ttpicks, ttgrid);
-    if (rc == EIK_OK) ctx->stats[0] = total.mark();
+    if (rc == EIK_OK) {
+        ctx->stats[0] = total.mark();
+        ctx->stats[17] = busy_ms(ctx, false, false);
+        ctx->stats[18] = busy_ms(ctx, false, true);
+    }
     // nothing of this call may still run on the block when the next call takes it
     (void)hipStreamSynchronize(ctx->stream);
     ctx->arena = nullptr;

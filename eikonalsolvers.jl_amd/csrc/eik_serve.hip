@@ -303,6 +303,7 @@ int eik_server_create(int32_t device, int32_t nslots, int64_t mem_budget, eik_se
             delete S;
             return rc;
         }
+        c->split = false;  // the slots already run concurrently, one stream each
         S->slots.push_back(c);
     }
     size_t fr = 0, tot = 0;

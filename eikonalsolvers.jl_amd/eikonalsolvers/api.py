@@ -48,13 +48,14 @@ class Context:
             pass
 
     def last_stats(self):
-        out = np.zeros(17)
-        _lib.load().eik_last_stats(self.handle, _lib.ptr(out), 17)
+        out = np.zeros(19)
+        _lib.load().eik_last_stats(self.handle, _lib.ptr(out), 19)
         return {"total_ms": out[0], "fine_ms": out[1], "coarse_ms": out[2], "adjoint_ms": out[3],
                 "tile_activations": out[4], "rounds": out[5], "_kernel_ms": out[6], "tile_sweeps": out[7],
                 "node_updates": out[8], "changed_updates": out[9], "key_only_updates": out[10],
                 "settle_stage": int(out[11]), "verify_rounds": int(out[12]), "launches": out[13],
-                "fine_kernel_ms": out[14], "fine_launches": out[15], "fine_tile_activations": out[16]}
+                "fine_kernel_ms": out[14], "fine_launches": out[15], "fine_tile_activations": out[16],
+                "kernel_busy_ms": out[17], "all_kernel_busy_ms": out[18]}
 
 
 _default_ctx = None

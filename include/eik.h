@@ -143,8 +143,14 @@ int eik_mem_info(eik_ctx* ctx, int64_t* free_bytes, int64_t* total_bytes);
  * [12] = verification rounds of the coarse solve (at convergence every tile is relaxed once more;
  * one round that changes nothing ends the solve), [13] = round-kernel launches of the coarse solve
  * (rounds that found no tile included), [14] = summed duration of the near-source (fine grid)
- * solve's round launches (ms), [15] = those launches, [16] = that solve's tile activations.
- * Values a call did not produce are 0. */
+ * solve's round launches (ms), [15] = those launches, [16] = that solve's tile activations,
+ * [17] = time in which at least one round launch of the coarse solve ran (ms, the union of the
+ * launches' spans over both streams of a split call; [6] sums overlapping launches), [18] = the
+ * same over the launches of both solves (near-source and coarse).
+ * eik_traveltime splits two or more sources (at least 2^22 nodes over all of them) into two halves
+ * on two streams (the context's own and a second one it creates on first use; not on a server's
+ * slots): the counts above are then summed over both halves, [1], [2] and [11] are the larger of
+ * the two.  Values a call did not produce are 0. */
 int eik_last_stats(const eik_ctx* ctx, double* out, int32_t nout);
 
 /* tracerays (src/EikSolv/eikutils.jl:201-328): for every source's receivers, the ray descending
