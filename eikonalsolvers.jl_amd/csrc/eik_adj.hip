@@ -750,16 +750,18 @@ __device__ __forceinline__ double taper_factor(const TaperDev& T, const InstDev&
 // The reference reuses one per-chunk gradient buffer across sources and only overwrites the
 // non-source nodes (T2Vb), so a source node starts from the previous source's tapered value
 // before T2Va and T1Va are added (calcgradsomesrc, eikmaingrad.jl:182-247).  Sequential over
-// sources, parallel over each source's points.
+// sources, parallel over each source's points.  carry (or null): the tapered image of the source
+// before the first one here (k_taper multiplies by the same factor, so it holds the same bits).
 __global__ void k_grad_quirk(const SrcPt* __restrict__ pts, const int* __restrict__ poff, int nsrc,
                              const InstDev* __restrict__ inst, const TaperDev* __restrict__ tap, int nd, int kind,
-                             const double* __restrict__ t2, const double* __restrict__ t1, double* __restrict__ img) {
+                             const double* __restrict__ t2, const double* __restrict__ t1, double* __restrict__ img,
+                             const double* __restrict__ carry) {
 #pragma clang fp contract(off)
     for (int s = 0; s < nsrc; s++) {
         for (int q = poff[s] + threadIdx.x; q < poff[s + 1]; q += blockDim.x) {
             const InstDev I = inst[s];
             const int64_t l = pts[q].node;
-            double prev = 0.0;
+            double prev = s == 0 && carry ? carry[l] : 0.0;
             if (s > 0) {
                 const int g[3] = {(int)(l % I.n[0]), (int)((l / I.n[0]) % I.n[1]),
                                   (int)(l / ((int64_t)I.n[0] * I.n[1]))};
@@ -791,13 +793,16 @@ __global__ void k_taper(const InstDev* __restrict__ inst, const TaperDev* __rest
     }
 }
 
-// gradvelall .+= gradvel1, source by source (eikmaingrad.jl:246)
-__global__ void k_sum_sources(const double* __restrict__ img, int nsrc, int64_t N, double* __restrict__ out) {
+// gradvelall .+= gradvel1, source by source (eikmaingrad.jl:246), over the images of the first
+// part's sources, then the second's
+__global__ void k_sum_sources(const double* __restrict__ img0, int n0, const double* __restrict__ img1, int n1,
+                              int64_t N, double* __restrict__ out) {
 #pragma clang fp contract(off)
     const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= N) return;
     double a = 0.0;
-    for (int s = 0; s < nsrc; s++) a += img[(int64_t)s * N + l];
+    for (int s = 0; s < n0; s++) a += img0[(int64_t)s * N + l];
+    for (int s = 0; s < n1; s++) a += img1[(int64_t)s * N + l];
     out[l] = a;
 }
 
@@ -999,8 +1004,8 @@ static void taper_box(const eik_grid& g, const double* src, int R, TaperDev& T) 
     }
 }
 
-int gradient_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, const eik_params* P, int whichgrad,
-                   const double* pickobs, const double* stdobs, double* gradvel, double* gradsrc, double* misf) {
+int gradient_part(eik_ctx* ctx, eik_model* M, const SourceSet& S, const eik_params* P, int whichgrad,
+                  const double* pickobs, const double* stdobs, double* gradsrc, bool defer, GradPart* out) {
     const int nd = M->g.ndim, kind = M->g.kind, prec = P->precision;
     const int64_t N = M->N, nsrc = S.nsrc, nrec = S.nrec();
     const bool refine = P->refinearoundsrc != 0;
@@ -1036,13 +1041,13 @@ int gradient_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, const eik_par
 
     // receivers: rhs, misfit
     DevBuf &drec = ctx->buf("rec"), &drs = ctx->buf("recsrc"), &dpo = ctx->buf("pobs"), &dso = ctx->buf("sobs"),
-           &dmt = ctx->buf("mterm"), &droff = ctx->buf("rec_off"), &dmis = ctx->buf("misf");
+           &dmt = ctx->buf("mterm"), &droff = ctx->buf("rec_off");
     if (drec.ensure((size_t)std::max<int64_t>(nrec, 1) * nd * sizeof(double)) ||
         drs.ensure((size_t)std::max<int64_t>(nrec, 1) * sizeof(int)) ||
         dpo.ensure((size_t)std::max<int64_t>(nrec, 1) * sizeof(double)) ||
         dso.ensure((size_t)std::max<int64_t>(nrec, 1) * sizeof(double)) ||
         dmt.ensure((size_t)std::max<int64_t>(nrec, 1) * sizeof(double)) ||
-        droff.ensure((size_t)(nsrc + 1) * sizeof(int64_t)) || dmis.ensure(sizeof(double)))
+        droff.ensure((size_t)(nsrc + 1) * sizeof(int64_t)))
         return ctx->fail(EIK_ENOMEM, "out of device memory (receivers)");
     std::vector<int> rs((size_t)std::max<int64_t>(nrec, 1));
     for (int64_t s = 0; s < nsrc; s++)
@@ -1067,8 +1072,10 @@ int gradient_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, const eik_par
                                                             (const double*)dso.p, (double*)b_c.p, (double*)dmt.p, flags, apend);
         CK(hipGetLastError());
     }
-    k_misfit<<<1, kMisfitThreads, 0, st>>>((const double*)dmt.p, (const int64_t*)droff.p, nsrc, (double*)dmis.p);
-    CK(hipMemcpyAsync(misf, dmis.p, sizeof(double), hipMemcpyDeviceToHost, st));
+    *out = GradPart{};
+    out->mterm = (const double*)dmt.p;
+    out->nrec = nrec;
+    out->nsrc = nsrc;
 
     FimStats ast{};
     rc = adj_run(ctx, nd, prec, dci, (int)nsrc, nsrc * tiles_per, (const uint16_t*)code_c.p, (const double*)G_c.p,
@@ -1223,10 +1230,9 @@ int gradient_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, const eik_par
         }
     }
 
-    if (gradvel && whichgrad != EIK_GRADSRCLOC) {
-        DevBuf &img = ctx->buf("grad_img"), &dtap = ctx->buf("taper"), &gout = ctx->buf("grad_out");
-        if (img.ensure((size_t)P_c * sizeof(double)) || dtap.ensure((size_t)nsrc * sizeof(TaperDev)) ||
-            gout.ensure((size_t)N * 2 * sizeof(double)))
+    if (whichgrad != EIK_GRADSRCLOC) {
+        DevBuf &img = ctx->buf("grad_img"), &dtap = ctx->buf("taper");
+        if (img.ensure((size_t)P_c * sizeof(double)) || dtap.ensure((size_t)nsrc * sizeof(TaperDev)))
             return ctx->fail(EIK_ENOMEM, "out of device memory (gradient images)");
         std::vector<TaperDev> tap(nsrc);
         for (int64_t s = 0; s < nsrc; s++) taper_box(M->g, S.src.data() + s * nd, P->radiussmoothgradsrc, tap[s]);
@@ -1238,11 +1244,66 @@ int gradient_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, const eik_par
         else
             k_grad_img<float><<<gi, 256, 0, st>>>(dci, (const float*)ttc, (const double*)lam_c.p, M->d_vel,
                                                   (double*)img.p, N);
-        k_grad_quirk<<<1, 256, 0, st>>>((const SrcPt*)dpts.p, (const int*)dpoff.p, (int)nsrc, dci,
-                                        (const TaperDev*)dtap.p, nd, kind, (const double*)dt2.p, (const double*)dt1.p,
-                                        (double*)img.p);
-        k_taper<<<(unsigned)nsrc, 256, 0, st>>>(dci, (const TaperDev*)dtap.p, nd, kind, (double*)img.p);
-        k_sum_sources<<<nb(N, 256), 256, 0, st>>>((const double*)img.p, (int)nsrc, N, (double*)gout.p);
+        if (defer) {
+            out->deferred = true;
+            out->inst = dci;
+            out->pts = dpts.p;
+            out->poff = dpoff.p;
+            out->tap = dtap.p;
+            out->t2 = dt2.p;
+            out->t1 = dt1.p;
+        } else {
+            k_grad_quirk<<<1, 256, 0, st>>>((const SrcPt*)dpts.p, (const int*)dpoff.p, (int)nsrc, dci,
+                                            (const TaperDev*)dtap.p, nd, kind, (const double*)dt2.p, (const double*)dt1.p,
+                                            (double*)img.p, nullptr);
+            k_taper<<<(unsigned)nsrc, 256, 0, st>>>(dci, (const TaperDev*)dtap.p, nd, kind, (double*)img.p);
+        }
+        CK(hipGetLastError());
+        out->img = (double*)img.p;
+    }
+    CK(hipStreamSynchronize(st));
+    ctx->stats[3] = tm.mark();
+    return EIK_OK;
+}
+
+int gradient_total(eik_ctx* ctx, eik_model* M, const eik_params* P, int whichgrad, const GradPart* parts, int nparts,
+                   const int64_t* rec_off, int64_t nsrc, double* gradvel, double* misf) {
+    const int nd = M->g.ndim;
+    const int64_t N = M->N, nrec = rec_off[nsrc];
+    hipStream_t st = ctx->stream;
+    Timer tm(ctx);
+    // misfit over the receivers' terms of every part, source by source
+    DevBuf &dmt = ctx->buf("mterm_all"), &droff = ctx->buf("rec_off_all"), &dmis = ctx->buf("misf");
+    if (dmt.ensure((size_t)std::max<int64_t>(nrec, 1) * sizeof(double)) ||
+        droff.ensure((size_t)(nsrc + 1) * sizeof(int64_t)) || dmis.ensure(sizeof(double)))
+        return ctx->fail(EIK_ENOMEM, "out of device memory (misfit)");
+    int64_t off = 0;
+    for (int i = 0; i < nparts; i++) {
+        if (parts[i].nrec > 0)
+            CK(hipMemcpyAsync((double*)dmt.p + off, parts[i].mterm, (size_t)parts[i].nrec * sizeof(double),
+                              hipMemcpyDeviceToDevice, st));
+        off += parts[i].nrec;
+    }
+    CK(hipMemcpyAsync(droff.p, rec_off, (size_t)(nsrc + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    k_misfit<<<1, kMisfitThreads, 0, st>>>((const double*)dmt.p, (const int64_t*)droff.p, nsrc, (double*)dmis.p);
+    CK(hipGetLastError());
+    CK(hipMemcpyAsync(misf, dmis.p, sizeof(double), hipMemcpyDeviceToHost, st));
+    if (gradvel && whichgrad != EIK_GRADSRCLOC) {
+        DevBuf& gout = ctx->buf("grad_out");
+        if (gout.ensure((size_t)N * 2 * sizeof(double))) return ctx->fail(EIK_ENOMEM, "out of device memory (gradient)");
+        // a continuing part's source-node carry-over (from the previous part's last tapered image), then its taper
+        for (int i = 1; i < nparts; i++) {
+            const GradPart& q = parts[i];
+            if (!q.deferred || q.nsrc == 0) continue;
+            const InstDev* qi = (const InstDev*)q.inst;
+            k_grad_quirk<<<1, 256, 0, st>>>((const SrcPt*)q.pts, (const int*)q.poff, (int)q.nsrc, qi,
+                                            (const TaperDev*)q.tap, nd, M->g.kind, (const double*)q.t2,
+                                            (const double*)q.t1, q.img, parts[i - 1].img + (parts[i - 1].nsrc - 1) * N);
+            k_taper<<<(unsigned)q.nsrc, 256, 0, st>>>(qi, (const TaperDev*)q.tap, nd, M->g.kind, q.img);
+            CK(hipGetLastError());
+        }
+        k_sum_sources<<<nb(N, 256), 256, 0, st>>>(parts[0].img, (int)parts[0].nsrc, nparts > 1 ? parts[1].img : nullptr,
+                                                  nparts > 1 ? (int)parts[1].nsrc : 0, N, (double*)gout.p);
         CK(hipGetLastError());
         double* res = (double*)gout.p;
         if (P->smoothgradkern > 0) {
@@ -1279,8 +1340,16 @@ int gradient_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, const eik_par
         CK(hipMemcpyAsync(gradvel, res, (size_t)N * sizeof(double), hipMemcpyDeviceToHost, st));
     }
     CK(hipStreamSynchronize(st));
-    ctx->stats[3] = tm.mark();
+    ctx->stats[3] += tm.mark();
     return EIK_OK;
+}
+
+int gradient_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, const eik_params* P, int whichgrad,
+                   const double* pickobs, const double* stdobs, double* gradvel, double* gradsrc, double* misf) {
+    GradPart part{};
+    const int rc = gradient_part(ctx, M, S, P, whichgrad, pickobs, stdobs, gradsrc, false, &part);
+    if (rc) return rc;
+    return gradient_total(ctx, M, P, whichgrad, &part, 1, S.rec_off.data(), S.nsrc, gradvel, misf);
 }
 
 }  // namespace eik

@@ -195,6 +195,23 @@ const char* eik_version(void) { return "libeik 0.1 (gfx950)"; }
 
 const char* eik_last_error(const eik_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
+// Streams a large traveltime or gradient call's sources are split over by default (EIK_STREAMS = 1
+// keeps one; eik_ctx_set_streams per context).  Each half runs its own round loop on its own
+// stream, so while the last tiles of one half's round finish, the other half's round kernel takes
+// the CUs they leave idle (201x201x101 x 16 sources: +11 % FP64, +14 % FP32; 600x400 x 256: +28 %;
+// 120x150x150 spherical x 32: +20 %; four streams are slower than two).  Calls below kSplitNodes
+// source-nodes stay on one stream (300x220 x 4 sources: the second thread and stream cost 6 %).
+// Results do not depend on the split: every source is solved on its own, and a gradient's sums
+// and its source-to-source carry-over run over both halves in source order.
+constexpr int64_t kSplitNodes = (int64_t)1 << 22;
+static int split_streams() {
+    static const int v = [] {
+        const char* e = getenv("EIK_STREAMS");
+        return e ? std::min(2, std::max(1, atoi(e))) : 2;
+    }();
+    return v;
+}
+
 // stream, counters and timers of a context (a peer shares its owner's tbase)
 static bool ctx_init(eik_ctx* ctx, int device, bool with_tbase) {
     ctx->device = device;
@@ -205,7 +222,7 @@ static bool ctx_init(eik_ctx* ctx, int device, bool with_tbase) {
         return false;
     for (auto& e : ctx->kev)
         if (hipEventCreate(&e) != hipSuccess) return false;
-    return !with_tbase || hipEventCreate(&ctx->tbase) == hipSuccess;
+    return !with_tbase || (hipEventCreate(&ctx->tbase) == hipSuccess && hipEventCreate(&ctx->tend) == hipSuccess);
 }
 
 int eik_ctx_create(int32_t device, eik_ctx** out) {
@@ -219,6 +236,7 @@ int eik_ctx_create(int32_t device, eik_ctx** out) {
         eik_ctx_destroy(ctx);
         return EIK_EDEVICE;
     }
+    ctx->split = split_streams() == 2;
     // the no-allocation entry point's scratch block: this device's address, resolved once
     scratch_setup(device);
     *out = ctx;
@@ -238,6 +256,7 @@ void eik_ctx_destroy(eik_ctx* ctx) {
     for (auto& e : ctx->kev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->tbase && !ctx->owner) (void)hipEventDestroy(ctx->tbase);
+    if (ctx->tend) (void)hipEventDestroy(ctx->tend);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -330,22 +349,6 @@ void eik_model_destroy(eik_model* M) {
     delete M;
 }
 
-// Streams a traveltime call's sources are split over (EIK_STREAMS = 1 keeps one).  Each half runs
-// its own round loop on its own stream, so while the last tiles of one half's round finish, the
-// other half's round kernel takes the CUs they leave idle (201x201x101 x 16 sources: +11 % FP64,
-// +14 % FP32; 600x400 x 256: +28 %; 120x150x150 spherical x 32: +20 %; four streams are slower
-// than two).  Calls below kSplitNodes source-nodes stay on one stream (300x220 x 4 sources: the
-// second thread and stream cost 6 %).  Results do not depend on the split: every source is solved
-// on its own.
-constexpr int64_t kSplitNodes = (int64_t)1 << 22;
-static int split_streams() {
-    static const int v = [] {
-        const char* e = getenv("EIK_STREAMS");
-        return e ? std::min(2, std::max(1, atoi(e))) : 2;
-    }();
-    return v;
-}
-
 // the context's peer, created on first use
 static eik_ctx* get_peer(eik_ctx* ctx) {
     if (!ctx->peer) {
@@ -378,7 +381,7 @@ static SourceSet sub_sources(const SourceSet& S, int64_t a, int64_t b) {
 static void merge_peer_stats(eik_ctx* ctx) {
     const double* q = ctx->peer->stats;
     double* s = ctx->stats;
-    for (int i : {1, 2, 11}) s[i] = std::max(s[i], q[i]);
+    for (int i : {1, 2, 3, 11}) s[i] = std::max(s[i], q[i]);
     for (int i : {4, 5, 6, 7, 8, 9, 10, 12, 13, 14, 15, 16}) s[i] += q[i];
 }
 
@@ -395,14 +398,13 @@ int eik_traveltime(eik_ctx* ctx, eik_model* model, int64_t nsrc, const double* c
     if (S.nrec() > 0 && !ttpicks) return ctx->fail(EIK_EINVAL, "null picks buffer");
     if (nsrc == 0) return EIK_OK;
     stats_begin(ctx);
-    Timer total(ctx);
-    const bool split = ctx->split && nsrc >= 2 && nsrc * model->N >= kSplitNodes && split_streams() == 2;
+    const bool split = ctx->split && nsrc >= 2 && nsrc * model->N >= kSplitNodes;
     eik_ctx* peer = split ? get_peer(ctx) : nullptr;
     if (!peer) {
         rc = forward_solve(ctx, model, S, params->precision, params);
         if (rc) return rc;
         rc = forward_picks(ctx, model, S, params->precision, ttpicks, ttgrid);
-        ctx->stats[0] = total.mark();
+        ctx->stats[0] = call_ms(ctx);
         ctx->stats[17] = busy_ms(ctx, false, false);
         ctx->stats[18] = busy_ms(ctx, false, true);
         return rc;
@@ -426,7 +428,7 @@ int eik_traveltime(eik_ctx* ctx, eik_model* model, int64_t nsrc, const double* c
     if (rc) return rc;
     if (rc1) return ctx->fail(rc1, "%s", peer->err.c_str());
     merge_peer_stats(ctx);
-    ctx->stats[0] = total.mark();
+    ctx->stats[0] = call_ms(ctx);
     ctx->stats[17] = busy_ms(ctx, true, false);
     ctx->stats[18] = busy_ms(ctx, true, true);
     return EIK_OK;
@@ -456,12 +458,11 @@ int eik_traveltime_batch(eik_ctx* ctx, int32_t nreq, const eik_request* reqs) {
     if (G.empty()) return EIK_OK;
     const eik_params pe = effective_params(&reqs[0].params, reqs[0].model->g.kind);
     stats_begin(ctx);
-    Timer total(ctx);
     int rc = forward_solve(ctx, G, pe.precision, &pe);
     if (rc) return rc;
     std::vector<double> picks((size_t)std::max<int64_t>(nrec, 1));
     rc = forward_picks(ctx, G, pe.precision, picks.data(), nullptr);
-    ctx->stats[0] = total.mark();
+    ctx->stats[0] = call_ms(ctx);
     ctx->stats[17] = busy_ms(ctx, false, false);
     ctx->stats[18] = busy_ms(ctx, false, true);
     if (rc) return rc;
@@ -531,12 +532,46 @@ int eik_gradient(eik_ctx* ctx, eik_model* model, int64_t nsrc, const double* coo
         return EIK_OK;
     }
     stats_begin(ctx);
-    Timer total(ctx);
-    rc = gradient_solve(ctx, model, S, params, whichgrad, pickobs, stdobs, gradvel, gradsrc, misf);
-    ctx->stats[0] = total.mark();
-    ctx->stats[17] = busy_ms(ctx, false, false);
-    ctx->stats[18] = busy_ms(ctx, false, true);
+    const bool split = ctx->split && nsrc >= 2 && nsrc * model->N >= kSplitNodes;
+    eik_ctx* peer = split ? get_peer(ctx) : nullptr;
+    if (!peer) {
+        rc = gradient_solve(ctx, model, S, params, whichgrad, pickobs, stdobs, gradvel, gradsrc, misf);
+        ctx->stats[0] = call_ms(ctx);
+        ctx->stats[17] = busy_ms(ctx, false, false);
+        ctx->stats[18] = busy_ms(ctx, false, true);
+        return rc;
+    }
+    // the second half of the sources on the peer's stream; the sums over sources (misfit, gradvel)
+    // then run here over both halves in source order, as they do unsplit
+    const int64_t h = (nsrc + 1) / 2, r1 = S.rec_off[h];
+    const SourceSet S0 = sub_sources(S, 0, h), S1 = sub_sources(S, h, nsrc);
+    const int nd = model->g.ndim;
+    GradPart parts[2] = {};
+    stats_begin(peer);
+    CK(hipEventSynchronize(ctx->tbase));
+    int rc1 = EIK_OK;
+    std::thread th([&] {
+        (void)hipSetDevice(peer->device);
+        rc1 = gradient_part(peer, model, S1, params, whichgrad, pickobs ? pickobs + r1 : nullptr,
+                            stdobs ? stdobs + r1 : nullptr, gradsrc ? gradsrc + h * nd : nullptr, true, &parts[1]);
+    });
+    rc = gradient_part(ctx, model, S0, params, whichgrad, pickobs, stdobs, gradsrc, false, &parts[0]);
+    th.join();
+    if (rc) return rc;
+    if (rc1) return ctx->fail(rc1, "%s", peer->err.c_str());
+    merge_peer_stats(ctx);
+    rc = gradient_total(ctx, model, params, whichgrad, parts, 2, S.rec_off.data(), nsrc, gradvel, misf);
+    ctx->stats[0] = call_ms(ctx);
+    ctx->stats[17] = busy_ms(ctx, true, false);
+    ctx->stats[18] = busy_ms(ctx, true, true);
     return rc;
+}
+
+int eik_ctx_set_streams(eik_ctx* ctx, int32_t nstreams) {
+    if (!ctx || nstreams < 1 || nstreams > 2) return EIK_EINVAL;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->split = nstreams == 2;
+    return EIK_OK;
 }
 
 int eik_mem_info(eik_ctx* ctx, int64_t* free_bytes, int64_t* total_bytes) {

@@ -373,7 +373,7 @@ struct eik_ctx {
     // Round-launch spans of the current call (ms after tbase, an event recorded when the call
     // starts): with the sources split over this context and its peer, the launches of the two
     // streams overlap, and the union of the spans is the time the round kernel held the GPU.
-    hipEvent_t tbase = nullptr;
+    hipEvent_t tbase = nullptr, tend = nullptr;
     std::vector<KSpan> kspans;
     bool span_fine = false;  // the launches being timed belong to the near-source solve
     // a second context on its own stream that solves part of a call's sources concurrently
@@ -410,11 +410,20 @@ inline int64_t ctx_pool_bytes(const eik_ctx* ctx) {
     for (const auto& kv : ctx->bufs) b += (int64_t)kv.second.cap;
     return b + (ctx->peer ? ctx_pool_bytes(ctx->peer) : 0);
 }
-// a call's statistics start from zero and its round-launch spans are timed from now
+// a call's statistics start from zero and its round-launch spans are timed from now (a peer's
+// from its owner's base event, which the owner records)
 inline void stats_begin(eik_ctx* ctx) {
     for (double& x : ctx->stats) x = 0.0;
     ctx->kspans.clear();
-    if (ctx->tbase) (void)hipEventRecord(ctx->tbase, ctx->stream);
+    if (ctx->tbase && !ctx->owner) (void)hipEventRecord(ctx->tbase, ctx->stream);
+}
+// device time of the call so far (from its tbase, on the context stream; waits for the stream)
+inline double call_ms(eik_ctx* ctx) {
+    float ms = 0.f;
+    if (!ctx->tbase || !ctx->tend || hipEventRecord(ctx->tend, ctx->stream) != hipSuccess ||
+        hipEventSynchronize(ctx->tend) != hipSuccess || hipEventElapsedTime(&ms, ctx->tbase, ctx->tend) != hipSuccess)
+        return 0.0;
+    return (double)ms;
 }
 // length of the union of the round-launch spans of a call on this context and its peer (ms):
 // the coarse solve's launches, and the near-source solve's too if `fine`
@@ -475,7 +484,8 @@ struct SourceSet {
     int64_t nrec() const { return rec_off.empty() ? 0 : rec_off.back(); }
 };
 
-// elapsed device time between marks on the context stream
+// elapsed device time between marks on the context stream (one Timer at a time per context: they
+// share its two events; a call's total is call_ms)
 struct Timer {
     eik_ctx* c;
     explicit Timer(eik_ctx* ctx) : c(ctx) {
@@ -513,6 +523,23 @@ inline int forward_picks(eik_ctx* ctx, eik_model* M, const SourceSet& S, int pre
     return forward_picks(ctx, std::vector<FwdGroup>{{M, &S}}, prec, picks, ttgrid);
 }
 int model_slowness(eik_model* M);
+// A gradient call in two parts: the per-source part (forward, adjoint, source terms and the
+// per-source gradient images, misfit terms per receiver), then the sums over sources in source
+// order (misfit, gradvel, smoothing), over the parts of one call (split over two contexts) in turn.
+struct GradPart {
+    double* img;           // nsrc gradient images (N doubles each), device
+    const double* mterm;   // nrec misfit terms, device
+    int64_t nsrc, nrec;
+    // a part that continues another one leaves the source-node carry-over and the taper of its
+    // images to gradient_total (they chain from the previous part's last source): its device
+    // instances, source points, their offsets, tapers and source terms
+    bool deferred;
+    const void *inst, *pts, *poff, *tap, *t2, *t1;
+};
+int gradient_part(eik_ctx* ctx, eik_model* M, const SourceSet& S, const eik_params* P, int whichgrad,
+                  const double* pickobs, const double* stdobs, double* gradsrc, bool defer, GradPart* out);
+int gradient_total(eik_ctx* ctx, eik_model* M, const eik_params* P, int whichgrad, const GradPart* parts, int nparts,
+                   const int64_t* rec_off, int64_t nsrc, double* gradvel, double* misf);
 int gradient_solve(eik_ctx* ctx, eik_model* M, const SourceSet& S, const eik_params* P, int whichgrad,
                    const double* pickobs, const double* stdobs, double* gradvel, double* gradsrc, double* misf);
 // fill kernel (eik_fim.hip), also used by the adjoint rounds

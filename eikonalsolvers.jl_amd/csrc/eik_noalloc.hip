@@ -129,11 +129,10 @@ extern "C" int eik_traveltime_noalloc(eik_ctx* ctx, eik_model* model, int64_t ns
     ctx->abufs.clear();
     ctx->arena = &A;
     stats_begin(ctx);
-    Timer total(ctx);
     rc = forward_solve(ctx, model, S, pe.precision, &pe);
     if (rc == EIK_OK) rc = forward_picks(ctx, model, S, pe.precision, ttpicks, ttgrid);
     if (rc == EIK_OK) {
-        ctx->stats[0] = total.mark();
+        ctx->stats[0] = call_ms(ctx);
         ctx->stats[17] = busy_ms(ctx, false, false);
         ctx->stats[18] = busy_ms(ctx, false, true);
     }

@@ -30,7 +30,7 @@ EXPORTS = (
     "eik_server_create", "eik_server_destroy", "eik_submit", "eik_wait", "eik_server_stats",
     "eik_tracerays", "eik_traveltime_batch", "eik_poll", "eik_tracerays_max_steps", "eik_wait_stats",
     "eik_request_bytes",
-    "eik_traveltime_noalloc", "eik_mem_info",
+    "eik_traveltime_noalloc", "eik_mem_info", "eik_ctx_set_streams",
 )
 
 
@@ -97,6 +97,8 @@ def load():
         L.eik_traveltime_noalloc.restype = ctypes.c_int
         L.eik_mem_info.argtypes = [vp, ip, ip]
         L.eik_mem_info.restype = ctypes.c_int
+        L.eik_ctx_set_streams.argtypes = [vp, ctypes.c_int32]
+        L.eik_ctx_set_streams.restype = ctypes.c_int
         L.eik_gradient.argtypes = [vp, vp, ctypes.c_int64, dp, ip, dp, dp, dp, ctypes.c_int32, P(EikParams),
                                    dp, dp, dp]
         L.eik_gradient.restype = ctypes.c_int

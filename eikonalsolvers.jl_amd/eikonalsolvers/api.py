@@ -36,6 +36,10 @@ class Context:
         self.handle = h
         self.device = device
 
+    def set_streams(self, nstreams: int):
+        """Streams large calls may split their sources over (2, the default, or 1)."""
+        _lib.check(_lib.load().eik_ctx_set_streams(self.handle, int(nstreams)), self.handle)
+
     def close(self):
         if self.handle:
             _lib.load().eik_ctx_destroy(self.handle)

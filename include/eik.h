@@ -131,6 +131,12 @@ int eik_traveltime_noalloc(eik_ctx* ctx, eik_model* model, int64_t nsrc, const d
  * for callers that size their calls; no reference counterpart. */
 int eik_mem_info(eik_ctx* ctx, int64_t* free_bytes, int64_t* total_bytes);
 
+/* Streams the context's large eik_traveltime and eik_gradient calls may split their sources over:
+ * 2 (the default; EIK_STREAMS=1 in the environment makes it 1) or 1.  A split call solves the
+ * second half of its sources on a second stream, concurrently, with the same results; callers
+ * that keep the GPU busy from threads of their own may prefer 1.  No reference counterpart. */
+int eik_ctx_set_streams(eik_ctx* ctx, int32_t nstreams);
+
 /* Timing of the last call's device work (ms), split by phase; no reference counterpart.
  * out[0] = total, [1] = near-source (fine grid) solve, [2] = coarse solve, [3] = adjoint,
  * [4] = tile activations of the coarse solve (node-visits = activations x tile nodes),
