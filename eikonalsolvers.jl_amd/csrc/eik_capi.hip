@@ -349,6 +349,13 @@ void eik_model_destroy(eik_model* M) {
     delete M;
 }
 
+// marks a context and its peer as running one split call until the scope ends
+struct SplitScope {
+    eik_ctx* c;
+    explicit SplitScope(eik_ctx* ctx) : c(ctx) { c->in_split = c->peer->in_split = true; }
+    ~SplitScope() { c->in_split = c->peer->in_split = false; }
+};
+
 // the context's peer, created on first use
 static eik_ctx* get_peer(eik_ctx* ctx) {
     if (!ctx->peer) {
@@ -415,6 +422,7 @@ int eik_traveltime(eik_ctx* ctx, eik_model* model, int64_t nsrc, const double* c
     double* picks1 = ttpicks ? ttpicks + S.rec_off[h] : nullptr;
     double* grid1 = ttgrid ? ttgrid + h * model->N : nullptr;
     stats_begin(peer);
+    SplitScope scope(ctx);
     CK(hipEventSynchronize(ctx->tbase));  // the peer's launch spans are timed from it
     int rc1 = EIK_OK;
     std::thread th([&] {
@@ -548,6 +556,7 @@ int eik_gradient(eik_ctx* ctx, eik_model* model, int64_t nsrc, const double* coo
     const int nd = model->g.ndim;
     GradPart parts[2] = {};
     stats_begin(peer);
+    SplitScope scope(ctx);
     CK(hipEventSynchronize(ctx->tbase));
     int rc1 = EIK_OK;
     std::thread th([&] {

@@ -8,6 +8,8 @@
 //           FP64 traveltime + FP64 slowness grid per refined near-source box.
 // Init (source) nodes are stored with the sign bit set (value -t); they count as accepted from
 // the start and never change.  Unreached nodes hold +inf.
+#include <array>
+
 #include "eik_internal.h"
 
 namespace eik {
@@ -1042,26 +1044,29 @@ static int fim_run_t(eik_ctx* ctx, const InstDev* d_inst, int ninst, int64_t nti
     unsigned* h_cnt = (unsigned*)ctx->h_small;
     unsigned long long* d_c64 = (unsigned long long*)(d_cnt + kCnt64);
     hipStream_t st = ctx->stream;
-    // resident blocks of this instantiation on the device (EIK_GRID overrides); the static is
+    // resident blocks of this instantiation on the device (EIK_GRID overrides); the statics are
     // initialised once, thread-safely, by the first caller.  More concurrent tiles are not
     // faster: neighbouring tiles relaxed at the same time see each other's old values, and the
     // runs each block claims get shorter.  Measured optimum per CU: 8 blocks in 3D (201x201x101
     // and 120x150x150 grids: 6 blocks 5 % slower on the first, 2 % faster on the second), 4-6 in
     // 2D (600x400 x 256 sources: 8 blocks 15 % slower in FP64; FP32 occupancy allows 12, 35 %
-    // slower than 6).
-    static const int grid = [ctx] {
+    // slower than 6).  A call split over two streams runs two launches at once: on 3D Cartesian
+    // grids 7 blocks per CU per launch then win in FP64 and mixed precision and 6 in FP32
+    // (201x201x101 x 16: +2.5 % and +4.6 % against 8); on the sphere 8 stay best (120x150x150 x 32:
+    // 6 blocks 7 % slower).
+    static const std::array<int, 3> dev = [ctx] {
         int ncu = 0, occ = 0;
         if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess) ncu = 1;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_fim_round<ND, T, SPH, MIX, NEAR>, kTileThreads, 0) != hipSuccess)
             occ = 1;
-        constexpr int kMaxPerCu = ND == 3 ? 8 : 5;
-        int gr = std::max(1, ncu * std::min(kMaxPerCu, std::max(1, occ)));
-        if (const char* e = getenv("EIK_GRID")) gr = std::max(1, atoi(e));
-        if (getenv("EIK_VERBOSE"))
-            fprintf(stderr, "libeik: k_fim_round<%d,%s,%d,%d> %d CUs x %d blocks -> grid %d\n", ND,
-                    sizeof(T) == 8 ? "double" : "float", (int)SPH, (int)MIX, ncu, occ, gr);
-        return gr;
+        const char* e = getenv("EIK_GRID");
+        return std::array<int, 3>{std::max(1, ncu), std::max(1, occ), e ? std::max(1, atoi(e)) : 0};
     }();
+    const int per_cu = ND == 2 ? 5 : (SPH || !ctx->in_split) ? 8 : (sizeof(T) == 4 && !MIX) ? 6 : 7;
+    const int grid = dev[2] > 0 ? dev[2] : dev[0] * std::min(per_cu, dev[1]);
+    if (getenv("EIK_VERBOSE"))
+        fprintf(stderr, "libeik: k_fim_round<%d,%s,%d,%d> %d CUs x %d blocks (occupancy %d) -> grid %d\n", ND,
+                sizeof(T) == 8 ? "double" : "float", (int)SPH, (int)MIX, dev[0], per_cu, dev[1], grid);
     const int g = (int)std::min<int64_t>(grid, ntiles);
     FimStats stt{};
     CK(hipMemsetAsync(d_cnt, 0, kSmallBytes, st));

@@ -381,6 +381,7 @@ struct eik_ctx {
     eik_ctx* peer = nullptr;
     eik_ctx* owner = nullptr;  // set on a peer: the context it belongs to (whose tbase it shares)
     bool split = true;         // large calls may split their sources with the peer
+    bool in_split = false;     // the current call is split (set on the context and its peer)
     int xcap = 1;  // capacity per source of the transferred-point list
     std::mutex mu;
     // arena mode (the no-allocation entry point): buffers are carved from this block instead
