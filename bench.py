@@ -129,17 +129,19 @@ def _roofline(prec: str, f: dict) -> dict:
     out = {
         "bound": "hbm",
         "kernel": KERNELS[prec],
-        "achieved": round(f["achieved_gbs"], 2) if f["achieved_gbs"] else None,
+        "achieved": round(f["per_launch_gbs"], 2) if f["per_launch_gbs"] else None,
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
-        "frac": round(f["achieved_gbs"] / HBM_PEAK_GBS, 5) if f["achieved_gbs"] else None,
+        "frac": round(f["per_launch_gbs"] / HBM_PEAK_GBS, 5) if f["per_launch_gbs"] else None,
         "traffic": None,
         "traffic_scope": "HBM bytes per launch of this kernel (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a plain "
                          "copy of known size)",
-        "achieved_scope": "algorithmic bytes of the step's launches / time in which at least one of them ran (the "
-                          "sources are split over two streams whose launches overlap); achieved_per_launch = "
-                          "algorithmic bytes per launch / average launch duration",
-        "achieved_per_launch": round(f["per_launch_gbs"], 2) if f["per_launch_gbs"] else None,
+        "achieved_scope": "algorithmic bytes per launch / average launch duration (HIP events on the stream each "
+                          "launch runs on); the call's sources are split over two streams whose launches overlap, "
+                          "so achieved_device = algorithmic bytes of the step's launches / time in which at least "
+                          "one of them ran",
+        "achieved_device": round(f["achieved_gbs"], 2) if f["achieved_gbs"] else None,
+        "frac_device": round(f["achieved_gbs"] / HBM_PEAK_GBS, 5) if f["achieved_gbs"] else None,
         "kernel_busy_ms_per_step": round(f["kernel_busy_ms_per_step"], 3),
         "bytes_per_node_visit": BYTES_PER_VISIT[prec],
         "algorithmic_bytes_per_launch": round(alg_per_launch),

@@ -1210,9 +1210,11 @@ static double band_mult(int nd, int prec, bool sph) {
     if (env == env) return env;
     if (nd == 2) return 16.0;  // mixed on 600x400 x 256 sources: 3 % faster than 12, like FP64
     if (prec == EIK_PREC_FP64) return 6.0;
-    // mixed and FP32 on 201x201x101 x 16 sources: 8 crossings 2-3 % faster than 6, 10 another 1-2 %;
-    // on the 120x150x150 spherical grid 8 is 2 % slower than 6
-    return sph ? 6.0 : 8.0;
+    // mixed and FP32 on 201x201x101 x 16 sources: 8 crossings 2-3 % faster than 6; FP32 gains
+    // another 2 % at 10, where mixed loses 1.5 %; on the 120x150x150 spherical grid 8 is 2 % slower
+    // than 6
+    if (sph) return 6.0;
+    return prec == EIK_PREC_FP32 ? 10.0 : 8.0;
 }
 
 // EIK_NODEBAND: the same for the nodes inside a tile (default 0: a listed tile relaxes all of
