@@ -635,11 +635,20 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(kTileThreads, kTileTh
     }
 }
 
-// Band scheduling over the active tiles, one tile per thread.  A block of kScanThreads
-// consecutive tiles almost always lies in one instance: its minimum and flagged count are then
-// reduced in LDS and published with one atomic each (atomics on a few hot addresses from every
-// wave serialise); blocks that straddle instances reduce per wave.
-constexpr int kScanThreads = 1024;
+// Band scheduling over the active tiles, one tile per thread.  A block of BS consecutive tiles
+// almost always lies in one instance: its minimum and flagged count are then reduced in LDS and
+// published with one atomic each (atomics on a few hot addresses from every wave serialise);
+// blocks that straddle instances reduce per wave.
+// Block size: while the other stream's round kernel holds the CUs (a split call), a scan block
+// starts only where all its waves fit beside it, and the FP64 and mixed round kernels leave about
+// one wave's registers per SIMD free: 256 threads (one wave per SIMD) then beat 1024 in 3D
+// (201x201x101 x 16 sources: FP64 +2.4 %, mixed +4.4 %, FP64 gradient +2 %; 120x150x150 sphere x 32:
+// FP64 +6 %); FP32 keeps 1024 (256 is 1.3 % slower there), and so does 2D (600x400 x 256: 256 is
+// 5 % slower); 128 is slower everywhere.
+template <int ND, typename T, bool MIX>
+constexpr int scan_threads() {
+    return (ND == 2 || (sizeof(T) == 4 && !MIX)) ? 1024 : 256;
+}
 // Instances whose first tile ids a scan block stages in LDS (one coalesced read instead of a
 // binary search through global memory, one dependent load per step); beyond it the search reads
 // the instance table itself.
@@ -676,13 +685,14 @@ __device__ __forceinline__ InstFinder stage_tile_bases(const InstDev* __restrict
 }
 
 // The lowest pending value per instance (into wmin, unless null) and the number of flagged tiles.
-__global__ void __launch_bounds__(kScanThreads) k_band_min(const unsigned* __restrict__ flags,
+template <int BS>
+__global__ void __launch_bounds__(BS) k_band_min(const unsigned* __restrict__ flags,
                                                          const unsigned* __restrict__ pend, int64_t ntiles,
                                                          const InstDev* __restrict__ inst, int ninst,
                                                          unsigned* __restrict__ wmin, unsigned* __restrict__ nflagged) {
-    __shared__ unsigned s_min[kScanThreads / 64], s_cnt[kScanThreads / 64];
+    __shared__ unsigned s_min[BS / 64], s_cnt[BS / 64];
     __shared__ int64_t s_tb[kScanLdsInst];
-    const int64_t b0 = (int64_t)blockIdx.x * kScanThreads;
+    const int64_t b0 = (int64_t)blockIdx.x * BS;
     const int64_t t = b0 + threadIdx.x;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     // the tile's words are read before the instance table is searched (no dependence)
@@ -690,7 +700,7 @@ __global__ void __launch_bounds__(kScanThreads) k_band_min(const unsigned* __res
     const unsigned pd = t < ntiles ? pend[t] : 0x7f800000u;
     const InstFinder find = stage_tile_bases(inst, ninst, s_tb);
     const int ilo = find(b0);
-    const int ihi = find((b0 + kScanThreads < ntiles ? b0 + kScanThreads : ntiles) - 1);
+    const int ihi = find((b0 + BS < ntiles ? b0 + BS : ntiles) - 1);
     const bool on = fl != 0u;
     unsigned long long m = __ballot(on);
     const unsigned v = on ? pd : 0x7f800000u;
@@ -714,7 +724,7 @@ __global__ void __launch_bounds__(kScanThreads) k_band_min(const unsigned* __res
     __syncthreads();
     if (threadIdx.x == 0) {
         unsigned c = 0u, mn = 0x7f800000u;
-        for (int w = 0; w < kScanThreads / 64; w++) {
+        for (int w = 0; w < BS / 64; w++) {
             c += s_cnt[w];
             mn = min(mn, s_min[w]);
         }
@@ -728,15 +738,16 @@ __global__ void __launch_bounds__(kScanThreads) k_band_min(const unsigned* __res
 // them; the others wait for a later round, and their lowest pending value per instance goes to
 // the next round's minimum (wnext; the round kernel adds the tiles it activates).  One list
 // reservation per block.
-__global__ void __launch_bounds__(kScanThreads) k_band_select(unsigned* __restrict__ flags, unsigned* __restrict__ pend,
+template <int BS>
+__global__ void __launch_bounds__(BS) k_band_select(unsigned* __restrict__ flags, unsigned* __restrict__ pend,
                                                             int64_t ntiles, const InstDev* __restrict__ inst,
                                                             int ninst, const unsigned* __restrict__ wmin,
                                                             unsigned* __restrict__ wnext, uint4* __restrict__ list,
                                                             unsigned* __restrict__ slot) {
-    __shared__ unsigned s_wc[kScanThreads / 64], s_left[kScanThreads / 64];
+    __shared__ unsigned s_wc[BS / 64], s_left[BS / 64];
     __shared__ unsigned s_base;
     __shared__ int64_t s_tb[kScanLdsInst];
-    const int64_t b0 = (int64_t)blockIdx.x * kScanThreads;
+    const int64_t b0 = (int64_t)blockIdx.x * BS;
     const int64_t t = b0 + threadIdx.x;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     // the tile's words are read before the instance table is searched (no dependence)
@@ -744,7 +755,7 @@ __global__ void __launch_bounds__(kScanThreads) k_band_select(unsigned* __restri
     const unsigned pd = t < ntiles ? pend[t] : 0x7f800000u;
     const InstFinder find = stage_tile_bases(inst, ninst, s_tb);
     const int ilo = find(b0);
-    const int ihi = find((b0 + kScanThreads < ntiles ? b0 + kScanThreads : ntiles) - 1);
+    const int ihi = find((b0 + BS < ntiles ? b0 + BS : ntiles) - 1);
     const float lim0 = __uint_as_float(wmin[ilo]) + (float)inst[ilo].band;
     bool sel = false;
     unsigned pv = 0x7f800000u;
@@ -780,7 +791,7 @@ __global__ void __launch_bounds__(kScanThreads) k_band_select(unsigned* __restri
     __syncthreads();
     if (threadIdx.x == 0) {
         unsigned tot = 0u, mn = 0x7f800000u;
-        for (int w = 0; w < kScanThreads / 64; w++) {
+        for (int w = 0; w < BS / 64; w++) {
             const unsigned c = s_wc[w];
             s_wc[w] = tot;
             tot += c;
@@ -1068,6 +1079,7 @@ static int fim_run_t(eik_ctx* ctx, const InstDev* d_inst, int ninst, int64_t nti
         fprintf(stderr, "libeik: k_fim_round<%d,%s,%d,%d> %d CUs x %d blocks (occupancy %d) -> grid %d\n", ND,
                 sizeof(T) == 8 ? "double" : "float", (int)SPH, (int)MIX, dev[0], per_cu, dev[1], grid);
     const int g = (int)std::min<int64_t>(grid, ntiles);
+    constexpr int kSB = scan_threads<ND, T, MIX>();
     FimStats stt{};
     CK(hipMemsetAsync(d_cnt, 0, kSmallBytes, st));
     // Settling starts after kSettleRound rounds (converging solves of the benchmark, test and
@@ -1094,7 +1106,7 @@ static int fim_run_t(eik_ctx* ctx, const InstDev* d_inst, int ninst, int64_t nti
     unsigned* SL = d_cnt + kCntSlots;   // 3 round slots (list length, cursor), by round mod 3
     // round 0's lowest pending values from the seeded tiles; the other two buffers start empty
     k_fill_u32<<<nblocks(3 * (int64_t)ninst, 256), 256, 0, st>>>(W, 3 * (int64_t)ninst, 0x7f800000u);
-    k_band_min<<<nblocks(ntiles, kScanThreads), kScanThreads, 0, st>>>(flags, pend, ntiles, d_inst, ninst, W,
+    k_band_min<kSB><<<nblocks(ntiles, kSB), kSB, 0, st>>>(flags, pend, ntiles, d_inst, ninst, W,
                                                                       d_cnt + kCntFlagged);
     CK(hipGetLastError());
     int64_t round = 0;
@@ -1109,7 +1121,7 @@ static int fim_run_t(eik_ctx* ctx, const InstDev* d_inst, int ninst, int64_t nti
         }
         for (int r = 0; r < kRoundBatch; r++, round++) {
             const int c = (int)(round % 3), nx = (int)((round + 1) % 3), rs = (int)((round + 2) % 3);
-            k_band_select<<<nblocks(ntiles, kScanThreads), kScanThreads, 0, st>>>(
+            k_band_select<kSB><<<nblocks(ntiles, kSB), kSB, 0, st>>>(
                 flags, pend, ntiles, d_inst, ninst, W + (size_t)c * ninst, W + (size_t)nx * ninst, (uint4*)list.p,
                 SL + c * kSlotWords);
             CK(hipEventRecord(ctx->kev[2 * r], st));
@@ -1122,7 +1134,7 @@ static int fim_run_t(eik_ctx* ctx, const InstDev* d_inst, int ninst, int64_t nti
         }
         // the tiles still flagged (the next round's lowest pending values are already gathered)
         CK(hipMemsetAsync(d_cnt + kCntFlagged, 0, sizeof(unsigned), st));
-        k_band_min<<<nblocks(ntiles, kScanThreads), kScanThreads, 0, st>>>(flags, pend, ntiles, d_inst, ninst, nullptr,
+        k_band_min<kSB><<<nblocks(ntiles, kSB), kSB, 0, st>>>(flags, pend, ntiles, d_inst, ninst, nullptr,
                                                                           d_cnt + kCntFlagged);
         CK(hipMemcpyAsync(h_cnt, d_cnt, kSmallBytes, hipMemcpyDeviceToHost, st));
         CK(hipStreamSynchronize(st));
