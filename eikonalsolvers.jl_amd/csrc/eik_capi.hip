@@ -2,6 +2,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <functional>
+#include <system_error>
 #include <thread>
 
 #include "eik_internal.h"
@@ -356,6 +358,22 @@ struct SplitScope {
     ~SplitScope() { c->in_split = c->peer->in_split = false; }
 };
 
+// The second half of a split call runs on a thread of its own; if no thread can be started, it
+// runs on the calling thread after the first half (still on the peer's stream).
+static std::thread start_thread(const std::function<void()>& f) {
+    try {
+        return std::thread(f);
+    } catch (const std::system_error&) {
+        return std::thread();
+    }
+}
+static void finish_thread(std::thread& th, const std::function<void()>& f) {
+    if (th.joinable())
+        th.join();
+    else
+        f();
+}
+
 // the context's peer, created on first use
 static eik_ctx* get_peer(eik_ctx* ctx) {
     if (!ctx->peer) {
@@ -384,7 +402,8 @@ static SourceSet sub_sources(const SourceSet& S, int64_t a, int64_t b) {
 }
 
 // a peer's statistics folded into its owner's: counts and launch durations add, phase times and
-// the settling stage take the larger, [17] is the union of both streams' round launches
+// the settling stage take the larger ([17] and [18], unions over both streams' launch spans, are
+// the caller's)
 static void merge_peer_stats(eik_ctx* ctx) {
     const double* q = ctx->peer->stats;
     double* s = ctx->stats;
@@ -425,14 +444,15 @@ int eik_traveltime(eik_ctx* ctx, eik_model* model, int64_t nsrc, const double* c
     SplitScope scope(ctx);
     CK(hipEventSynchronize(ctx->tbase));  // the peer's launch spans are timed from it
     int rc1 = EIK_OK;
-    std::thread th([&] {
+    const auto second = [&] {
         (void)hipSetDevice(peer->device);
         rc1 = forward_solve(peer, model, S1, params->precision, params);
         if (rc1 == EIK_OK) rc1 = forward_picks(peer, model, S1, params->precision, picks1, grid1);
-    });
+    };
+    std::thread th = start_thread(second);
     rc = forward_solve(ctx, model, S0, params->precision, params);
     if (rc == EIK_OK) rc = forward_picks(ctx, model, S0, params->precision, ttpicks, ttgrid);
-    th.join();
+    finish_thread(th, second);
     if (rc) return rc;
     if (rc1) return ctx->fail(rc1, "%s", peer->err.c_str());
     merge_peer_stats(ctx);
@@ -559,13 +579,14 @@ int eik_gradient(eik_ctx* ctx, eik_model* model, int64_t nsrc, const double* coo
     SplitScope scope(ctx);
     CK(hipEventSynchronize(ctx->tbase));
     int rc1 = EIK_OK;
-    std::thread th([&] {
+    const auto second = [&] {
         (void)hipSetDevice(peer->device);
         rc1 = gradient_part(peer, model, S1, params, whichgrad, pickobs ? pickobs + r1 : nullptr,
                             stdobs ? stdobs + r1 : nullptr, gradsrc ? gradsrc + h * nd : nullptr, true, &parts[1]);
-    });
+    };
+    std::thread th = start_thread(second);
     rc = gradient_part(ctx, model, S0, params, whichgrad, pickobs, stdobs, gradsrc, false, &parts[0]);
-    th.join();
+    finish_thread(th, second);
     if (rc) return rc;
     if (rc1) return ctx->fail(rc1, "%s", peer->err.c_str());
     merge_peer_stats(ctx);

@@ -155,8 +155,8 @@ int eik_ctx_set_streams(eik_ctx* ctx, int32_t nstreams);
  * same over the launches of both solves (near-source and coarse).
  * eik_traveltime splits two or more sources (at least 2^22 nodes over all of them) into two halves
  * on two streams (the context's own and a second one it creates on first use; not on a server's
- * slots): the counts above are then summed over both halves, [1], [2] and [11] are the larger of
- * the two.  Values a call did not produce are 0. */
+ * slots): the counts above are then summed over both halves, [1], [2], [3] and [11] are the larger
+ * of the two.  Values a call did not produce are 0. */
 int eik_last_stats(const eik_ctx* ctx, double* out, int32_t nout);
 
 /* tracerays (src/EikSolv/eikutils.jl:201-328): for every source's receivers, the ray descending
