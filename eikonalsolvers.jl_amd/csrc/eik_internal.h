@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../../include/eik.h"
+#include "eik_spans.h"
 #include "eik_stencil.h"
 
 // ------------------------------------------------------------------ device-side descriptors
@@ -434,19 +435,7 @@ inline double busy_ms(const eik_ctx* ctx, bool with_peer, bool fine) {
         if (c)
             for (const KSpan& k : c->kspans)
                 if (fine || !k.fine) v.emplace_back(k.a, k.b);
-    std::sort(v.begin(), v.end());
-    double tot = 0.0, lo = 0.0, hi = -1.0;
-    for (const auto& x : v) {
-        if (x.first > hi) {
-            if (hi > lo) tot += hi - lo;
-            lo = x.first;
-            hi = x.second;
-        } else if (x.second > hi) {
-            hi = x.second;
-        }
-    }
-    if (hi > lo) tot += hi - lo;
-    return tot;
+    return span_union(std::move(v));
 }
 // release a context's reusable pools once its stream is idle (the next call allocates again)
 inline void ctx_trim(eik_ctx* ctx) {
