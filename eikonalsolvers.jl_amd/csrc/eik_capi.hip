@@ -202,10 +202,13 @@ const char* eik_last_error(const eik_ctx* ctx) { return ctx ? ctx->err.c_str() :
 // stream, so while the last tiles of one half's round finish, the other half's round kernel takes
 // the CUs they leave idle (201x201x101 x 16 sources: +11 % FP64, +14 % FP32; 600x400 x 256: +28 %;
 // 120x150x150 spherical x 32: +20 %; four streams are slower than two).  Calls below kSplitNodes
-// source-nodes stay on one stream (300x220 x 4 sources: the second thread and stream cost 6 %).
-// Results do not depend on the split: every source is solved on its own, and a gradient's sums
-// and its source-to-source carry-over run over both halves in source order.
-constexpr int64_t kSplitNodes = (int64_t)1 << 22;
+// source-nodes stay on one stream: there the halves' rounds are too small to fill the device and
+// the split costs time (1-2 x 2^22 source-nodes: 600x400 x 18 and x 32 sources 20-23 % slower,
+// 161x161x81 x 4 10 %, 101x101x51 x 9 14 %; 300x220 x 4: 6 %), from about 3.7 x 2^22 it gains
+// (600x400 x 64 +4 %, 161x161x81 x 8 +4 %, the larger workloads above).  Results do not depend on
+// the split: every source is solved on its own, and a gradient's sums and its source-to-source
+// carry-over run over both halves in source order.
+constexpr int64_t kSplitNodes = (int64_t)3 << 22;
 static int split_streams() {
     static const int v = [] {
         const char* e = getenv("EIK_STREAMS");

@@ -153,7 +153,7 @@ int eik_ctx_set_streams(eik_ctx* ctx, int32_t nstreams);
  * [17] = time in which at least one round launch of the coarse solve ran (ms, the union of the
  * launches' spans over both streams of a split call; [6] sums overlapping launches), [18] = the
  * same over the launches of both solves (near-source and coarse).
- * eik_traveltime splits two or more sources (at least 2^22 nodes over all of them) into two halves
+ * eik_traveltime splits two or more sources (at least 3 x 2^22 nodes over all of them) into two halves
  * on two streams (the context's own and a second one it creates on first use; not on a server's
  * slots): the counts above are then summed over both halves, [1], [2], [3] and [11] are the larger
  * of the two.  Values a call did not produce are 0. */

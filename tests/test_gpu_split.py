@@ -1,9 +1,9 @@
-"""Traveltime and gradient calls of 2^22 source-nodes or more split their sources into two halves
+"""Traveltime and gradient calls of 3 x 2^22 source-nodes or more split their sources into two halves
 solved concurrently on two streams (the context's own and a second one it creates); results must
 not depend on that.  FP64, 3D 101x101x51 (h = 1, velocity 3 + 0.02 z plus a Gaussian random
-field), 9 sources (5 + 4 on the two streams), 16 top-face receivers each, bit for bit:
+field), 25 sources (13 + 12 on the two streams), 16 top-face receivers each, bit for bit:
 
-* picks and traveltime grids against nine single-source calls (never split);
+* picks and traveltime grids against single-source calls (never split);
 * the gradient (misfit, velocity and source-location gradients) against the same call on a
   context held to one stream (eik_ctx_set_streams): a gradient is not a per-source sum of
   single-source gradients, since a source node's value carries over from the previous source
@@ -17,7 +17,7 @@ from eikonalsolvers.synthetic import gaussian_random_field, normal, uniform
 
 pytestmark = pytest.mark.gpu
 
-NSRC = 9
+NSRC = 25
 
 
 @pytest.fixture(scope="module")
@@ -30,7 +30,7 @@ def case(eik):
     lat = np.linspace(8.0, 92.0, 4)
     rec = np.array([[x, y, 0.0] for y in lat for x in lat])
     recs = [rec.copy() for _ in range(NSRC)]
-    assert NSRC * grd.npts >= 1 << 22 > grd.npts  # the whole call splits, a single source does not
+    assert NSRC * grd.npts >= 3 << 22 > grd.npts  # the whole call splits, a single source does not
     model = eik.EikModel(vel, grd)
     yield grd, vel, src, recs, model
     model.close()
